@@ -1,0 +1,132 @@
+/*
+ * ps4.h -- C ABI of the MI355X census engine (libps4.so).
+ *
+ * The reference (panjanek/pairwise-square-sums, exhaustive/s4.c) is a process
+ * CLI with no FFI; these entry points are the library cut through it: what its
+ * main() does per interval (exhaustive/s4.c:414-509) minus argv/stdio, with the
+ * hot loops (process_chunk :280-342, handle_4set :225-278, extend4 :160-207)
+ * running as HIP kernels on one gfx950 device.  The drop-in CLI `ps4`
+ * (pairwise-square-sums_amd/csrc/ps4_cli.c) is the reference's own interface
+ * rebuilt on top of this ABI.
+ *
+ * Plain C types only.  The caller owns every array it passes; text handed to a
+ * sink belongs to the library and is valid only during the callback.  A
+ * context is used from one host thread at a time.  Functions return 0 on
+ * success or a negative PS4_E_* code; nothing is printed.
+ */
+#ifndef PS4_H
+#define PS4_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes (ps4_strerror gives the text). */
+#define PS4_OK          0
+#define PS4_E_ARG      -1   /* bad argument: S_hi <= S_lo, range beyond the tables, NULL pointer   */
+#define PS4_E_CAPACITY -2   /* a reference capacity was exceeded: reps > cap_reps, divisors >
+                               cap_divs or extensions > cap_ext (s4.c fatal(), exhaustive/s4.c:56-59,
+                               176, 201, 321).  The interval is NOT certified complete.          */
+#define PS4_E_HIP      -3   /* a HIP runtime call failed                                           */
+#define PS4_E_OOM      -4   /* device or host allocation failed                                    */
+#define PS4_E_VERIFY   -5   /* PS4_VERIFY found a 4-set whose six roots do not check
+                               (exhaustive/s4.c:229-233)                                           */
+#define PS4_E_SINK     -6   /* the sink callback returned nonzero                                  */
+#define PS4_E_INTERNAL -7   /* an internal invariant failed                                        */
+
+/* Flags of ps4_config.flags: the reference's switches (exhaustive/s4.c:416-423). */
+#define PS4_NOEXT   0x1u    /* --noext: count 4-sets, do not extend (:242)                          */
+#define PS4_VERIFY  0x2u    /* --verify: re-check the six roots of every 4-set (:229-233)           */
+#define PS4_TEST6   0x4u    /* --test-sixset: exercise the 6-set alert on the first M: line (:255-260) */
+
+/* Opaque context: resident tables, device buffers, streams. */
+typedef struct ps4_ctx ps4_ctx;
+
+/* Counter vector: the stats_t fields the reference reports (exhaustive/s4.c:80-91)
+ * with the aggregation of :488-506 (sums, and maxima for the three max_*). */
+typedef struct {
+    uint64_t S_done, pairs, S_ge3, triples, sets4, sets5, sets6, multi, ext_cand, ext_sqrt;
+    uint64_t hist5[100];
+    uint64_t max_reps, max_divisors, max_extensions;
+} ps4_counters;
+
+typedef struct {
+    int device;            /* HIP device ordinal                                              */
+    uint64_t S_hi_max;     /* largest S_hi any later call may use; sizes the factor tables
+                              (exhaustive/s4.c:439-447: n < 2*isqrt(S_hi)+64)                   */
+    uint32_t flags;        /* PS4_NOEXT | PS4_VERIFY | PS4_TEST6                                */
+    int64_t dump4_limit;   /* emit the first dump4_limit 4-sets per search as "4:" lines
+                              (--dump4 N, :237-241; 0 = none)                                   */
+    uint32_t cap_reps, cap_divs, cap_ext;   /* capacity limits; 0 = the reference's 512 / 16384 / 64
+                                               (S4_MAX_REPS/DIVS/EXT, :45-54)                     */
+} ps4_config;
+
+/* Receives result text in the reference's line grammar ("5: [...] S=..",
+ * "M: [...] ext=[...] S=..", "!!!!!!!! FOUND 6-SET: [...]", "4: [...]";
+ * exhaustive/s4.c:214-220, 239, 251-253, 274).  Return 0 to continue. */
+typedef int (*ps4_sink)(void *user, const char *lines, size_t nbytes);
+
+/* Receives each 6-set line (and the PS4_TEST6 drill line) the moment it is
+ * found, before ordinary output: the durable-alert hook of exhaustive/s4.c:129-137. */
+typedef void (*ps4_alert)(void *user, const char *line);
+
+/* Open a context on cfg->device and build the resident factor tables for
+ * S < cfg->S_hi_max.  Replaces the table set-up of exhaustive/s4.c:436-447. */
+int ps4_open(const ps4_config *cfg, ps4_ctx **ctx_out);
+
+/* Census of every S in [S_lo, S_hi): the work of one whole s4 invocation
+ * (exhaustive/s4.c:449-506; per chunk process_chunk :280-342).  Result lines
+ * go to sink (may be NULL), counters to *out.  S_lo < 1 is taken as 1 (:427). */
+int ps4_search(ps4_ctx *ctx, uint64_t S_lo, uint64_t S_hi, ps4_sink sink, void *user, ps4_counters *out);
+
+/* Many independent intervals [lo[i], hi[i]) in one persistent launch with a
+ * device work-stealing queue; out[i] and sink(users[i], ...) equal what
+ * ps4_search would give for interval i alone.  Intervals may overlap.  No
+ * reference counterpart beyond calling s4 once per interval (the block
+ * controller, run_exhaustive.py, drives s4 block by block). */
+int ps4_search_batch(ps4_ctx *ctx, size_t n, const uint64_t *lo, const uint64_t *hi,
+                     ps4_sink sink, void *const *users, ps4_counters *out);
+
+/* Triple query: every d in [1, d_max] \ {a,b,c} with d+a, d+b, d+c square,
+ * ascending.  The reference has no such entry point; it is extend4
+ * (exhaustive/s4.c:160-207) applied to three known members.  *n_out receives
+ * the full count; at most cap values are written. */
+int ps4_extend_triple(ps4_ctx *ctx, const uint64_t abc[3], uint64_t d_max,
+                      uint64_t *d_out, size_t cap, size_t *n_out);
+
+/* Install the durable-alert hook.  With a hook installed, 6-set lines and the
+ * PS4_TEST6 drill line go to the hook only; without one they go to the sink. */
+int ps4_set_alert(ps4_ctx *ctx, ps4_alert alert, void *user);
+
+/* Change the "4:" line budget of later searches (--dump4 N, exhaustive/s4.c:237;
+ * the reference's budget is per thread, a driver that mirrors its threads as
+ * sub-intervals sets it per sub-interval). */
+int ps4_set_dump4(ps4_ctx *ctx, int64_t limit);
+
+/* Triple queries in one launch: abc holds 3n values, query i writes its count
+ * to n_out[i] and its first cap values (ascending) to d_out[i*cap ...]. */
+int ps4_extend_triple_batch(ps4_ctx *ctx, size_t n, const uint64_t *abc, uint64_t d_max,
+                            uint64_t *d_out, size_t cap, size_t *n_out);
+
+/* Device time of the census kernels (ms, HIP events on the context's stream)
+ * accumulated since the last reset, and the number of batches launched.  No
+ * reference counterpart (the reference reports per-thread seconds,
+ * exhaustive/s4.c:496). */
+int ps4_kernel_time(ps4_ctx *ctx, double *ms_rq, double *ms_ext, uint64_t *batches, int reset);
+
+/* Close a context and free its device memory. */
+void ps4_close(ps4_ctx *ctx);
+
+/* Text for a PS4_E_* code. */
+const char *ps4_strerror(int rc);
+
+/* Detail of the last error on ctx (e.g. which capacity), or "" . */
+const char *ps4_last_error(ps4_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PS4_H */

@@ -1,0 +1,1007 @@
+// ps4_kernels.hip -- gfx950 kernels of the census (exhaustive/s4.c:160-342 of
+// panjanek/pairwise-square-sums, redesigned for CDNA4).
+//
+// k_rq   one workgroup per superwindow of S values (persistent, work queue):
+//        walks every lattice column p once, bins the points (p,q), p<=q,
+//        p^2+q^2 in the superwindow into 16K-wide windows in a per-workgroup
+//        global scratch, then per window builds the representation buckets in
+//        LDS (count, scan, fill, sort -- process_chunk passes 1-3,
+//        exhaustive/s4.c:285-310) and enumerates the representation triples of
+//        every S with >= 3 representations, S != 2 mod 4, into 4-sets (pass 4,
+//        :311-341).  Triples are spread evenly over the threads; 4-sets go to a
+//        global queue in a count-then-write pair of sweeps (one atomic per window).
+// k_ext  one workgroup per tile of 128 queued 4-sets (persistent, work queue):
+//        per thread the pair choice (:163-171), the factorisation of
+//        D = (ra-rb)(ra+rb) by a resident smallest-prime table, the closed-form
+//        candidate count, and two half divisor lists in an LDS arena; then the
+//        divisor pairs of all 128 4-sets are flattened over the threads.  Each
+//        candidate is filtered by per-4-set quadratic-residue masks (mod 64, 63,
+//        65: the ext_sqrt rule of :192-198), by the exact e>0 / e not in the set
+//        conditions, then by residue masks mod 11, 13, 17, 19 and a size bound;
+//        the few survivors get the exact 128-bit square tests (:199-200).
+//
+// All integer work is exact: uint64, doubles only where every value is an
+// integer below 2^53, and unsigned __int128 for e = u^2 - A.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ps4_internal.h"
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+typedef int64_t i64;
+typedef unsigned __int128 u128;
+
+// ----------------------------------------------------------------------------
+// small helpers
+// ----------------------------------------------------------------------------
+
+// floor(sqrt(n)), n < 2^64: FP64 estimate, exact integer correction
+// (the closed form r^2 <= n < (r+1)^2; exhaustive/s4.c:95-100).
+__device__ __forceinline__ u64 d_isqrt64(u64 n)
+{
+    u64 r = (u64)__builtin_sqrt((double)n);
+    if (r > 0xFFFFFFFFull) r = 0xFFFFFFFFull;
+    while (r * r > n) --r;
+    while (r < 0xFFFFFFFFull && (r + 1) * (r + 1) <= n) ++r;
+    return r;
+}
+
+// n is a perfect square, n < 2^106 (exhaustive/s4.c:101-108).
+__device__ __noinline__ bool d_is_square128(u128 n)
+{
+    const double x = (double)(u64)(n >> 64) * 18446744073709551616.0 + (double)(u64)n;
+    u64 r = (u64)__builtin_sqrt(x);
+    while ((u128)r * r > n) --r;
+    while ((u128)(r + 1) * (r + 1) <= n) ++r;
+    return (u128)r * r == n;
+}
+
+__device__ __forceinline__ u32 lane_id() { return threadIdx.x & 63u; }
+
+// A value every thread of the workgroup holds (read from LDS after a barrier),
+// made provably uniform so that branches and loop exits around barriers stay
+// scalar and the compiler cannot split them across lanes.
+__device__ __forceinline__ u32 uni(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
+
+__device__ __forceinline__ u32 wave_incl_scan(u32 x)
+{
+    const u32 lane = lane_id();
+#pragma unroll
+    for (u32 d = 1; d < 64; d <<= 1) {
+        const u32 y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+// Exclusive block scan; *total gets the block sum.  Contains two barriers.
+template <int NT>
+__device__ __forceinline__ u32 block_excl_scan(u32 v, u32 *s_w, u32 *total)
+{
+    const u32 incl = wave_incl_scan(v);
+    const u32 wid = threadIdx.x >> 6;
+    if (lane_id() == 63) s_w[wid] = incl;
+    __syncthreads();
+    u32 pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+        const u32 t = s_w[w];
+        pre += ((u32)w < wid) ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + incl - v;
+}
+
+__device__ __forceinline__ u64 shfl_xor_u64(u64 v, int m)
+{
+    const u32 lo = __shfl_xor((u32)v, m, 64), hi = __shfl_xor((u32)(v >> 32), m, 64);
+    return ((u64)hi << 32) | lo;
+}
+
+template <int NT>
+__device__ __forceinline__ u64 block_sum_u64(u64 v, u64 *s_w)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_u64(v, m);
+    const u32 wid = threadIdx.x >> 6;
+    if (lane_id() == 0) s_w[wid] = v;
+    __syncthreads();
+    u64 t = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) t += s_w[w];
+    __syncthreads();
+    return t;
+}
+
+template <int NT>
+__device__ __forceinline__ u64 block_max_u64(u64 v, u64 *s_w)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) { const u64 o = shfl_xor_u64(v, m); v = o > v ? o : v; }
+    const u32 wid = threadIdx.x >> 6;
+    if (lane_id() == 0) s_w[wid] = v;
+    __syncthreads();
+    u64 t = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) t = s_w[w] > t ? s_w[w] : t;
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ u32 tri3(u32 k) { return (u32)(((u64)k * (k - 1) * (k - 2)) / 6u); }
+
+// ----------------------------------------------------------------------------
+// k_rq: representations and 4-sets
+// ----------------------------------------------------------------------------
+
+#define RQ_NT 256
+#define RQ_LOGW 14
+#define RQ_W (1u << RQ_LOGW)
+#define RQ_PV_CAP 5120
+#define RQ_QS_CAP 1536
+#define RQ_NB_MAX 512
+#define RQ_K_MAX 2000u      // largest bucket whose triples a window indexes (C(2000,3) < 2^31)
+
+struct RqArgs {
+    const ps4_item *items;
+    u32 n_items;
+    u32 bin_cap;        // points per window held in scratch
+    u32 nb_alloc;       // windows per superwindow the scratch holds (<= RQ_NB_MAX)
+    ps4_ctl *ctl;
+    u64 *ctr;           // [rid][CT_N]
+    u64 *scratch;       // per workgroup: RQ_NB_MAX * bin_cap points
+    ps4_quad *quads;
+    u64 quad_cap;
+    u32 cap_reps;
+    u32 verify;
+};
+
+// Walk the representation triples with flat indices [c0, c1) of one window
+// (lexicographic x<y<z inside each S, S in bucket order) and emit the 4-sets
+// of both classes (exhaustive/s4.c:324-339).  With WRITE false only counts.
+template <bool WRITE>
+__device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u32 *s_tp, const u32 *s_pv,
+                       u32 nq, u64 B0, ps4_quad *out, u64 obase, u32 rid, u32 verify, u32 *err)
+{
+    if (c0 >= c1) return 0;
+    u32 lo = 0, hi = nq;
+    while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (s_tp[mid] <= c0) lo = mid; else hi = mid;
+    }
+    u32 j = lo;
+    u32 tau = c0 - s_tp[j];
+    u32 sk = s_sk[j], k = sk >> 16, st = s_st[j];
+    u64 S = B0 + (sk & 0xFFFFu);
+    u32 x = 0;
+    for (;;) {
+        const u32 m = k - 1 - x, c = m * (m - 1) / 2;
+        if (tau < c) break;
+        tau -= c;
+        ++x;
+    }
+    u32 y = x + 1;
+    while (tau >= k - 1 - y) { tau -= k - 1 - y; ++y; }
+    u32 z = y + 1 + tau;
+    double Sd = (double)S;
+    u32 Sodd = (u32)(S & 1);
+    u32 n = 0;
+    for (u32 c = c0; c < c1; ++c) {
+        const u32 px = s_pv[st + x], py = s_pv[st + y], pz = s_pv[st + z];
+        const double Px = (double)px * px, Py = (double)py * py, Pz = (double)pz * pz;
+        const double sxy = Px + Py;
+        const u32 par = (px + py + pz) & 1u;
+        // class A: sums (Q_x,Q_y,Q_z), p-sum even, d = (P_x+P_y-P_z)/2 > 0
+        const bool ca = (par == 0) && (sxy > Pz);
+        // class B: sums (Q_x,Q_y,P_z), p-sum = S mod 2, 2P_z != S, d = (P_x+P_y+P_z-S)/2 > 0
+        const bool cb = (par == Sodd) && (Pz + Pz != Sd) && (sxy + Pz > Sd);
+        if (WRITE) {
+            if (ca || cb) {
+                const u32 qx = (u32)__builtin_rint(__builtin_sqrt(Sd - Px));
+                const u32 qy = (u32)__builtin_rint(__builtin_sqrt(Sd - Py));
+                const u32 qz = (u32)__builtin_rint(__builtin_sqrt(Sd - Pz));
+                if (ca) {
+                    uint4 *o = (uint4 *)(out + obase + n);
+                    o[0] = make_uint4(qx, qy, qz, pz);
+                    o[1] = make_uint4(py, px, rid, 0u);
+                    ++n;
+                }
+                if (cb) {
+                    uint4 *o = (uint4 *)(out + obase + n);
+                    o[0] = make_uint4(qx, qy, pz, qz);
+                    o[1] = make_uint4(py, px, rid, 0u);
+                    ++n;
+                }
+                if (verify) {
+                    // the six roots of both 4-sets (exhaustive/s4.c:229-233)
+                    for (int cl = 0; cl < 2; ++cl) {
+                        if (cl == 0 ? !ca : !cb) continue;
+                        const u64 rab = qx, rac = qy, rad = cl == 0 ? qz : pz, rbc = cl == 0 ? pz : qz, rbd = py, rcd = px;
+                        const u64 va = (rab * rab + rac * rac - rbc * rbc) / 2;
+                        const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
+                        const bool ok = va > vb && vb > vc && vc > vd && vd > 0 && vb + vc == rbc * rbc &&
+                                        vb + vd == rbd * rbd && vc + vd == rcd * rcd && va + vd == rad * rad &&
+                                        va + vb + vc + vd == S;
+                        if (!ok) *err |= ERR_VERIFY;
+                    }
+                }
+            }
+        } else {
+            n += (u32)ca + (u32)cb;
+        }
+        if (++z == k) {
+            if (++y == k - 1) {
+                if (++x == k - 2) {
+                    if (c + 1 < c1) {
+                        ++j;
+                        sk = s_sk[j];
+                        k = sk >> 16;
+                        st = s_st[j];
+                        S = B0 + (sk & 0xFFFFu);
+                        Sd = (double)S;
+                        Sodd = (u32)(S & 1);
+                    }
+                    x = 0;
+                }
+                y = x + 1;
+            }
+            z = y + 1;
+        }
+    }
+    return n;
+}
+
+__global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
+{
+    __shared__ u32 s_cnt[RQ_W / 2];        // u16 bucket counts, two per word; later offsets
+    __shared__ u32 s_pv[RQ_PV_CAP];        // p of every representation of a counted S
+    __shared__ u32 s_sk[RQ_QS_CAP];        // counted S: window offset | k << 16
+    __shared__ u32 s_st[RQ_QS_CAP];        // first slot in s_pv
+    __shared__ u32 s_tp[RQ_QS_CAP + 1];    // triple prefix
+    __shared__ u32 s_bin[RQ_NB_MAX];       // points per window of the superwindow
+    __shared__ u32 s_w[RQ_NT / 64];
+    __shared__ u64 s_w64[RQ_NT / 64];
+    __shared__ u32 s_item;
+    __shared__ u64 s_gbase;
+
+    const u32 tid = threadIdx.x;
+    u64 *scr = a.scratch + (u64)blockIdx.x * a.nb_alloc * a.bin_cap;
+
+    for (;;) {
+        // The barrier first: it separates this claim from the previous item's
+        // counter flush, so the two tid == 0 regions cannot be merged across the
+        // loop back-edge (which would leave the other lanes spinning on a stale item).
+        __syncthreads();
+        if (tid == 0) s_item = atomicAdd(&a.ctl->item_next, 1u);
+        __syncthreads();
+        const u32 it = uni(s_item);
+        if (it >= a.n_items) break;
+        const ps4_item item = a.items[it];
+        const u64 X0 = item.lo, X1 = item.hi, len = X1 - X0;
+        const u32 nb = (u32)((len + RQ_W - 1) >> RQ_LOGW);
+        for (u32 i = tid; i < nb; i += RQ_NT) s_bin[i] = 0;
+        __syncthreads();
+
+        // ---- every lattice point 1 <= p <= q, X0 <= p^2+q^2 < X1, binned by window.
+        // Each thread owns a run of consecutive columns p and follows the inner
+        // circle incrementally: off = p^2 + q^2 - X0 with q the least admissible root.
+        {
+            const u64 pmax = d_isqrt64((X1 - 1) >> 1);
+            const u64 per = (pmax + RQ_NT - 1) / RQ_NT;
+            u64 p = 1 + (u64)tid * per;
+            if (per > 0 && p <= pmax) {
+                const u64 pend = (p + per - 1 < pmax) ? p + per - 1 : pmax;
+                u64 q;
+                const u64 pp = p * p;
+                if (X0 <= 2 * pp) {
+                    q = p;
+                } else {
+                    const u64 t = X0 - pp;
+                    q = d_isqrt64(t);
+                    if (q * q < t) ++q;
+                }
+                u64 off = pp + q * q - X0;
+                for (;;) {
+                    u64 o = off, qq = q;
+                    while (o < len) {
+                        const u32 bin = (u32)(o >> RQ_LOGW);
+                        const u32 slot = atomicAdd(&s_bin[bin], 1u);
+                        if (slot < a.bin_cap) scr[(u64)bin * a.bin_cap + slot] = (p << 16) | (o & (RQ_W - 1));
+                        o += 2 * qq + 1;
+                        ++qq;
+                    }
+                    if (p == pend) break;
+                    off += 2 * p + 1;
+                    ++p;
+                    if (q < p) {
+                        off += 2 * p - 1;   // q was p-1: move onto the diagonal q = p
+                        q = p;
+                    } else {
+                        while (q > p && off >= 2 * q - 1) { off -= 2 * q - 1; --q; }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        u64 c_pairs = 0, c_sge3 = 0, c_trip = 0, c_sets4 = 0, c_maxk = 0;
+        u32 err = 0;
+        for (u32 b = 0; b < nb; ++b) {
+            const u64 B0 = X0 + ((u64)b << RQ_LOGW);
+            u32 npts = uni(s_bin[b]);
+            if (npts > a.bin_cap) { err |= ERR_BIN_OVF; npts = a.bin_cap; }
+            const u64 *pts = scr + (u64)b * a.bin_cap;
+            for (u32 i = tid; i < RQ_W / 2; i += RQ_NT) s_cnt[i] = 0;
+            __syncthreads();
+            // pass 1: count (exhaustive/s4.c:285-298)
+            for (u32 i = tid; i < npts; i += RQ_NT) {
+                const u32 s = (u32)(pts[i] & 0xFFFFu);
+                atomicAdd(&s_cnt[s >> 1], 1u << ((s & 1u) << 4));
+            }
+            __syncthreads();
+            // pass 2: which S count, offsets (exhaustive/s4.c:300-305, 314-321)
+            constexpr u32 SPT = RQ_W / RQ_NT;
+            const u32 s0 = tid * SPT;
+            u32 nq = 0, qr = 0, tr = 0;
+            bool big = false;
+            for (u32 w = 0; w < SPT / 2; ++w) {
+                const u32 word = s_cnt[(s0 >> 1) + w];
+#pragma unroll
+                for (u32 h = 0; h < 2; ++h) {
+                    const u32 k = (word >> (16 * h)) & 0xFFFFu;
+                    const u64 S = B0 + s0 + 2 * w + h;
+                    c_pairs += k;
+                    c_maxk = k > c_maxk ? k : c_maxk;
+                    if (k >= 3 && (S & 3) != 2) {
+                        ++nq;
+                        qr += k;
+                        if (k > a.cap_reps) err |= ERR_CAP_REPS;
+                        if (k > RQ_K_MAX) big = true;
+                        tr += tri3(k < RQ_K_MAX ? k : RQ_K_MAX);
+                    }
+                }
+            }
+            c_sge3 += nq;
+            c_trip += tr;
+            u32 tot_nq, tot_qr, tot_tr;
+            u32 b_nq = block_excl_scan<RQ_NT>(nq, s_w, &tot_nq);
+            u32 b_qr = block_excl_scan<RQ_NT>(qr, s_w, &tot_qr);
+            u32 b_tr = block_excl_scan<RQ_NT>(tr, s_w, &tot_tr);
+            // window lists beyond the LDS capacity, or a bucket too large for the
+            // 32-bit triple index: the window is not processed and the batch fails
+            tot_nq = uni(tot_nq);
+            tot_qr = uni(tot_qr);
+            tot_tr = uni(tot_tr);
+            if (uni((u32)__syncthreads_or(big)) || tot_nq > RQ_QS_CAP || tot_qr > RQ_PV_CAP) {
+                err |= ERR_LDS_OVF;
+                continue;
+            }
+            for (u32 w = 0; w < SPT / 2; ++w) {
+                const u32 word = s_cnt[(s0 >> 1) + w];
+                u32 nw = 0;
+#pragma unroll
+                for (u32 h = 0; h < 2; ++h) {
+                    const u32 k = (word >> (16 * h)) & 0xFFFFu;
+                    const u32 s = s0 + 2 * w + h;
+                    const u64 S = B0 + s;
+                    u32 o = 0xFFFFu;
+                    if (k >= 3 && (S & 3) != 2) {
+                        s_sk[b_nq] = s | (k << 16);
+                        s_st[b_nq] = b_qr;
+                        s_tp[b_nq] = b_tr;
+                        o = b_qr;
+                        ++b_nq;
+                        b_qr += k;
+                        b_tr += tri3(k);
+                    }
+                    nw |= o << (16 * h);
+                }
+                s_cnt[(s0 >> 1) + w] = nw;
+            }
+            if (tid == 0) s_tp[tot_nq] = tot_tr;
+            __syncthreads();
+            // pass 3: fill the buckets of counted S only (exhaustive/s4.c:307-310)
+            for (u32 i = tid; i < npts; i += RQ_NT) {
+                const u64 pt = pts[i];
+                const u32 s = (u32)(pt & 0xFFFFu), sh = (s & 1u) << 4;
+                if (((s_cnt[s >> 1] >> sh) & 0xFFFFu) != 0xFFFFu) {
+                    const u32 old = atomicAdd(&s_cnt[s >> 1], 1u << sh);
+                    s_pv[(old >> sh) & 0xFFFFu] = (u32)(pt >> 16);
+                }
+            }
+            __syncthreads();
+            // buckets in ascending p, which makes every 4-set come out a > b > c > d
+            for (u32 j = tid; j < tot_nq; j += RQ_NT) {
+                const u32 st = s_st[j], k = s_sk[j] >> 16;
+                for (u32 x = 1; x < k; ++x) {
+                    const u32 v = s_pv[st + x];
+                    u32 y = x;
+                    while (y > 0 && s_pv[st + y - 1] > v) { s_pv[st + y] = s_pv[st + y - 1]; --y; }
+                    s_pv[st + y] = v;
+                }
+            }
+            __syncthreads();
+            // pass 4: triples -> 4-sets, count then write
+            const u32 T = tot_tr;   // uniform
+            const u32 c0 = (u32)(((u64)tid * T) / RQ_NT), c1 = (u32)(((u64)(tid + 1) * T) / RQ_NT);
+            const u32 n4 = rq_walk<false>(c0, c1, s_sk, s_st, s_tp, s_pv, tot_nq, B0, nullptr, 0, 0, 0, &err);
+            u32 tot4;
+            const u32 b4 = block_excl_scan<RQ_NT>(n4, s_w, &tot4);
+            tot4 = uni(tot4);
+            if (tid == 0)
+                s_gbase = tot4 ? (u64)atomicAdd((unsigned long long *)&a.ctl->quad_count, (unsigned long long)tot4) : 0ull;
+            __syncthreads();
+            const u64 gb = ((u64)uni((u32)(s_gbase >> 32)) << 32) | uni((u32)s_gbase);
+            c_sets4 += n4;
+            if (gb + tot4 > a.quad_cap) {
+                err |= ERR_QUAD_OVF;
+            } else if (n4) {
+                rq_walk<true>(c0, c1, s_sk, s_st, s_tp, s_pv, tot_nq, B0, a.quads, gb + b4, item.rid, a.verify, &err);
+            }
+            __syncthreads();
+        }
+        // ---- counters of this superwindow
+        const u64 t_pairs = block_sum_u64<RQ_NT>(c_pairs, s_w64);
+        const u64 t_sge3 = block_sum_u64<RQ_NT>(c_sge3, s_w64);
+        const u64 t_trip = block_sum_u64<RQ_NT>(c_trip, s_w64);
+        const u64 t_sets4 = block_sum_u64<RQ_NT>(c_sets4, s_w64);
+        const u64 t_maxk = block_max_u64<RQ_NT>(c_maxk, s_w64);
+        if (err) atomicOr(&a.ctl->err, err);
+        if (tid == 0) {
+            unsigned long long *c = (unsigned long long *)(a.ctr + (u64)item.rid * CT_N);
+            atomicAdd(c + CT_S_DONE, (unsigned long long)len);
+            atomicAdd(c + CT_PAIRS, (unsigned long long)t_pairs);
+            atomicAdd(c + CT_S_GE3, (unsigned long long)t_sge3);
+            atomicAdd(c + CT_TRIPLES, (unsigned long long)t_trip);
+            atomicAdd(c + CT_SETS4, (unsigned long long)t_sets4);
+            atomicMax(c + CT_MAX_REPS, (unsigned long long)t_maxk);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// k_ext: extension of queued 4-sets
+// ----------------------------------------------------------------------------
+
+#define EX_NT 128
+#define EX_ARENA 3072
+#define EX_MAXF 16
+
+struct ExArgs {
+    const ps4_quad *quads;
+    u64 quad_cap;
+    ps4_ctl *ctl;
+    const u64 *ftab;        // n -> (cofactor) | (spf << 32) | (exponent << 59)
+    u64 tab_n;              // entries in ftab / ndiv
+    const uint16_t *ndiv;   // n -> d(n)
+    const ps4_qrtab *qr;
+    u64 *ctr;
+    ps4_hit *hits;
+    u32 hit_cap;
+    u32 cap_divs;
+};
+
+// x mod m for an integer-valued double |x| < 2^53; result in [0, m).
+__device__ __forceinline__ double dmod(double x, double m, double inv_m)
+{
+    const double q = __builtin_floor(x * inv_m);
+    double r = __builtin_fma(-q, m, x);
+    r = r < 0.0 ? r + m : r;
+    r = r >= m ? r - m : r;
+    return r;
+}
+
+// Residues of a value mod 4095 = 63*65 given its residue mod 262080 = 64*4095.
+__device__ __forceinline__ u32 fold4095(u32 r)
+{
+    u32 s = (r >> 12) + (r & 4095u);
+    return s >= 4095u ? s - 4095u : s;
+}
+// floor(s/63), floor(s/65) for s < 4095 and floor(s/11..19) for s < 46189,
+// by 24-bit multiply-shift (checked exhaustively on the host).
+__device__ __forceinline__ u32 mod63(u32 s) { return s - 63u * (__umul24(s, 16645u) >> 20); }
+__device__ __forceinline__ u32 mod65(u32 s) { return s - 65u * (__umul24(s, 16132u) >> 20); }
+__device__ __forceinline__ u32 mod11(u32 s) { return s - 11u * (__umul24(s, 47663u) >> 19); }
+__device__ __forceinline__ u32 mod13(u32 s) { return s - 13u * (__umul24(s, 40330u) >> 19); }
+__device__ __forceinline__ u32 mod17(u32 s) { return s - 17u * (__umul24(s, 30841u) >> 19); }
+__device__ __forceinline__ u32 mod19(u32 s) { return s - 19u * (__umul24(s, 55189u) >> 20); }
+
+#define M1 262080.0
+#define M2 46189.0
+
+
+// One surviving candidate: exact e = u^2 - A and the two square tests of
+// exhaustive/s4.c:195-200.  Rare (after all residue filters), so out of line.
+__device__ __noinline__ u32 ext_exact(double u, u64 A, u64 C1, u64 C2, const ps4_quad *quads, u64 qidx,
+                                      u32 best, ps4_ctl *ctl, ps4_hit *hits, u32 hit_cap)
+{
+    const u64 uu = (u64)u;
+    const u128 e = (u128)uu * uu - A;
+    if (!d_is_square128(e + C1) || !d_is_square128(e + C2)) return 0;
+    const u32 slot = atomicAdd(&ctl->hit_count, 1u);
+    if (slot >= hit_cap) return ERR_HIT_OVF;
+    const ps4_quad qd = quads[qidx];
+    ps4_hit h;
+    h.r[0] = qd.r_ab; h.r[1] = qd.r_ac; h.r[2] = qd.r_ad;
+    h.r[3] = qd.r_bc; h.r[4] = qd.r_bd; h.r[5] = qd.r_cd;
+    h.rid = qd.rid;
+    h.pad = best;
+    h.e_lo = (u64)e;
+    h.e_hi = (u64)(e >> 64);
+    hits[slot] = h;
+    return 0;
+}
+
+__global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
+{
+    __shared__ double s_ar[EX_ARENA];          // half divisor lists, exact doubles
+    __shared__ u32 s_fac[EX_MAXF * EX_NT];     // odd prime factors of D: p << 5 | e, [slot][thread]
+    __shared__ u32 s_pref[EX_NT + 1];          // candidate prefix of the current round
+    __shared__ u32 s_l0[EX_NT], s_l1[EX_NT];   // offL | nL<<16,  offH | nH<<16 | half<<31
+    __shared__ u64 s_m64[EX_NT], s_m63[EX_NT], s_m65[EX_NT], s_msm[EX_NT];
+    __shared__ double s_sqA[EX_NT], s_f1[EX_NT], s_f2[EX_NT], s_f3[EX_NT], s_f4[EX_NT], s_um[EX_NT];
+    __shared__ u64 s_A[EX_NT], s_C1[EX_NT], s_C2[EX_NT];
+    __shared__ u32 s_esq[EX_NT], s_best[EX_NT];
+    __shared__ u32 s_w[EX_NT / 64];
+    __shared__ u64 s_w64[EX_NT / 64];
+    __shared__ u32 s_tile, s_rid0;
+
+    const u32 tid = threadIdx.x;
+    const u64 qc = *(volatile u64 *)&a.ctl->quad_count;
+    u64 nquads = ((u64)uni((u32)(qc >> 32)) << 32) | uni((u32)qc);
+    if (nquads > a.quad_cap) nquads = a.quad_cap;
+    const u64 ntiles = (nquads + EX_NT - 1) / EX_NT;
+    const ps4_qrtab *qr = a.qr;
+
+    for (;;) {
+        __syncthreads();   // keeps this claim apart from the previous tile's tid == 0 work
+        if (tid == 0) s_tile = atomicAdd(&a.ctl->tile_next, 1u);
+        __syncthreads();
+        const u64 tile = uni(s_tile);
+        if (tile >= ntiles) break;
+        const u64 qidx = tile * EX_NT + tid;
+        const bool valid = qidx < nquads;
+        u32 err = 0, my_rid = 0;
+        u64 my_cand = 0, my_maxdiv = 0;
+        u32 nf = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0;
+        if (tid == 0) s_rid0 = a.quads[tile * EX_NT].rid;
+        s_esq[tid] = 0;
+        if (valid) {
+            const ps4_quad qd = a.quads[qidx];
+            my_rid = qd.rid;
+            const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
+            const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
+            const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
+            // Pair choice: fewest estimated divisors d(ra-rb)*d(ra+rb) of D = v_i - v_j,
+            // first minimum in the scan order of exhaustive/s4.c:163-171.  Members are
+            // a > b > c > d, so the reference's swap never fires.
+            const u64 ra_[6] = {rac, rac, rab, rab, rab, rab};
+            const u64 rb_[6] = {rbc, rad, rbc, rad, rbd, rac};
+            u32 best = 0;
+            u64 bestv = ~0ull;
+#pragma unroll
+            for (int t = 0; t < 6; ++t) {
+                const u64 e = (u64)a.ndiv[ra_[t] - rb_[t]] * a.ndiv[ra_[t] + rb_[t]];
+                if (e < bestv) { bestv = e; best = (u32)t; }
+            }
+            // A = v_i, the other two members C1, C2 in index order, and the three
+            // roots of pairs containing A (u = one of them means e is a member).
+            u64 A, C1, C2, F1, F2, F3, ra, rb;
+            switch (best) {
+            case 0: A = va; C1 = vc; C2 = vd; F1 = rab; F2 = rac; F3 = rad; ra = rac; rb = rbc; break;
+            case 1: A = vc; C1 = va; C2 = vb; F1 = rac; F2 = rbc; F3 = rcd; ra = rac; rb = rad; break;
+            case 2: A = va; C1 = vb; C2 = vd; F1 = rab; F2 = rac; F3 = rad; ra = rab; rb = rbc; break;
+            case 3: A = vb; C1 = va; C2 = vc; F1 = rab; F2 = rbc; F3 = rbd; ra = rab; rb = rad; break;
+            case 4: A = va; C1 = vb; C2 = vc; F1 = rab; F2 = rac; F3 = rad; ra = rab; rb = rbd; break;
+            default: A = vb; C1 = va; C2 = vd; F1 = rab; F2 = rbc; F3 = rbd; ra = rab; rb = rac; break;
+            }
+            s_best[tid] = best;
+            // D = m1 * m2: strip the twos, then walk the odd parts through the table
+            u64 m1 = ra - rb, m2 = ra + rb;
+            const u32 z1 = (u32)__builtin_ctzll(m1), z2 = (u32)__builtin_ctzll(m2);
+            const u32 e2 = z1 + z2;
+            m1 >>= z1;
+            m2 >>= z2;
+            u32 n1 = 0;
+            for (int pass = 0; pass < 2; ++pass) {
+                u64 m = pass ? m2 : m1;
+                for (int guard = 0; m > 1; ++guard) {
+                    if (m >= a.tab_n || guard >= 64) { err |= ERR_FACTORS; break; }   // never for a valid 4-set
+                    const u64 en = a.ftab[m];
+                    const u32 p = (u32)(en >> 32) & 0x7FFFFFFu, e = (u32)(en >> 59);
+                    m = en & 0xFFFFFFFFull;
+                    u32 sl = nf;
+                    if (pass)
+                        for (u32 i = 0; i < n1; ++i)
+                            if ((s_fac[i * EX_NT + tid] >> 5) == p) sl = i;
+                    if (sl < nf) {
+                        s_fac[sl * EX_NT + tid] += e;
+                    } else if (nf < EX_MAXF) {
+                        s_fac[nf * EX_NT + tid] = (p << 5) | e;
+                        ++nf;
+                    } else {
+                        err |= ERR_FACTORS;
+                    }
+                }
+                if (!pass) n1 = nf;
+            }
+            u64 ndodd = 1;
+            for (u32 i = 0; i < nf; ++i) ndodd *= (s_fac[i * EX_NT + tid] & 31u) + 1;
+            const u64 ndD = (u64)(e2 + 1) * ndodd;
+            my_maxdiv = ndD;
+            if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
+            // Divisor pairs f < g of D' = D (D odd, u = (f+g)/2) or D/4 (4 | D, u = f+g);
+            // none when D = 2 mod 4.  Their number is the reference's ext_cand.
+            if (e2 != 1 && ndD <= (1ull << 20)) {
+                half = (e2 == 0);
+                e2p = half ? 0u : e2 - 2;
+                const u64 ndp = (u64)(e2p + 1) * ndodd;
+                my_cand = ndp / 2;
+                if (ndp > 1) {
+                    // split the factor sequence (2 first when e2' > 0, then the odd
+                    // primes) into a prefix L and the rest H, minimising nL + nH
+                    const u32 nfa = nf + (e2p ? 1u : 0u);
+                    u64 nl = 1, bestc = 1 + ndp, bestnl = 1;
+                    for (u32 s = 1; s <= nfa; ++s) {
+                        const u32 ee = e2p ? (s == 1 ? e2p : (s_fac[(s - 2) * EX_NT + tid] & 31u))
+                                           : (s_fac[(s - 1) * EX_NT + tid] & 31u);
+                        nl *= ee + 1;
+                        const u64 c = nl + ndp / nl;
+                        if (c < bestc) { bestc = c; sL = s; bestnl = nl; }
+                    }
+                    nL = (u32)bestnl;
+                    nH = (u32)(ndp / bestnl);
+                    lsize = nL + nH;
+                    ncand = (u32)ndp;
+                }
+            }
+            // quadratic-residue masks of this 4-set: bit r of M_m set iff r^2 + (C1-A)
+            // and r^2 + (C2-A) are both squares mod m (the filters of s4.c:192-194)
+            const double d1 = (double)(i64)(C1 - A), d2 = (double)(i64)(C2 - A);
+            const u32 r1a = (u32)dmod(d1, M1, 1.0 / M1), r1b = (u32)dmod(d2, M1, 1.0 / M1);
+            const u32 r2a = (u32)dmod(d1, M2, 1.0 / M2), r2b = (u32)dmod(d2, M2, 1.0 / M2);
+            const u32 fa = fold4095(r1a), fb = fold4095(r1b);
+            s_m64[tid] = qr->t64[r1a & 63u] & qr->t64[r1b & 63u];
+            s_m63[tid] = qr->t63[mod63(fa)] & qr->t63[mod63(fb)];
+            const u32 i65a = mod65(fa), i65b = mod65(fb);
+            s_m65[tid] = qr->t65[i65a][0] & qr->t65[i65b][0];
+            const u64 m65hi = qr->t65[i65a][1] & qr->t65[i65b][1] & 1ull;
+            const u64 m11 = qr->t11[mod11(r2a)] & qr->t11[mod11(r2b)];
+            const u64 m13 = qr->t13[mod13(r2a)] & qr->t13[mod13(r2b)];
+            const u64 m17 = qr->t17[mod17(r2a)] & qr->t17[mod17(r2b)];
+            const u64 m19 = qr->t19[mod19(r2a)] & qr->t19[mod19(r2b)];
+            s_msm[tid] = m65hi | (m11 << 1) | (m13 << 12) | (m17 << 25) | (m19 << 42);
+            // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m, i.e. u is a
+            // root of a pair sum containing A, or u^2 = 2A
+            const u64 sa = d_isqrt64(A), s2a = d_isqrt64(2 * A);
+            s_sqA[tid] = (double)sa;
+            s_f1[tid] = (double)F1;
+            s_f2[tid] = (double)F2;
+            s_f3[tid] = (double)F3;
+            s_f4[tid] = (s2a * s2a == 2 * A) ? (double)s2a : -1.0;
+            // u^2 + (C - A) can be square only if u <= (|C - A| + 1) / 2
+            const u64 ad1 = C1 > A ? C1 - A : A - C1, ad2 = C2 > A ? C2 - A : A - C2;
+            s_um[tid] = (double)(((ad1 < ad2 ? ad1 : ad2) + 1) >> 1);
+            s_A[tid] = A;
+            s_C1[tid] = C1;
+            s_C2[tid] = C2;
+        }
+        __syncthreads();
+
+        // ---- rounds: divisor lists of as many 4-sets as the arena holds, then all
+        //      their candidates flattened over the workgroup
+        bool pending = ncand > 0;
+        while (uni((u32)__syncthreads_or(pending))) {
+            u32 ltot;
+            const u32 loff = block_excl_scan<EX_NT>(pending ? lsize : 0u, s_w, &ltot);
+            bool fits = pending && loff + lsize <= EX_ARENA;
+            if (pending && loff == 0 && lsize > EX_ARENA) { err |= ERR_ARENA; pending = false; }
+            const u32 mine = fits ? ncand : 0u;
+            u32 ctot;
+            const u32 cpre = block_excl_scan<EX_NT>(mine, s_w, &ctot);
+            ctot = uni(ctot);
+            s_pref[tid] = cpre;
+            if (tid == EX_NT - 1) s_pref[EX_NT] = cpre + mine;
+            if (fits) {
+                s_l0[tid] = loff | (nL << 16);
+                s_l1[tid] = (loff + nL) | (nH << 16) | (half << 31);
+                // mixed-radix lists over the factor sequence
+                const u32 nfa = nf + (e2p ? 1u : 0u);
+                double *lst = s_ar + loff;
+                u32 n = 1;
+                lst[0] = 1.0;
+                for (u32 s = 0; s < nfa; ++s) {
+                    if (s == sL) {   // start the H list
+                        lst = s_ar + loff + nL;
+                        n = 1;
+                        lst[0] = 1.0;
+                    }
+                    u32 p, e;
+                    if (e2p && s == 0) { p = 2; e = e2p; }
+                    else { const u32 w = s_fac[(s - (e2p ? 1u : 0u)) * EX_NT + tid]; p = w >> 5; e = w & 31u; }
+                    const u32 cur = n;
+                    double pk = 1.0;
+                    for (u32 ee = 0; ee < e; ++ee) {
+                        pk *= (double)p;
+                        for (u32 j = 0; j < cur; ++j) lst[n++] = lst[j] * pk;
+                    }
+                }
+                if (sL == nfa) s_ar[loff + nL] = 1.0;   // empty H part: the list {1}
+            }
+            __syncthreads();
+            const u32 g0 = (u32)(((u64)tid * ctot) / EX_NT), g1 = (u32)(((u64)(tid + 1) * ctot) / EX_NT);
+            if (g0 < g1) {
+                u32 lo = 0, hi = EX_NT;
+                while (hi - lo > 1) {
+                    const u32 mid = (lo + hi) >> 1;
+                    if (s_pref[mid] <= g0) lo = mid; else hi = mid;
+                }
+                u32 k = lo;
+                u32 offL, cnL, offH, cnH;
+                bool hf;
+                u64 m64, m63, m65, msm;
+                double sqA, f1, f2, f3, f4, um;
+#define EX_LOAD_PARAMS()                                                                  \
+    do {                                                                                  \
+        const u32 L0 = s_l0[k], L1 = s_l1[k];                                             \
+        offL = L0 & 0xFFFFu; cnL = L0 >> 16; offH = L1 & 0xFFFFu; cnH = (L1 >> 16) & 0x7FFFu; \
+        hf = (L1 >> 31) != 0;                                                             \
+        m64 = s_m64[k]; m63 = s_m63[k]; m65 = s_m65[k]; msm = s_msm[k];                   \
+        sqA = s_sqA[k]; f1 = s_f1[k]; f2 = s_f2[k]; f3 = s_f3[k]; f4 = s_f4[k]; um = s_um[k]; \
+    } while (0)
+                EX_LOAD_PARAMS();
+                const u32 loc = g0 - s_pref[k];
+                u32 jj = loc / cnL, ii = loc - jj * cnL;
+                double hj = s_ar[offH + jj], hjc = s_ar[offH + cnH - 1 - jj];
+                u32 esq = 0;
+                for (u32 g = g0; g < g1; ++g) {
+                    const double f = s_ar[offL + ii] * hj, gg = s_ar[offL + cnL - 1 - ii] * hjc;
+                    if (f < gg) {
+                        const double u = hf ? (f + gg) * 0.5 : f + gg;
+                        const u32 r1 = (u32)dmod(u, M1, 1.0 / M1);
+                        const u32 r4095 = fold4095(r1);
+                        const u32 r63 = mod63(r4095), r65 = mod65(r4095);
+                        const u64 b65 = r65 < 64u ? (m65 >> r65) : msm;
+                        const bool pass = ((m64 >> (r1 & 63u)) & (m63 >> r63) & b65 & 1ull) != 0;
+                        if (pass && u > sqA && u != f1 && u != f2 && u != f3 && u != f4) {
+                            ++esq;
+                            if (u <= um) {
+                                const u32 r2 = (u32)dmod(u, M2, 1.0 / M2);
+                                const u64 t = (msm >> (1u + mod11(r2))) & (msm >> (12u + mod13(r2))) &
+                                              (msm >> (25u + mod17(r2))) & (msm >> (42u + mod19(r2)));
+                                if (t & 1ull)
+                                    err |= ext_exact(u, s_A[k], s_C1[k], s_C2[k], a.quads, tile * EX_NT + k, s_best[k],
+                                                     a.ctl, a.hits, a.hit_cap);
+                            }
+                        }
+                    }
+                    if (++ii == cnL) {
+                        ii = 0;
+                        if (++jj == cnH) {
+                            jj = 0;
+                            if (g + 1 < g1) {
+                                atomicAdd(&s_esq[k], esq);
+                                esq = 0;
+                                do { ++k; } while (k < EX_NT - 1 && s_pref[k + 1] == s_pref[k]);
+                                EX_LOAD_PARAMS();
+                            }
+                        }
+                        hj = s_ar[offH + jj];
+                        hjc = s_ar[offH + cnH - 1 - jj];
+                    }
+                }
+#undef EX_LOAD_PARAMS
+                atomicAdd(&s_esq[k], esq);
+            }
+            __syncthreads();
+            if (fits) pending = false;
+        }
+
+        // ---- counters of this tile (per interval)
+        const u64 my_esq = valid ? (u64)s_esq[tid] : 0ull;
+        const bool uniform = uni((u32)__syncthreads_and(!valid || my_rid == s_rid0)) != 0;
+        if (uniform) {
+            const u64 t_cand = block_sum_u64<EX_NT>(my_cand, s_w64);
+            const u64 t_esq = block_sum_u64<EX_NT>(my_esq, s_w64);
+            const u64 t_md = block_max_u64<EX_NT>(my_maxdiv, s_w64);
+            if (tid == 0) {
+                unsigned long long *c = (unsigned long long *)(a.ctr + (u64)s_rid0 * CT_N);
+                atomicAdd(c + CT_EXT_CAND, (unsigned long long)t_cand);
+                atomicAdd(c + CT_EXT_SQRT, (unsigned long long)t_esq);
+                atomicMax(c + CT_MAX_DIVS, (unsigned long long)t_md);
+            }
+        } else if (valid) {
+            unsigned long long *c = (unsigned long long *)(a.ctr + (u64)my_rid * CT_N);
+            atomicAdd(c + CT_EXT_CAND, (unsigned long long)my_cand);
+            atomicAdd(c + CT_EXT_SQRT, (unsigned long long)my_esq);
+            atomicMax(c + CT_MAX_DIVS, (unsigned long long)my_maxdiv);
+        }
+        if (err) atomicOr(&a.ctl->err, err);
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------------
+// launch wrappers (called by the host library)
+// ----------------------------------------------------------------------------
+
+extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *items, u32 n_items, u32 bin_cap,
+                                    u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u64 *scratch, ps4_quad *quads,
+                                    u64 quad_cap, u32 cap_reps, u32 verify)
+{
+    RqArgs a;
+    a.items = items; a.n_items = n_items; a.bin_cap = bin_cap; a.nb_alloc = nb_alloc; a.ctl = ctl; a.ctr = ctr;
+    a.scratch = scratch;
+    a.quads = quads; a.quad_cap = quad_cap; a.cap_reps = cap_reps; a.verify = verify;
+    hipLaunchKernelGGL(k_rq, dim3(grid), dim3(RQ_NT), 0, st, a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
+                                     const u64 *ftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
+                                     ps4_hit *hits, u32 hit_cap, u32 cap_divs)
+{
+    ExArgs a;
+    a.quads = quads; a.quad_cap = quad_cap; a.ctl = ctl; a.ftab = ftab; a.tab_n = tab_n; a.ndiv = ndiv; a.qr = qr;
+    a.ctr = ctr;
+    a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs;
+    hipLaunchKernelGGL(k_ext, dim3(grid), dim3(EX_NT), 0, st, a);
+    return hipGetLastError();
+}
+
+// Reset of the control block and the counters, as a kernel (no memset nodes,
+// so the launch sequence stays capturable).
+__global__ void k_reset(ps4_ctl *ctl, u64 *ctr, u32 n_ctr)
+{
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctl->item_next = 0; ctl->tile_next = 0; ctl->err = 0; ctl->hit_count = 0;
+        ctl->quad_count = 0; ctl->pad = 0;
+    }
+    if (i < n_ctr) ctr[i] = 0;
+}
+
+extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u32 n_ctr)
+{
+    const u32 n = n_ctr > 0 ? n_ctr : 1;
+    hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, st, ctl, ctr, n_ctr);
+    return hipGetLastError();
+}
+
+extern "C" int ps4_rq_threads(void) { return RQ_NT; }
+extern "C" int ps4_ext_threads(void) { return EX_NT; }
+extern "C" int ps4_rq_window(void) { return (int)RQ_W; }
+extern "C" int ps4_rq_nb_max(void) { return RQ_NB_MAX; }
+
+extern "C" void *ps4_k_rq_ptr(void) { return (void *)&k_rq; }
+extern "C" void *ps4_k_ext_ptr(void) { return (void *)&k_ext; }
+
+// ----------------------------------------------------------------------------
+// k_triple: d with d+a, d+b, d+c square for given square triples {a,b,c}
+// ----------------------------------------------------------------------------
+// One workgroup per query.  With a > b > c, d + a = u^2 and d + b = w^2 give
+// (u-w)(u+w) = a - b = (r_ac - r_bc)(r_ac + r_bc): the divisor step of
+// extend4 (exhaustive/s4.c:172-190) with one remaining member to test.
+
+#define TR_NT 256
+#define TR_LIST 1024
+
+__global__ __launch_bounds__(TR_NT) void k_triple(const u64 *abc, u64 d_max, const u64 *ftab, u64 tab_n,
+                                                  u64 *d_out, u32 cap, u32 *count, u32 *err_out)
+{
+    __shared__ double s_L[TR_LIST];
+    __shared__ u32 s_n[4];     // nL, nH, half, ok
+    __shared__ u64 s_v[3];
+    const u32 tid = threadIdx.x;
+    const u64 qi = blockIdx.x;
+    if (tid == 0) {
+        u64 x = abc[3 * qi], y = abc[3 * qi + 1], z = abc[3 * qi + 2], t;
+        if (x < y) { t = x; x = y; y = t; }
+        if (y < z) { t = y; y = z; z = t; }
+        if (x < y) { t = x; x = y; y = t; }
+        s_v[0] = x; s_v[1] = y; s_v[2] = z;
+        const u64 rac = d_isqrt64(x + z), rbc = d_isqrt64(y + z);
+        u64 m1 = rac - rbc, m2 = rac + rbc;
+        u32 ok = (rac * rac == x + z && rbc * rbc == y + z && m2 < tab_n && x > y && y > z && z > 0) ? 1u : 0u;
+        u32 nL = 0, nH = 0, half = 0;
+        if (ok) {
+            const u32 z1 = (u32)__builtin_ctzll(m1), z2 = (u32)__builtin_ctzll(m2), e2 = z1 + z2;
+            m1 >>= z1;
+            m2 >>= z2;
+            u32 pr[32], ex[32], nf = 0;
+            if (e2 >= 2) { pr[0] = 2; ex[0] = e2 - 2; nf = (e2 > 2) ? 1u : 0u; }
+            for (int pass = 0; pass < 2 && ok; ++pass) {
+                u64 m = pass ? m2 : m1;
+                for (int guard = 0; m > 1; ++guard) {
+                    if (m >= tab_n || guard >= 64) { ok = 0; break; }
+                    const u64 en = ftab[m];
+                    const u32 p = (u32)(en >> 32) & 0x7FFFFFFu, e = (u32)(en >> 59);
+                    m = en & 0xFFFFFFFFull;
+                    u32 i = 0;
+                    while (i < nf && pr[i] != p) ++i;
+                    if (i < nf) ex[i] += e;
+                    else if (nf < 32) { pr[nf] = p; ex[nf] = e; ++nf; }
+                    else ok = 0;
+                }
+            }
+            if (e2 == 1) ok = 0;   // a - b = 2 mod 4: no divisor pair of equal parity
+            half = (e2 == 0);
+            if (ok) {
+                u64 nd = 1;
+                for (u32 i = 0; i < nf; ++i) nd *= ex[i] + 1;
+                // split into L (prefix) and H with nL + nH small
+                u64 nl = 1, bestc = 1 + nd, bestnl = 1;
+                u32 sL = 0;
+                for (u32 s = 1; s <= nf; ++s) {
+                    nl *= ex[s - 1] + 1;
+                    const u64 c = nl + nd / nl;
+                    if (c < bestc) { bestc = c; sL = s; bestnl = nl; }
+                }
+                nL = (u32)bestnl;
+                nH = (u32)(nd / bestnl);
+                if (nL + nH > TR_LIST) { ok = 0; atomicOr(err_out, ERR_ARENA); }
+                if (ok) {
+                    double *lst = s_L;
+                    u32 n = 1;
+                    lst[0] = 1.0;
+                    for (u32 s = 0; s < nf; ++s) {
+                        if (s == sL) { lst = s_L + nL; n = 1; lst[0] = 1.0; }
+                        const u32 cur = n;
+                        double pk = 1.0;
+                        for (u32 ee = 0; ee < ex[s]; ++ee) {
+                            pk *= (double)pr[s];
+                            for (u32 j = 0; j < cur; ++j) lst[n++] = lst[j] * pk;
+                        }
+                    }
+                    if (sL == nf) s_L[nL] = 1.0;
+                }
+            }
+        }
+        s_n[0] = nL; s_n[1] = nH; s_n[2] = half; s_n[3] = ok;
+    }
+    __syncthreads();
+    if (!s_n[3]) return;
+    const u32 nL = s_n[0], nH = s_n[1];
+    const bool half = s_n[2] != 0;
+    const u64 A = s_v[0], B = s_v[1], C = s_v[2];
+    const u32 total = nL * nH;
+    for (u32 t = tid; t < total; t += TR_NT) {
+        const u32 j = t / nL, i = t - j * nL;
+        const double f = s_L[i] * s_L[nL + j], g = s_L[nL - 1 - i] * s_L[nL + nH - 1 - j];
+        if (!(f < g)) continue;
+        const u64 u = (u64)(half ? (f + g) * 0.5 : f + g);
+        const u128 uu = (u128)u * u;
+        if (uu <= A) continue;
+        const u128 d = uu - A;
+        if (d > d_max || d == A || d == B || d == C) continue;
+        if (!d_is_square128(d + C)) continue;
+        const u32 slot = atomicAdd(&count[qi], 1u);
+        if (slot < cap) d_out[qi * cap + slot] = (u64)d;
+    }
+}
+
+extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u64 d_max, const u64 *ftab,
+                                        u64 tab_n, u64 *d_out, u32 cap, u32 *count, u32 *err)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_triple, dim3((u32)n), dim3(TR_NT), 0, st, abc, d_max, ftab, tab_n, d_out, cap, count, err);
+    return hipGetLastError();
+}
+
+// Fill n 32-bit words with a value (used instead of memset so launch
+// sequences stay free of memset nodes).
+__global__ void k_fill_u32(u32 *p, u32 n, u32 v)
+{
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+extern "C" hipError_t ps4_launch_fill_u32(hipStream_t st, u32 *p, u32 n, u32 v)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}

@@ -1,0 +1,758 @@
+// ps4_lib.cpp -- host side of libps4.so: the C ABI of include/ps4.h.
+//
+// A context owns one HIP device: the resident factor tables (built once, like
+// the table set-up of exhaustive/s4.c:436-447), the work-queue control block,
+// the 4-set queue, the extension buffer and one default-priority stream.  A
+// search cuts its intervals into superwindows ("items"), groups them into
+// batches sized to the 4-set queue, and per batch launches reset -> k_rq ->
+// k_ext on the stream, then reads back counters and extensions.  Extensions
+// are rare (about 1e-5 per S), so turning them into the reference's result
+// lines (M:/5:/6-set, exhaustive/s4.c:245-277) happens here on the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ps4.h"
+#include "ps4_internal.h"
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+typedef unsigned __int128 u128;
+
+extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *items, u32 n_items, u32 bin_cap,
+                                    u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u64 *scratch, ps4_quad *quads,
+                                    u64 quad_cap, u32 cap_reps, u32 verify);
+extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
+                                     const u64 *ftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
+                                     ps4_hit *hits, u32 hit_cap, u32 cap_divs);
+extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u32 n_ctr);
+extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u64 d_max, const u64 *ftab,
+                                        u64 tab_n, u64 *d_out, u32 cap, u32 *count, u32 *err);
+extern "C" hipError_t ps4_launch_fill_u32(hipStream_t st, u32 *p, u32 n, u32 v);
+extern "C" int ps4_rq_threads(void);
+extern "C" int ps4_ext_threads(void);
+extern "C" int ps4_rq_window(void);
+extern "C" int ps4_rq_nb_max(void);
+extern "C" void *ps4_k_rq_ptr(void);
+extern "C" void *ps4_k_ext_ptr(void);
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// exact host arithmetic for the result lines
+// ---------------------------------------------------------------------------
+
+u64 isqrt64(u64 n)
+{
+    u64 r = (u64)std::sqrt((double)n);
+    if (r > 0xFFFFFFFFull) r = 0xFFFFFFFFull;
+    while (r * r > n) --r;
+    while (r < 0xFFFFFFFFull && (r + 1) * (r + 1) <= n) ++r;
+    return r;
+}
+
+u64 isqrt128(u128 n)
+{
+    if ((n >> 64) == 0) return isqrt64((u64)n);
+    u64 r = (u64)sqrtl((long double)n);
+    while ((u128)r * r > n) --r;
+    while ((u128)(r + 1) * (r + 1) <= n) ++r;
+    return r;
+}
+
+bool is_square(u128 n)
+{
+    const u64 r = isqrt128(n);
+    return (u128)r * r == n;
+}
+
+std::string dec(u128 v)
+{
+    char tmp[48];
+    int n = 0;
+    do { tmp[n++] = (char)('0' + (int)(v % 10)); v /= 10; } while (v);
+    std::string s;
+    while (n) s.push_back(tmp[--n]);
+    return s;
+}
+
+// "tag [x...] S=..." with members largest first, and the verify note of
+// exhaustive/s4.c:209-220.
+std::string format_set(const char *tag, std::vector<u128> x, u64 S)
+{
+    std::sort(x.begin(), x.end(), [](u128 p, u128 q) { return p > q; });
+    bool ok = true;
+    for (size_t i = 0; i < x.size() && ok; ++i)
+        for (size_t j = i + 1; j < x.size(); ++j)
+            if (x[i] == x[j] || !is_square(x[i] + x[j])) { ok = false; break; }
+    std::string s = tag;
+    s += " [";
+    for (size_t i = 0; i < x.size(); ++i) {
+        if (i) s += ",";
+        s += dec(x[i]);
+    }
+    s += "] S=" + std::to_string((unsigned long long)S);
+    if (!ok) s += "  *** VERIFY FAILED ***";
+    s += "\n";
+    return s;
+}
+
+// Prime factors of m ascending (trial division; m < 2^33 here).
+void factor_into(u64 m, std::vector<u64> &p, std::vector<int> &e)
+{
+    for (u64 d = 2; m > 1; d += (d == 2 ? 1 : 2)) {
+        if (d * d > m) d = m;
+        if (m % d) continue;
+        int k = 0;
+        while (m % d == 0) { m /= d; ++k; }
+        size_t i = 0;
+        while (i < p.size() && p[i] != d) ++i;
+        if (i == p.size()) { p.push_back(d); e.push_back(k); }
+        else e[i] += k;
+    }
+}
+
+struct Quad6 {
+    u64 rab, rac, rad, rbc, rbd, rcd;
+    u64 a, b, c, d, S;
+};
+
+Quad6 members(const u32 *r)
+{
+    Quad6 q;
+    q.rab = r[0]; q.rac = r[1]; q.rad = r[2]; q.rbc = r[3]; q.rbd = r[4]; q.rcd = r[5];
+    q.a = (q.rab * q.rab + q.rac * q.rac - q.rbc * q.rbc) / 2;
+    q.b = q.rab * q.rab - q.a;
+    q.c = q.rac * q.rac - q.a;
+    q.d = q.rad * q.rad - q.a;
+    q.S = q.a + q.b + q.c + q.d;
+    return q;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+struct ps4_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    uint32_t flags = 0;
+    int64_t dump4_limit = 0;
+    u32 cap_reps = 512, cap_divs = 16384, cap_ext = 64;
+    u64 S_hi_max = 0, tab_n = 0;
+    u32 n_cu = 0, rq_grid = 0, ext_grid = 0;
+    u32 W = 0, nb_alloc = 0, bin_cap = 0;
+    u64 quad_cap = 0;
+    u32 hit_cap = 0, items_cap = 0, ctr_cap = 0;
+    // device
+    u64 *d_ftab = nullptr;
+    uint16_t *d_ndiv = nullptr;
+    ps4_qrtab *d_qr = nullptr;
+    ps4_ctl *d_ctl = nullptr;
+    u64 *d_ctr = nullptr;
+    ps4_item *d_items = nullptr;
+    u64 *d_scratch = nullptr;
+    ps4_quad *d_quads = nullptr;
+    ps4_hit *d_hits = nullptr;
+    // pinned host
+    ps4_ctl *h_ctl = nullptr;
+    u64 *h_ctr = nullptr;
+    ps4_item *h_items = nullptr;
+    ps4_hit *h_hits = nullptr;
+    // state
+    ps4_alert alert = nullptr;
+    void *alert_user = nullptr;
+    bool test6_fired = false;
+    std::string last_error;
+    // kernel time accumulated over the launches since the last reset (ms)
+    double ms_rq = 0.0, ms_ext = 0.0;
+    u64 n_batches = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // triple queries
+    u64 *d_tabc = nullptr, *d_tout = nullptr;
+    u32 *d_tcnt = nullptr, *d_terr = nullptr;
+    u32 tq_cap = 0, tq_out = 0;
+};
+
+#define HIP_OK(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ctx->last_error = std::string(#call) + ": " + hipGetErrorString(e_);         \
+            return PS4_E_HIP;                                                            \
+        }                                                                                \
+    } while (0)
+
+namespace {
+
+// Upper bound on 4-sets per S near S (observed 0.03 at 1e7 .. 0.31 at 1.2e13).
+double quad_density(u64 S)
+{
+    const double l2 = std::log2((double)(S < 2 ? 2 : S));
+    double d = 0.02 * l2 - 0.3;
+    return d < 0.08 ? 0.08 : (d > 0.7 ? 0.7 : d);
+}
+
+int build_tables(ps4_ctx *ctx)
+{
+    // n < 2*isqrt(S_hi)+64 covers every root sum ra+rb (exhaustive/s4.c:440)
+    const u64 n = 2 * isqrt64(ctx->S_hi_max) + 64;
+    ctx->tab_n = n;
+    std::vector<u32> spf(n, 0);
+    for (u64 i = 2; i < n; ++i) {
+        if (spf[i]) continue;
+        spf[i] = (u32)i;
+        if (i * i < n)
+            for (u64 j = i * i; j < n; j += i)
+                if (!spf[j]) spf[j] = (u32)i;
+    }
+    std::vector<u64> ftab(n, 0);
+    std::vector<uint16_t> ndiv(n, 0);
+    std::vector<u32> ex(n, 0);
+    if (n > 1) ndiv[1] = 1;
+    for (u64 i = 2; i < n; ++i) {
+        const u32 p = spf[i];
+        const u64 q = i / p;
+        u32 e;
+        u64 cof;
+        if (q > 1 && spf[q] == p) { e = ex[q] + 1; cof = ftab[q] & 0xFFFFFFFFull; }
+        else { e = 1; cof = q; }
+        ex[i] = e;
+        ftab[i] = cof | ((u64)p << 32) | ((u64)e << 59);
+        ndiv[i] = (uint16_t)((e + 1) * ndiv[cof]);
+    }
+    ps4_qrtab qr;
+    memset(&qr, 0, sizeof qr);
+    auto isq = [](u32 x, u32 m) { for (u32 r = 0; r < m; ++r) if ((r * r) % m == x) return true; return false; };
+    for (u32 a = 0; a < 64; ++a) for (u32 r = 0; r < 64; ++r) if (isq((r * r + a) % 64, 64)) qr.t64[a] |= 1ull << r;
+    for (u32 a = 0; a < 63; ++a) for (u32 r = 0; r < 63; ++r) if (isq((r * r + a) % 63, 63)) qr.t63[a] |= 1ull << r;
+    for (u32 a = 0; a < 65; ++a) for (u32 r = 0; r < 65; ++r) if (isq((r * r + a) % 65, 65)) qr.t65[a][r >> 6] |= 1ull << (r & 63);
+    for (u32 a = 0; a < 11; ++a) for (u32 r = 0; r < 11; ++r) if (isq((r * r + a) % 11, 11)) qr.t11[a] |= 1u << r;
+    for (u32 a = 0; a < 13; ++a) for (u32 r = 0; r < 13; ++r) if (isq((r * r + a) % 13, 13)) qr.t13[a] |= 1u << r;
+    for (u32 a = 0; a < 17; ++a) for (u32 r = 0; r < 17; ++r) if (isq((r * r + a) % 17, 17)) qr.t17[a] |= 1u << r;
+    for (u32 a = 0; a < 19; ++a) for (u32 r = 0; r < 19; ++r) if (isq((r * r + a) % 19, 19)) qr.t19[a] |= 1u << r;
+    HIP_OK(hipMalloc(&ctx->d_ftab, n * sizeof(u64)));
+    HIP_OK(hipMalloc(&ctx->d_ndiv, n * sizeof(uint16_t)));
+    HIP_OK(hipMalloc(&ctx->d_qr, sizeof(ps4_qrtab)));
+    HIP_OK(hipMemcpy(ctx->d_ftab, ftab.data(), n * sizeof(u64), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ctx->d_ndiv, ndiv.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ctx->d_qr, &qr, sizeof qr, hipMemcpyHostToDevice));
+    return 0;
+}
+
+u64 env_u64(const char *name, u64 dflt)
+{
+    const char *s = getenv(name);
+    return (s && *s) ? strtoull(s, nullptr, 10) : dflt;
+}
+
+int ensure_ctr(ps4_ctx *ctx, u32 n)
+{
+    if (n <= ctx->ctr_cap) return 0;
+    u32 cap = std::max<u32>(n, 64);
+    if (ctx->d_ctr) { (void)hipFree(ctx->d_ctr); ctx->d_ctr = nullptr; }
+    if (ctx->h_ctr) { (void)hipHostFree(ctx->h_ctr); ctx->h_ctr = nullptr; }
+    HIP_OK(hipMalloc(&ctx->d_ctr, (size_t)cap * CT_N * sizeof(u64)));
+    HIP_OK(hipHostMalloc(&ctx->h_ctr, (size_t)cap * CT_N * sizeof(u64)));
+    ctx->ctr_cap = cap;
+    return 0;
+}
+
+int alloc_scratch(ps4_ctx *ctx)
+{
+    if (ctx->d_scratch) { (void)hipFree(ctx->d_scratch); ctx->d_scratch = nullptr; }
+    HIP_OK(hipMalloc(&ctx->d_scratch, (size_t)ctx->rq_grid * ctx->nb_alloc * ctx->bin_cap * sizeof(u64)));
+    return 0;
+}
+
+// Superwindow length at S: a power of two near 2*sqrt(S), so walking the
+// ~0.7*sqrt(S) lattice columns once costs a few operations per S.
+u64 superwindow(const ps4_ctx *ctx, u64 S)
+{
+    const u64 target = 2 * isqrt64(S) + 1;
+    u64 sw = ctx->W;
+    const u64 swmax = (u64)ctx->W * ctx->nb_alloc;
+    while (sw < target && sw < swmax) sw <<= 1;
+    return sw;
+}
+
+struct Interval {
+    u64 lo, hi, cur;
+    u32 rid;
+};
+
+struct Group {
+    u32 rid;
+    Quad6 q;
+    u32 best;
+    std::vector<u128> ext;
+};
+
+// Order of extensions inside an M: line: the divisor index t of the reference's
+// mixed-radix divisor list of D (exhaustive/s4.c:149-156, 185-204).
+void order_extensions(Group &g)
+{
+    if (g.ext.size() < 2) return;
+    const Quad6 &q = g.q;
+    u64 A, B, ra, rb;
+    switch (g.best) {
+    case 0: A = q.a; B = q.b; ra = q.rac; rb = q.rbc; break;
+    case 1: A = q.c; B = q.d; ra = q.rac; rb = q.rad; break;
+    case 2: A = q.a; B = q.c; ra = q.rab; rb = q.rbc; break;
+    case 3: A = q.b; B = q.d; ra = q.rab; rb = q.rad; break;
+    case 4: A = q.a; B = q.d; ra = q.rab; rb = q.rbd; break;
+    default: A = q.b; B = q.c; ra = q.rab; rb = q.rac; break;
+    }
+    std::vector<u64> p;
+    std::vector<int> e;
+    factor_into(ra - rb, p, e);
+    factor_into(ra + rb, p, e);
+    std::vector<std::pair<u64, u128>> keyed;
+    for (u128 x : g.ext) {
+        const u64 u = isqrt128(x + A), w = isqrt128(x + B);
+        u64 fd = u - w, t = 0, stride = 1;
+        for (size_t i = 0; i < p.size(); ++i) {
+            u64 dig = 0;
+            while (fd % p[i] == 0) { fd /= p[i]; ++dig; }
+            t += dig * stride;
+            stride *= (u64)(e[i] + 1);
+        }
+        keyed.push_back({t, x});
+    }
+    std::sort(keyed.begin(), keyed.end(), [](const std::pair<u64, u128> &x, const std::pair<u64, u128> &y) { return x.first < y.first; });
+    for (size_t i = 0; i < keyed.size(); ++i) g.ext[i] = keyed[i].second;
+}
+
+int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink sink, void *const *users,
+            void *single_user, ps4_counters *out)
+{
+    std::vector<Interval> iv(n);
+    for (size_t i = 0; i < n; ++i) {
+        u64 lo = lo_in[i] < 1 ? 1 : lo_in[i];
+        u64 hi = hi_in[i];
+        if (hi < lo) { ctx->last_error = "S_hi < S_lo"; return PS4_E_ARG; }
+        if (hi > ctx->S_hi_max) { ctx->last_error = "S_hi beyond the tables of this context (S_hi_max)"; return PS4_E_ARG; }
+        iv[i] = {lo, hi, lo, (u32)i};
+        memset(&out[i], 0, sizeof(ps4_counters));
+    }
+    if (int rc = ensure_ctr(ctx, (u32)n)) return rc;
+    std::vector<int64_t> dump_left(n, ctx->dump4_limit);
+    const bool noext = (ctx->flags & PS4_NOEXT) != 0;
+    double budget_scale = 1.0;
+    size_t next_iv = 0;
+    for (;;) {
+        while (next_iv < n && iv[next_iv].cur >= iv[next_iv].hi) ++next_iv;
+        if (next_iv >= n) break;
+        // ---- assemble one batch of superwindows
+        std::vector<Interval> save(iv.begin() + next_iv, iv.end());
+        double quads_expected = 0.0;
+        const double quads_budget = 0.8 * (double)ctx->quad_cap * budget_scale;
+        u32 n_items = 0;
+        u32 rid_lo = iv[next_iv].rid, rid_hi = rid_lo;
+        for (size_t i = next_iv; i < n && n_items < ctx->items_cap && quads_expected < quads_budget; ++i) {
+            Interval &v = iv[i];
+            while (v.cur < v.hi && n_items < ctx->items_cap && quads_expected < quads_budget) {
+                u64 len = superwindow(ctx, v.cur);
+                if (len > v.hi - v.cur) len = v.hi - v.cur;
+                const double dq = quad_density(v.cur + len) * (double)len;
+                if (n_items > 0 && quads_expected + dq > quads_budget) {
+                    // shrink the last item to the remaining budget, in whole windows
+                    u64 room = (u64)((quads_budget - quads_expected) / quad_density(v.cur + len));
+                    room = room / ctx->W * ctx->W;
+                    if (room == 0) { quads_expected = quads_budget; break; }
+                    len = room;
+                }
+                ctx->h_items[n_items++] = {v.cur, v.cur + len, v.rid, 0};
+                quads_expected += quad_density(v.cur + len) * (double)len;
+                v.cur += len;
+                rid_hi = v.rid;
+            }
+        }
+        const u32 n_ctr = (rid_hi + 1) * CT_N;
+        HIP_OK(hipMemcpyAsync(ctx->d_items, ctx->h_items, n_items * sizeof(ps4_item), hipMemcpyHostToDevice, ctx->st));
+        HIP_OK(ps4_launch_reset(ctx->st, ctx->d_ctl, ctx->d_ctr, n_ctr));
+        HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
+        HIP_OK(ps4_launch_rq(ctx->st, ctx->rq_grid, ctx->d_items, n_items, ctx->bin_cap, ctx->nb_alloc, ctx->d_ctl,
+                             ctx->d_ctr, ctx->d_scratch, ctx->d_quads, ctx->quad_cap, ctx->cap_reps,
+                             (ctx->flags & PS4_VERIFY) ? 1u : 0u));
+        HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
+        if (!noext)
+            HIP_OK(ps4_launch_ext(ctx->st, ctx->ext_grid, ctx->d_quads, ctx->quad_cap, ctx->d_ctl, ctx->d_ftab,
+                                  ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, ctx->d_hits, ctx->hit_cap,
+                                  ctx->cap_divs));
+        HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
+        HIP_OK(hipMemcpyAsync(ctx->h_ctl, ctx->d_ctl, sizeof(ps4_ctl), hipMemcpyDeviceToHost, ctx->st));
+        HIP_OK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, (size_t)n_ctr * sizeof(u64), hipMemcpyDeviceToHost, ctx->st));
+        HIP_OK(hipStreamSynchronize(ctx->st));
+        float t_rq = 0.f, t_ext = 0.f;
+        HIP_OK(hipEventElapsedTime(&t_rq, ctx->ev[0], ctx->ev[1]));
+        HIP_OK(hipEventElapsedTime(&t_ext, ctx->ev[1], ctx->ev[2]));
+        ctx->ms_rq += t_rq;
+        ctx->ms_ext += t_ext;
+        ++ctx->n_batches;
+        const u32 err = ctx->h_ctl->err;
+        if (err & ERR_CAP_REPS) { ctx->last_error = "representation capacity exceeded"; return PS4_E_CAPACITY; }
+        if (err & ERR_CAP_DIVS) { ctx->last_error = "divisor capacity exceeded"; return PS4_E_CAPACITY; }
+        if (err & ERR_VERIFY) { ctx->last_error = "quadruple verification failed"; return PS4_E_VERIFY; }
+        if (err & (ERR_ARENA | ERR_FACTORS | ERR_LDS_OVF)) {
+            char buf[96];
+            snprintf(buf, sizeof buf, "kernel capacity flag 0x%x", err);
+            ctx->last_error = buf;
+            return PS4_E_INTERNAL;
+        }
+        if (err & (ERR_QUAD_OVF | ERR_HIT_OVF | ERR_BIN_OVF)) {
+            // never truncate: redo this batch smaller (or with larger point bins)
+            std::copy(save.begin(), save.end(), iv.begin() + next_iv);
+            if (err & ERR_BIN_OVF) {
+                if (ctx->bin_cap >= 4 * ctx->W) { ctx->last_error = "window point bins overflow"; return PS4_E_INTERNAL; }
+                ctx->bin_cap *= 2;
+                if (int rc = alloc_scratch(ctx)) return rc;
+            }
+            if (err & (ERR_QUAD_OVF | ERR_HIT_OVF)) budget_scale *= 0.5;
+            if (budget_scale < 1e-6) { ctx->last_error = "batch cannot shrink further"; return PS4_E_INTERNAL; }
+            continue;
+        }
+        // ---- counters
+        for (u32 r = rid_lo; r <= rid_hi; ++r) {
+            const u64 *c = ctx->h_ctr + (size_t)r * CT_N;
+            ps4_counters &o = out[r];
+            o.S_done += c[CT_S_DONE];
+            o.pairs += c[CT_PAIRS];
+            o.S_ge3 += c[CT_S_GE3];
+            o.triples += c[CT_TRIPLES];
+            o.sets4 += c[CT_SETS4];
+            o.ext_cand += c[CT_EXT_CAND];
+            o.ext_sqrt += c[CT_EXT_SQRT];
+            o.max_reps = std::max(o.max_reps, c[CT_MAX_REPS]);
+            o.max_divisors = std::max(o.max_divisors, c[CT_MAX_DIVS]);
+        }
+        std::vector<std::string> text(rid_hi - rid_lo + 1);
+        // ---- "4:" lines (exhaustive/s4.c:237-241), in the reference's single-thread order
+        bool want_dump = false;
+        for (u32 r = rid_lo; r <= rid_hi; ++r) want_dump |= dump_left[r] > 0;
+        if (want_dump && ctx->h_ctl->quad_count) {
+            const u64 nq = ctx->h_ctl->quad_count;
+            std::vector<ps4_quad> qs(nq);
+            HIP_OK(hipMemcpy(qs.data(), ctx->d_quads, nq * sizeof(ps4_quad), hipMemcpyDeviceToHost));
+            struct K { u64 S, px, py, pz; u32 cls; size_t i; };
+            std::vector<K> keys(nq);
+            for (u64 i = 0; i < nq; ++i) {
+                const ps4_quad &x = qs[i];
+                const u64 S = (u64)x.r_ab * x.r_ab + (u64)x.r_cd * x.r_cd;
+                const u32 cls = x.r_ad < x.r_bc ? 1u : 0u;
+                keys[i] = {S, x.r_cd, x.r_bd, cls ? x.r_ad : x.r_bc, cls, (size_t)i};
+            }
+            std::sort(keys.begin(), keys.end(), [&](const K &x, const K &y) {
+                if (qs[x.i].rid != qs[y.i].rid) return qs[x.i].rid < qs[y.i].rid;
+                if (x.S != y.S) return x.S < y.S;
+                if (x.px != y.px) return x.px < y.px;
+                if (x.py != y.py) return x.py < y.py;
+                if (x.pz != y.pz) return x.pz < y.pz;
+                return x.cls < y.cls;
+            });
+            char line[160];
+            for (const K &k : keys) {
+                const ps4_quad &x = qs[k.i];
+                if (dump_left[x.rid] <= 0) continue;
+                --dump_left[x.rid];
+                const u32 r6[6] = {x.r_ab, x.r_ac, x.r_ad, x.r_bc, x.r_bd, x.r_cd};
+                const Quad6 q = members(r6);
+                snprintf(line, sizeof line, "4: [%lld,%lld,%lld,%lld] S=%llu\n", (long long)q.a, (long long)q.b,
+                         (long long)q.c, (long long)q.d, (unsigned long long)q.S);
+                text[x.rid - rid_lo] += line;
+            }
+        }
+        // ---- extensions -> M:, 5: and 6-set lines (exhaustive/s4.c:245-277)
+        const u32 nh = ctx->h_ctl->hit_count;
+        if (nh) {
+            HIP_OK(hipMemcpy(ctx->h_hits, ctx->d_hits, (size_t)nh * sizeof(ps4_hit), hipMemcpyDeviceToHost));
+            std::vector<u32> idx(nh);
+            for (u32 i = 0; i < nh; ++i) idx[i] = i;
+            auto key = [&](const ps4_hit &h) { return std::make_tuple(h.rid, h.r[0], h.r[1], h.r[2], h.r[3], h.r[4], h.r[5]); };
+            std::sort(idx.begin(), idx.end(), [&](u32 x, u32 y) {
+                const ps4_hit &p = ctx->h_hits[x], &q = ctx->h_hits[y];
+                const u64 Sp = (u64)p.r[0] * p.r[0] + (u64)p.r[5] * p.r[5], Sq = (u64)q.r[0] * q.r[0] + (u64)q.r[5] * q.r[5];
+                if (p.rid != q.rid) return p.rid < q.rid;
+                if (Sp != Sq) return Sp < Sq;
+                return key(p) < key(q);
+            });
+            for (u32 i = 0; i < nh;) {
+                const ps4_hit &h0 = ctx->h_hits[idx[i]];
+                Group g;
+                g.rid = h0.rid;
+                g.q = members(h0.r);
+                g.best = h0.pad;
+                u32 j = i;
+                while (j < nh && key(ctx->h_hits[idx[j]]) == key(h0)) {
+                    const ps4_hit &h = ctx->h_hits[idx[j]];
+                    g.ext.push_back(((u128)h.e_hi << 64) | h.e_lo);
+                    ++j;
+                }
+                i = j;
+                ps4_counters &o = out[g.rid];
+                const u64 ne = g.ext.size();
+                if (ne > ctx->cap_ext) { ctx->last_error = "extension capacity exceeded"; return PS4_E_CAPACITY; }
+                o.max_extensions = std::max(o.max_extensions, ne);
+                order_extensions(g);
+                std::string &t = text[g.rid - rid_lo];
+                const Quad6 &q = g.q;
+                if (ne >= 2) {
+                    ++o.multi;
+                    std::string m = "M: [" + std::to_string((unsigned long long)q.a) + "," + std::to_string((unsigned long long)q.b) +
+                                    "," + std::to_string((unsigned long long)q.c) + "," + std::to_string((unsigned long long)q.d) +
+                                    "] ext=[";
+                    for (size_t k = 0; k < ne; ++k) { if (k) m += ","; m += dec(g.ext[k]); }
+                    m += "] S=" + std::to_string((unsigned long long)q.S) + "\n";
+                    t += m;
+                    if ((ctx->flags & PS4_TEST6) && !ctx->test6_fired) {
+                        ctx->test6_fired = true;
+                        const std::string l = format_set("TEST-6SET (NOT A SOLUTION, --test-sixset drill):",
+                                                         {q.a, q.b, q.c, q.d, g.ext[0], g.ext[1]}, q.S);
+                        if (ctx->alert) ctx->alert(ctx->alert_user, l.c_str()); else t += l;
+                    }
+                }
+                for (size_t k = 0; k < ne; ++k) {
+                    if (g.ext[k] <= q.a) continue;
+                    ++o.sets5;
+                    int lg = 0;
+                    for (u128 x = g.ext[k]; x > 1; x >>= 1) ++lg;
+                    ++o.hist5[lg > 99 ? 99 : lg];
+                    t += format_set("5:", {q.a, q.b, q.c, q.d, g.ext[k]}, q.S);
+                }
+                for (size_t k = 0; k < ne; ++k)
+                    for (size_t s = k + 1; s < ne; ++s)
+                        if (is_square(g.ext[k] + g.ext[s])) {
+                            ++o.sets6;
+                            const std::string l = format_set("!!!!!!!! FOUND 6-SET:", {q.a, q.b, q.c, q.d, g.ext[k], g.ext[s]}, q.S);
+                            if (ctx->alert) ctx->alert(ctx->alert_user, l.c_str()); else t += l;
+                        }
+            }
+        }
+        if (sink)
+            for (u32 r = rid_lo; r <= rid_hi; ++r) {
+                const std::string &t = text[r - rid_lo];
+                if (t.empty()) continue;
+                void *u = users ? users[r] : single_user;
+                if (sink(u, t.data(), t.size())) { ctx->last_error = "sink returned nonzero"; return PS4_E_SINK; }
+            }
+        budget_scale = std::min(1.0, budget_scale * 2.0);
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" const char *ps4_strerror(int rc)
+{
+    switch (rc) {
+    case PS4_OK: return "ok";
+    case PS4_E_ARG: return "invalid argument";
+    case PS4_E_CAPACITY: return "capacity exceeded; the interval is NOT certified complete";
+    case PS4_E_HIP: return "HIP runtime error";
+    case PS4_E_OOM: return "out of memory";
+    case PS4_E_VERIFY: return "quadruple verification failed";
+    case PS4_E_SINK: return "output sink failed";
+    case PS4_E_INTERNAL: return "internal error";
+    default: return "unknown error";
+    }
+}
+
+extern "C" const char *ps4_last_error(ps4_ctx *ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+extern "C" void ps4_close(ps4_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    void *dev[] = {ctx->d_ftab, ctx->d_ndiv, ctx->d_qr, ctx->d_ctl, ctx->d_ctr, ctx->d_items, ctx->d_scratch,
+                   ctx->d_quads, ctx->d_hits, ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
+    for (void *p : dev) if (p) (void)hipFree(p);
+    void *host[] = {ctx->h_ctl, ctx->h_ctr, ctx->h_items, ctx->h_hits};
+    for (void *p : host) if (p) (void)hipHostFree(p);
+    for (hipEvent_t e : ctx->ev) if (e) (void)hipEventDestroy(e);
+    if (ctx->st) (void)hipStreamDestroy(ctx->st);
+    delete ctx;
+}
+
+static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
+{
+    ctx->device = cfg->device;
+    ctx->flags = cfg->flags;
+    ctx->dump4_limit = cfg->dump4_limit;
+    if (cfg->cap_reps) ctx->cap_reps = cfg->cap_reps;
+    if (cfg->cap_divs) ctx->cap_divs = cfg->cap_divs;
+    if (cfg->cap_ext) ctx->cap_ext = cfg->cap_ext;
+    ctx->S_hi_max = cfg->S_hi_max;
+    // 2^50: keeps every exponent of an odd prime in D below 32 (5-bit table field)
+    // and every divisor product an exact double
+    if (ctx->S_hi_max < 2 || ctx->S_hi_max > (1ull << 50)) { ctx->last_error = "S_hi_max out of range"; return PS4_E_ARG; }
+    HIP_OK(hipSetDevice(ctx->device));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, ctx->device));
+    ctx->n_cu = (u32)prop.multiProcessorCount;
+    HIP_OK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
+    for (hipEvent_t &e : ctx->ev) HIP_OK(hipEventCreate(&e));
+    if (int rc = build_tables(ctx)) return rc;
+    int b_rq = 0, b_ext = 0;
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b_rq, ps4_k_rq_ptr(), ps4_rq_threads(), 0));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b_ext, ps4_k_ext_ptr(), ps4_ext_threads(), 0));
+    ctx->rq_grid = (u32)env_u64("PS4_RQ_GRID", (u64)ctx->n_cu * (u64)std::max(1, b_rq));
+    ctx->ext_grid = (u32)env_u64("PS4_EXT_GRID", (u64)ctx->n_cu * (u64)std::max(1, b_ext));
+    ctx->W = (u32)ps4_rq_window();
+    // superwindows up to 2*sqrt(S_hi_max), at most 128 windows
+    u32 nb = 1;
+    while ((u64)ctx->W * nb < 2 * isqrt64(ctx->S_hi_max) + 1 && nb < 128) nb <<= 1;
+    ctx->nb_alloc = std::min<u32>(nb, (u32)ps4_rq_nb_max());
+    ctx->bin_cap = ctx->W / 2 + 1024;
+    // 4-set queue: enough for the whole range when it is small, else 2^28 entries (8 GiB)
+    const double whole = quad_density(ctx->S_hi_max) * (double)ctx->S_hi_max * 1.5 + 65536.0;
+    u64 qcap = env_u64("PS4_QUAD_CAP", 1ull << 28);
+    if ((double)qcap > whole) qcap = (u64)whole;
+    size_t fr = 0, tot = 0;
+    HIP_OK(hipMemGetInfo(&fr, &tot));
+    while (qcap > (1u << 16) && (double)qcap * sizeof(ps4_quad) > 0.4 * (double)fr) qcap >>= 1;
+    ctx->quad_cap = qcap;
+    ctx->hit_cap = 1u << 20;
+    ctx->items_cap = 1u << 20;
+    HIP_OK(hipMalloc(&ctx->d_ctl, sizeof(ps4_ctl)));
+    HIP_OK(hipHostMalloc(&ctx->h_ctl, sizeof(ps4_ctl)));
+    HIP_OK(hipMalloc(&ctx->d_items, (size_t)ctx->items_cap * sizeof(ps4_item)));
+    HIP_OK(hipHostMalloc(&ctx->h_items, (size_t)ctx->items_cap * sizeof(ps4_item)));
+    HIP_OK(hipMalloc(&ctx->d_quads, (size_t)ctx->quad_cap * sizeof(ps4_quad)));
+    HIP_OK(hipMalloc(&ctx->d_hits, (size_t)ctx->hit_cap * sizeof(ps4_hit)));
+    HIP_OK(hipHostMalloc(&ctx->h_hits, (size_t)ctx->hit_cap * sizeof(ps4_hit)));
+    if (int rc = alloc_scratch(ctx)) return rc;
+    if (int rc = ensure_ctr(ctx, 64)) return rc;
+    return 0;
+}
+
+extern "C" int ps4_open(const ps4_config *cfg, ps4_ctx **ctx_out)
+{
+    if (!cfg || !ctx_out) return PS4_E_ARG;
+    *ctx_out = nullptr;
+    ps4_ctx *ctx = new (std::nothrow) ps4_ctx();
+    if (!ctx) return PS4_E_OOM;
+    const int rc = open_impl(cfg, ctx);
+    if (rc) {
+        fprintf(stderr, "ps4_open: %s\n", ctx->last_error.c_str());
+        ps4_close(ctx);
+        return rc;
+    }
+    *ctx_out = ctx;
+    return 0;
+}
+
+extern "C" int ps4_set_dump4(ps4_ctx *ctx, int64_t limit)
+{
+    if (!ctx) return PS4_E_ARG;
+    ctx->dump4_limit = limit;
+    return 0;
+}
+
+extern "C" int ps4_set_alert(ps4_ctx *ctx, ps4_alert alert, void *user)
+{
+    if (!ctx) return PS4_E_ARG;
+    ctx->alert = alert;
+    ctx->alert_user = user;
+    return 0;
+}
+
+extern "C" int ps4_search(ps4_ctx *ctx, uint64_t S_lo, uint64_t S_hi, ps4_sink sink, void *user, ps4_counters *out)
+{
+    if (!ctx || !out) return PS4_E_ARG;
+    HIP_OK(hipSetDevice(ctx->device));
+    return run_job(ctx, 1, &S_lo, &S_hi, sink, nullptr, user, out);
+}
+
+extern "C" int ps4_search_batch(ps4_ctx *ctx, size_t n, const uint64_t *lo, const uint64_t *hi, ps4_sink sink,
+                                void *const *users, ps4_counters *out)
+{
+    if (!ctx || !out || (n && (!lo || !hi))) return PS4_E_ARG;
+    if (n == 0) return 0;
+    HIP_OK(hipSetDevice(ctx->device));
+    return run_job(ctx, n, lo, hi, sink, users, nullptr, out);
+}
+
+// Kernel time (ms, HIP events on the context's stream) accumulated since the
+// last reset, and the number of batches; reset when reset != 0.
+extern "C" int ps4_kernel_time(ps4_ctx *ctx, double *ms_rq, double *ms_ext, uint64_t *batches, int reset)
+{
+    if (!ctx) return PS4_E_ARG;
+    if (ms_rq) *ms_rq = ctx->ms_rq;
+    if (ms_ext) *ms_ext = ctx->ms_ext;
+    if (batches) *batches = ctx->n_batches;
+    if (reset) { ctx->ms_rq = ctx->ms_ext = 0.0; ctx->n_batches = 0; }
+    return 0;
+}
+
+// Triple queries, n at once (one workgroup each on the device).  abc holds 3n
+// values; query i writes its count to n_out[i] and up to cap values to
+// d_out[i*cap ...], ascending.
+extern "C" int ps4_extend_triple_batch(ps4_ctx *ctx, size_t n, const uint64_t *abc, uint64_t d_max,
+                                       uint64_t *d_out, size_t cap, size_t *n_out)
+{
+    if (!ctx || !abc || !n_out || (cap && !d_out)) return PS4_E_ARG;
+    if (n == 0) return 0;
+    HIP_OK(hipSetDevice(ctx->device));
+    for (size_t i = 0; i < n; ++i) {
+        u64 v[3] = {abc[3 * i], abc[3 * i + 1], abc[3 * i + 2]};
+        std::sort(v, v + 3);
+        if (v[0] == 0 || v[0] == v[1] || v[1] == v[2] || !is_square((u128)v[0] + v[1]) || !is_square((u128)v[0] + v[2]) ||
+            !is_square((u128)v[1] + v[2])) {
+            ctx->last_error = "not a triple of distinct positive integers with square pair sums";
+            return PS4_E_ARG;
+        }
+        if (2 * isqrt64(v[2] + v[0]) + 2 >= ctx->tab_n) { ctx->last_error = "triple beyond the tables of this context"; return PS4_E_ARG; }
+    }
+    const u32 per = 1u << 16;
+    if (n > ctx->tq_cap) {
+        void *old[] = {ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
+        for (void *p : old) if (p) (void)hipFree(p);
+        const u32 c = (u32)std::max<size_t>(n, 64);
+        HIP_OK(hipMalloc(&ctx->d_tabc, (size_t)c * 3 * sizeof(u64)));
+        HIP_OK(hipMalloc(&ctx->d_tout, (size_t)c * per * sizeof(u64)));
+        HIP_OK(hipMalloc(&ctx->d_tcnt, (size_t)c * sizeof(u32)));
+        HIP_OK(hipMalloc(&ctx->d_terr, sizeof(u32)));
+        ctx->tq_cap = c;
+    }
+    HIP_OK(hipMemcpyAsync(ctx->d_tabc, abc, n * 3 * sizeof(u64), hipMemcpyHostToDevice, ctx->st));
+    HIP_OK(ps4_launch_fill_u32(ctx->st, ctx->d_tcnt, (u32)n, 0u));
+    HIP_OK(ps4_launch_fill_u32(ctx->st, ctx->d_terr, 1u, 0u));
+    HIP_OK(ps4_launch_triple(ctx->st, ctx->d_tabc, n, d_max, ctx->d_ftab, ctx->tab_n, ctx->d_tout, per, ctx->d_tcnt,
+                             ctx->d_terr));
+    std::vector<u32> cnt(n);
+    u32 terr = 0;
+    HIP_OK(hipMemcpyAsync(cnt.data(), ctx->d_tcnt, n * sizeof(u32), hipMemcpyDeviceToHost, ctx->st));
+    HIP_OK(hipMemcpyAsync(&terr, ctx->d_terr, sizeof(u32), hipMemcpyDeviceToHost, ctx->st));
+    HIP_OK(hipStreamSynchronize(ctx->st));
+    if (terr) { ctx->last_error = "divisor lists of a triple exceed the kernel's arena"; return PS4_E_INTERNAL; }
+    std::vector<u64> buf;
+    for (size_t i = 0; i < n; ++i) {
+        if (cnt[i] > per) { ctx->last_error = "too many solutions for one triple"; return PS4_E_CAPACITY; }
+        n_out[i] = cnt[i];
+        if (!cap || !cnt[i]) continue;
+        buf.resize(cnt[i]);
+        HIP_OK(hipMemcpy(buf.data(), ctx->d_tout + i * per, cnt[i] * sizeof(u64), hipMemcpyDeviceToHost));
+        std::sort(buf.begin(), buf.end());
+        for (size_t k = 0; k < buf.size() && k < cap; ++k) d_out[i * cap + k] = buf[k];
+    }
+    return 0;
+}
+
+extern "C" int ps4_extend_triple(ps4_ctx *ctx, const uint64_t abc[3], uint64_t d_max, uint64_t *d_out, size_t cap,
+                                 size_t *n_out)
+{
+    return ps4_extend_triple_batch(ctx, 1, abc, d_max, d_out, cap, n_out);
+}

@@ -1,0 +1,206 @@
+"""Python face of the MI355X census engine (ctypes over lib/libps4.so).
+
+The engine is the reference's s4 census (exhaustive/s4.c of
+panjanek/pairwise-square-sums) on one gfx950 device.  This module only marshals
+arguments: every count and every result line comes from the HIP kernels behind
+include/ps4.h.  If the shared library is missing the import of the binding
+raises; there is no CPU fallback.
+
+    from ps4 import Engine
+    with Engine(S_hi_max=10**7) as e:
+        counters, lines = e.search(1, 10**7)
+"""
+import ctypes
+import os
+import subprocess
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libps4.so")
+CLI_PATH = os.path.join(PKG_DIR, "lib", "ps4")
+REPO_DIR = os.path.dirname(PKG_DIR)
+
+NOEXT, VERIFY, TEST6 = 0x1, 0x2, 0x4
+ERRORS = {0: "PS4_OK", -1: "PS4_E_ARG", -2: "PS4_E_CAPACITY", -3: "PS4_E_HIP", -4: "PS4_E_OOM",
+          -5: "PS4_E_VERIFY", -6: "PS4_E_SINK", -7: "PS4_E_INTERNAL"}
+# TOTAL-line keys of the reference (exhaustive/s4.c:498-500), in order
+TOTAL_KEYS = ["S_done", "pairs", "S_ge3", "triples", "4-sets", "5-sets", "6-sets",
+              "multi-ext-4sets", "ext_cand", "ext_sqrt"]
+_FIELDS = ["S_done", "pairs", "S_ge3", "triples", "sets4", "sets5", "sets6", "multi", "ext_cand", "ext_sqrt"]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_uint64) for n in _FIELDS] + [("hist5", ctypes.c_uint64 * 100)] +
+                [(n, ctypes.c_uint64) for n in ("max_reps", "max_divisors", "max_extensions")])
+
+    def as_dict(self):
+        """TOTAL counters, 5-set histogram and capacity maxima, keyed like the
+        reference's stderr report."""
+        tot = {k: int(getattr(self, f)) for k, f in zip(TOTAL_KEYS, _FIELDS)}
+        hist = {str(i): int(self.hist5[i]) for i in range(100) if self.hist5[i]}
+        cap = {"reps": int(self.max_reps), "divisors": int(self.max_divisors),
+               "extensions": int(self.max_extensions)}
+        return {"counters": tot, "hist5": hist, "capacity": cap}
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("S_hi_max", ctypes.c_uint64), ("flags", ctypes.c_uint32),
+                ("dump4_limit", ctypes.c_int64), ("cap_reps", ctypes.c_uint32), ("cap_divs", ctypes.c_uint32),
+                ("cap_ext", ctypes.c_uint32)]
+
+
+SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t)
+ALERT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p)
+
+_lib = None
+
+
+class PS4Error(RuntimeError):
+    def __init__(self, rc, detail=""):
+        self.rc = rc
+        self.name = ERRORS.get(rc, str(rc))
+        super().__init__(f"{self.name}: {detail}" if detail else self.name)
+
+
+def load():
+    """Load lib/libps4.so (built by __graft_entry__.build() / `make -C pairwise-square-sums_amd`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build the HIP extension first")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    u64, u64p = ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)
+    lib.ps4_open.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(P)]
+    lib.ps4_search.argtypes = [P, u64, u64, SINK, P, ctypes.POINTER(Counters)]
+    lib.ps4_search_batch.argtypes = [P, ctypes.c_size_t, u64p, u64p, SINK, ctypes.POINTER(P),
+                                     ctypes.POINTER(Counters)]
+    lib.ps4_extend_triple.argtypes = [P, u64p, u64, u64p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    lib.ps4_extend_triple_batch.argtypes = [P, ctypes.c_size_t, u64p, u64, u64p, ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_size_t)]
+    lib.ps4_set_alert.argtypes = [P, ALERT, P]
+    lib.ps4_set_dump4.argtypes = [P, ctypes.c_int64]
+    lib.ps4_close.argtypes = [P]
+    lib.ps4_close.restype = None
+    lib.ps4_strerror.argtypes = [ctypes.c_int]
+    lib.ps4_strerror.restype = ctypes.c_char_p
+    lib.ps4_last_error.argtypes = [P]
+    lib.ps4_last_error.restype = ctypes.c_char_p
+    lib.ps4_kernel_time.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                    u64p, ctypes.c_int]
+    _lib = lib
+    return lib
+
+
+class Engine:
+    """One device context (resident tables for S < S_hi_max)."""
+
+    def __init__(self, S_hi_max, device=0, noext=False, verify=False, test6=False, dump4=0,
+                 cap_reps=0, cap_divs=0, cap_ext=0):
+        self.lib = load()
+        cfg = Config(device, int(S_hi_max), (NOEXT if noext else 0) | (VERIFY if verify else 0) |
+                     (TEST6 if test6 else 0), int(dump4), cap_reps, cap_divs, cap_ext)
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.ps4_open(ctypes.byref(cfg), ctypes.byref(self.ctx))
+        if rc:
+            raise PS4Error(rc, "ps4_open")
+        self.alerts = []
+        self._alert_cb = None
+
+    def _check(self, rc):
+        if rc:
+            raise PS4Error(rc, self.lib.ps4_last_error(self.ctx).decode())
+
+    def set_alert(self, collect=True):
+        """Route 6-set / drill lines to self.alerts instead of the result lines."""
+        if collect:
+            def cb(_user, line):
+                self.alerts.append(line.decode())
+            self._alert_cb = ALERT(cb)
+        else:
+            self._alert_cb = ALERT(0)
+        self._check(self.lib.ps4_set_alert(self.ctx, self._alert_cb, None))
+
+    def set_dump4(self, n):
+        self._check(self.lib.ps4_set_dump4(self.ctx, int(n)))
+
+    def search(self, lo, hi):
+        """Census of S in [lo, hi): (counter dict, list of result lines)."""
+        chunks = []
+
+        def sink(_user, text, n):
+            chunks.append(ctypes.string_at(text, n))
+            return 0
+        cb = SINK(sink)
+        c = Counters()
+        self._check(self.lib.ps4_search(self.ctx, int(lo), int(hi), cb, None, ctypes.byref(c)))
+        return c.as_dict(), b"".join(chunks).decode().splitlines()
+
+    def search_batch(self, los, his):
+        """Independent intervals in one persistent launch: list of (counters, lines)."""
+        n = len(los)
+        lo = (ctypes.c_uint64 * n)(*[int(x) for x in los])
+        hi = (ctypes.c_uint64 * n)(*[int(x) for x in his])
+        texts = [[] for _ in range(n)]
+        users = (ctypes.c_void_p * n)(*range(1, n + 1))
+
+        def sink(user, text, nb):
+            texts[int(user) - 1].append(ctypes.string_at(text, nb))
+            return 0
+        cb = SINK(sink)
+        cs = (Counters * n)()
+        self._check(self.lib.ps4_search_batch(self.ctx, n, lo, hi, cb, users, cs))
+        return [(cs[i].as_dict(), b"".join(texts[i]).decode().splitlines()) for i in range(n)]
+
+    def extend_triple(self, abc, d_max, cap=1 << 16):
+        arr = (ctypes.c_uint64 * 3)(*[int(x) for x in abc])
+        out = (ctypes.c_uint64 * cap)()
+        n = ctypes.c_size_t()
+        self._check(self.lib.ps4_extend_triple(self.ctx, arr, int(d_max), out, cap, ctypes.byref(n)))
+        return [int(out[i]) for i in range(min(n.value, cap))]
+
+    def extend_triple_batch(self, triples, d_max, cap=4096):
+        n = len(triples)
+        arr = (ctypes.c_uint64 * (3 * n))(*[int(x) for t in triples for x in t])
+        out = (ctypes.c_uint64 * (n * cap))()
+        cnt = (ctypes.c_size_t * n)()
+        self._check(self.lib.ps4_extend_triple_batch(self.ctx, n, arr, int(d_max), out, cap, cnt))
+        return [[int(out[i * cap + k]) for k in range(min(cnt[i], cap))] for i in range(n)]
+
+    def kernel_time(self, reset=False):
+        """(ms in k_rq, ms in k_ext, batches) accumulated on the device stream."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
+        self._check(self.lib.ps4_kernel_time(self.ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n),
+                                             1 if reset else 0))
+        return a.value, b.value, n.value
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            self.lib.ps4_close(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_cli(args, cwd=None, timeout=600):
+    """Run the drop-in CLI (lib/ps4) with s4 arguments; returns CompletedProcess."""
+    if not os.path.exists(CLI_PATH):
+        raise RuntimeError(f"{CLI_PATH} is missing: build first")
+    return subprocess.run([CLI_PATH] + [str(a) for a in args], cwd=cwd, stdout=subprocess.PIPE,
+                          stderr=subprocess.PIPE, timeout=timeout)
+
+
+def build(arch="gfx950", jobs=8):
+    """Compile lib/libps4.so and lib/ps4 for the given gfx target (always rebuilds)."""
+    subprocess.run(["make", "-B", f"-j{jobs}", "-C", PKG_DIR, f"ARCH={arch}"], check=True,
+                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT)

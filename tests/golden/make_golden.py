@@ -1,0 +1,141 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures under tests/golden/ from the reference s4 binary.
+
+The binary is oracle/_ref/s4, compiled by `make -C oracle ref` from the
+reference's single source file (exhaustive/s4.c).  This script only runs it on
+fixed ranges and records what it prints: the TOTAL counter vector
+(exhaustive/s4.c:498-500), the 5-set histogram (:501-503), the CAPACITY maxima
+(:504-506), the number of result lines, the SHA-256 of the byte-sorted lines
+and, when the output is small, the sorted lines themselves.
+
+It also records counter vectors that the reference ships as data
+(exhaustive/logs/*.err) and the known-answer lines of exhaustive/test_verify.py
+and bruteforce-scratchpad/dotnet/1mln.txt.
+
+Usage:  python3 tests/golden/make_golden.py [--big]
+        --big adds [1,1e9) and [1,1e10) (about 4 minutes on 8 cores).
+"""
+import hashlib
+import json
+import os
+import random
+import re
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+S4 = os.path.join(REPO, "oracle", "_ref", "s4")
+REF = "/root/reference"
+OUT = os.path.join(HERE, "golden.json")
+
+KEYS = ["S_done", "pairs", "S_ge3", "triples", "4-sets", "5-sets", "6-sets",
+        "multi-ext-4sets", "ext_cand", "ext_sqrt"]
+MAX_INLINE_LINES = 2000
+
+
+def parse_err(text):
+    tot = re.search(r"^TOTAL: (.*)$", text, re.M).group(1)
+    kv = {k: int(v) for k, v in re.findall(r"([\w-]+)=(\d+)", tot)}
+    counters = {k: kv[k] for k in KEYS}
+    hist = {}
+    m = re.search(r"^5-sets by log2\(largest element\)[^:]*:(.*)$", text, re.M)
+    if m:
+        for a, b in re.findall(r"(\d+):(\d+)", m.group(1)):
+            hist[a] = int(b)
+    cap = re.search(r"^CAPACITY: reps=(\d+)/\d+ divisors=(\d+)/\d+ extensions=(\d+)/\d+", text, re.M)
+    capacity = None
+    if cap:
+        capacity = {"reps": int(cap.group(1)), "divisors": int(cap.group(2)), "extensions": int(cap.group(3))}
+    return counters, hist, capacity
+
+
+def run_s4(lo, hi, threads, extra=()):
+    with tempfile.TemporaryDirectory() as tmp:
+        outp = os.path.join(tmp, "o.out")
+        cmd = [S4, str(lo), str(hi), str(threads), "20", "--out", outp] + list(extra)
+        res = subprocess.run(cmd, cwd=tmp, stdout=subprocess.PIPE, stderr=subprocess.PIPE, check=True)
+        with open(outp, "rb") as f:
+            lines = f.read().splitlines(keepends=True)
+    lines.sort()
+    counters, hist, capacity = parse_err(res.stderr.decode())
+    rec = {"lo": lo, "hi": hi, "flags": list(extra), "threads": threads, "counters": counters,
+           "hist5": hist, "capacity": capacity, "n_lines": len(lines),
+           "sorted_sha256": hashlib.sha256(b"".join(lines)).hexdigest()}
+    if len(lines) <= MAX_INLINE_LINES:
+        rec["lines"] = [ln.decode().rstrip("\n") for ln in lines]
+    return rec, lines
+
+
+def reference_logs():
+    """Counter vectors held by the reference in exhaustive/logs/*.err."""
+    logs = os.path.join(REF, "exhaustive", "logs")
+    recs = []
+    for name in sorted(os.listdir(logs)):
+        if not name.endswith(".err"):
+            continue
+        text = open(os.path.join(logs, name)).read()
+        hdr = re.search(r"s4: S in \[(\d+),\s*(\d+)\)", text)
+        counters, hist, capacity = parse_err(text)
+        recs.append({"source": "exhaustive/logs/" + name, "lo": int(hdr.group(1)), "hi": int(hdr.group(2)),
+                     "counters": counters, "hist5": hist})
+    return recs
+
+
+def main():
+    big = "--big" in sys.argv
+    if not os.path.exists(S4):
+        sys.exit("build the reference first: make -C oracle ref")
+    data = {"generator": "tests/golden/make_golden.py", "runs": [], "dump4": [], "reference_logs": reference_logs()}
+    plain = [(1, 10**5), (1, 4 * 10**6), (1, 10**7), (1, 10**8),
+             (9 * 10**11, 9 * 10**11 + 10**7),
+             (10**12, 10**12 + 2 * 10**6),
+             (1954600000000, 1954600000000 + 10**6),
+             (11762800000000, 11762800000000 + 3 * 10**5)]
+    rng = random.Random(20261203)
+    for _ in range(12):                       # scattered windows up to 1e12
+        lo = rng.randrange(1, 10**12)
+        plain.append((lo, lo + rng.choice([1, 7, 4096, 65536, 200003])))
+    for lo in (2, 3, 5, 6, 25, 50, 65, 325, 1105, 4205):   # tiny ranges around small S with reps
+        plain.append((lo, lo + 1))
+    # the S of the known lines in exhaustive/test_verify.py:13 and :31-32
+    plain += [(362141, 362142), (50578642000, 50578642001)]
+    plain.append((1, 2))
+    if big:
+        plain += [(1, 10**9), (1, 10**10)]
+    for lo, hi in plain:
+        threads = 8 if hi - lo > 10**8 else 1
+        rec, _ = run_s4(lo, hi, threads)
+        data["runs"].append(rec)
+        print(lo, hi, rec["counters"]["4-sets"], rec["n_lines"], flush=True)
+    # --noext --dump4 runs: every 4-set, as printed by exhaustive/s4.c:237-241
+    for lo, hi, inline in [(1, 10**5, True), (1, 4 * 10**7, False), (9 * 10**11, 9 * 10**11 + 2 * 10**4, True)]:
+        rec, lines = run_s4(lo, hi, 1, ["--noext", "--dump4", "9223372036854775807"])
+        if not inline:
+            rec.pop("lines", None)
+        # members of each 4-set are printed largest first; count the sets with largest element <= 1e7
+        small = 0
+        for ln in lines:
+            vals = [int(x) for x in re.match(rb"4: \[([\d,]+)\]", ln).group(1).split(b",")]
+            small += max(vals) <= 10**7
+        rec["n_max_le_1e7"] = small
+        data["dump4"].append(rec)
+        print("dump4", lo, hi, rec["n_lines"], flush=True)
+    data["known_lines"] = {
+        "exhaustive/test_verify.py:13": "5: [763442,177458,148583,28658,7442] S=362141",
+        "exhaustive/test_verify.py:31-32": "M: [28860026400,17124487200,4431625200,162503200] "
+                                           "ext=[4981255200,2553991200,1674390681] S=50578642000",
+    }
+    onemln = os.path.join(REF, "bruteforce-scratchpad", "dotnet", "1mln.txt")
+    if os.path.exists(onemln):
+        # the only 5-sets with largest element below 10^6 (a JSON-like list with trailing commas)
+        data["known_5sets_below_1e6"] = [[int(x) for x in m.split(",")]
+                                         for m in re.findall(r"\[(\d+(?:,\d+)+)\]", open(onemln).read())]
+    with open(OUT, "w") as f:
+        json.dump(data, f, indent=1)
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,327 @@
+"""Parity of the HIP census with the reference -- needs an MI355X (-m gpu).
+
+Every test calls the engine through the C ABI (ctypes over libps4.so) or the
+drop-in CLI built on it, and compares with
+  * fixtures written by the reference s4.c (tests/golden/golden.json),
+  * the counter vectors the reference ships (exhaustive/logs/*.err),
+  * the CPU restatement (oracle/) on seeded windows the fixtures do not cover.
+Bit-exact throughout: counters, 5-set histogram, capacity maxima, and the
+byte-sorted result lines.  Tests run from small to large ranges.
+"""
+import os
+import random
+import re
+import subprocess
+import sys
+
+import pytest
+
+import ora
+import ps4
+from conftest import sha_sorted
+
+pytestmark = pytest.mark.gpu
+
+S_HI_MAX = 2 * 10**13
+
+
+def say(msg):
+    sys.__stderr__.write(f"[gpu-test] {msg}\n")
+    sys.__stderr__.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    say("open engine S_hi_max=2e13")
+    e = ps4.Engine(S_hi_max=S_HI_MAX)
+    yield e
+    e.close()
+
+
+def check(rec, res, lines, what=""):
+    assert res["counters"] == rec["counters"], what
+    assert res["hist5"] == rec["hist5"], what
+    if rec.get("capacity"):
+        assert res["capacity"] == rec["capacity"], what
+    assert len(lines) == rec["n_lines"], what
+    assert sha_sorted(lines) == rec["sorted_sha256"], what
+
+
+def test_tiny_ranges(eng, golden):
+    """Single S values and tiny windows (including S = 1..7 edge cases)."""
+    for rec in golden["runs"]:
+        if rec["hi"] - rec["lo"] <= 8:
+            say(f"tiny [{rec['lo']},{rec['hi']})")
+            res, lines = eng.search(rec["lo"], rec["hi"])
+            check(rec, res, lines, (rec["lo"], rec["hi"]))
+
+
+def test_empty_range(eng):
+    res, lines = eng.search(10**6, 10**6)
+    assert res["counters"]["S_done"] == 0 and lines == []
+
+
+def test_small_fixtures(eng, golden):
+    for rec in golden["runs"]:
+        if 8 < rec["hi"] - rec["lo"] <= 10**7 and rec["lo"] < S_HI_MAX:
+            say(f"fixture [{rec['lo']},{rec['hi']})")
+            res, lines = eng.search(rec["lo"], rec["hi"])
+            check(rec, res, lines, (rec["lo"], rec["hi"]))
+            if "lines" in rec:
+                assert sorted(lines) == sorted(rec["lines"])
+
+
+def test_readme_known_answer(eng):
+    say("[1,1e7) known answer")
+    res, lines = eng.search(1, 10**7)
+    c = res["counters"]
+    assert (c["5-sets"], c["multi-ext-4sets"], c["6-sets"]) == (123, 2, 0)
+
+
+def test_reference_log_singletons(eng, golden):
+    for r in golden["reference_logs"]:
+        if r["source"].startswith("exhaustive/logs/boundary_"):
+            say(r["source"])
+            res, _ = eng.search(r["lo"], r["hi"])
+            assert res["counters"] == r["counters"], r["source"]
+
+
+def test_seeded_windows_vs_oracle(eng):
+    """Windows the fixtures do not cover, against the CPU restatement."""
+    rng = random.Random(42)
+    for i in range(10):
+        mag = rng.choice([10**6, 10**8, 10**10, 10**11, 10**12])
+        lo = rng.randrange(1, mag)
+        span = rng.choice([1, 17, 4096, 16383, 16384, 16385, 65537, 150000])
+        say(f"oracle window {i}: [{lo},{lo + span})")
+        res, lines = eng.search(lo, lo + span)
+        want, wl = ora.census(lo, lo + span)
+        assert res == want, (lo, span)
+        assert sorted(lines) == wl, (lo, span)
+
+
+def test_dump4(golden):
+    e = ps4.Engine(S_hi_max=10**12, noext=True, dump4=2**62)
+    try:
+        for rec in golden["dump4"]:
+            say(f"dump4 [{rec['lo']},{rec['hi']})")
+            res, lines = e.search(rec["lo"], rec["hi"])
+            assert res["counters"] == rec["counters"]
+            assert len(lines) == rec["n_lines"]
+            assert sha_sorted(lines) == rec["sorted_sha256"]
+            if "n_max_le_1e7" in rec:
+                small = sum(max(int(x) for x in ln.split("[")[1].split("]")[0].split(",")) <= 10**7 for ln in lines)
+                assert small == rec["n_max_le_1e7"]
+    finally:
+        e.close()
+
+
+def test_dump4_limit_takes_first_in_s_order():
+    """--dump4 N prints the first N 4-sets in the reference's single-thread order."""
+    e = ps4.Engine(S_hi_max=10**7, noext=True, dump4=50)
+    try:
+        say("dump4 limit 50")
+        _, lines = e.search(1, 10**6)
+    finally:
+        e.close()
+    _, first = ora.census(1, 10**6, noext=True, dump4=50)
+    assert len(lines) == 50
+    assert sorted(lines) == first
+
+
+def test_batch_equals_individual(eng):
+    rng = random.Random(7)
+    los, his = [], []
+    for _ in range(24):
+        lo = rng.randrange(1, 10**11)
+        span = int(10 ** rng.uniform(2, 6))
+        los.append(lo)
+        his.append(lo + span)
+    los.append(los[3]); his.append(his[3] + 1000)        # overlapping intervals
+    los.append(5); his.append(5)                          # empty
+    say("batch of 26 intervals")
+    batch = eng.search_batch(los, his)
+    for lo, hi, (res, lines) in zip(los, his, batch):
+        one, l1 = eng.search(lo, hi)
+        assert res == one, (lo, hi)
+        assert sorted(lines) == sorted(l1), (lo, hi)
+
+
+def test_partition_invariance(eng):
+    lo, hi = 7 * 10**9, 7 * 10**9 + 3 * 10**7
+    say("partition invariance")
+    whole, lw = eng.search(lo, hi)
+    cuts = [lo, lo + 123457, lo + 10**7 + 3, hi]
+    parts = [eng.search(a, b) for a, b in zip(cuts, cuts[1:])]
+    for k in whole["counters"]:
+        assert whole["counters"][k] == sum(p[0]["counters"][k] for p in parts)
+    assert sorted(lw) == sorted(sum((p[1] for p in parts), []))
+
+
+def test_triple_queries_vs_oracle(eng):
+    rng = random.Random(11)
+    _, lines = ora.census(10**9, 10**9 + 3000, noext=True, dump4=2**62)
+    _, lines2 = ora.census(10**5, 10**5 + 20000, noext=True, dump4=2**62)
+    quads = [[int(x) for x in ln.split("[")[1].split("]")[0].split(",")] for ln in lines + lines2]
+    rng.shuffle(quads)
+    triples = []
+    for v in quads[:40]:
+        drop = rng.randrange(4)
+        triples.append([x for i, x in enumerate(v) if i != drop])
+    say(f"{len(triples)} triple queries")
+    for d_max in (10**12, 10**8):
+        got = eng.extend_triple_batch(triples, d_max)
+        for abc, g in zip(triples, got):
+            assert g == ora.extend_triple(abc, d_max), abc
+    assert eng.extend_triple(triples[0], 10**12) == ora.extend_triple(triples[0], 10**12)
+    with pytest.raises(ps4.PS4Error):
+        eng.extend_triple([1, 2, 3], 100)
+
+
+def test_capacity_failures_are_fatal():
+    """Forced capacity overflows fail with PS4_E_CAPACITY, never truncate
+    (the -DS4_MAX_* preflight of the reference, as runtime caps)."""
+    for kw, lo, hi in (({"cap_divs": 64}, 1, 10**6), ({"cap_reps": 3}, 1, 10**6), ({"cap_ext": 1}, 1, 10**7)):
+        say(f"capacity {kw}")
+        e = ps4.Engine(S_hi_max=10**7, **kw)
+        try:
+            with pytest.raises(ps4.PS4Error) as ex:
+                e.search(lo, hi)
+            assert ex.value.rc == -2
+        finally:
+            e.close()
+
+
+def test_verify_and_drill(golden):
+    e = ps4.Engine(S_hi_max=10**7, verify=True, test6=True)
+    try:
+        e.set_alert(True)
+        say("verify + test6 [1,1e7)")
+        res, lines = e.search(1, 10**7)
+        rec = next(r for r in golden["runs"] if (r["lo"], r["hi"]) == (1, 10**7))
+        check(rec, res, lines)
+        assert len(e.alerts) == 1
+        assert e.alerts[0].startswith("TEST-6SET (NOT A SOLUTION, --test-sixset drill): [")
+        assert e.alerts[0].rstrip("\n").endswith("*** VERIFY FAILED ***")
+    finally:
+        e.close()
+
+
+def _run_cli(args, cwd):
+    say("cli " + " ".join(str(a) for a in args))
+    return ps4.run_cli(args, cwd=cwd, timeout=300)
+
+
+def test_cli_drop_in(tmp_path, golden):
+    """The ps4 CLI: s4 grammar, output file in append mode, stderr reports that the
+    reference's own parsers accept (exhaustive/verify.py:103-135, compact.py:107-109,
+    run_exhaustive.py:80-82)."""
+    out = tmp_path / "o.out"
+    r = _run_cli([1, 10**7, 4, 20, "--out", out, "--ckpt", tmp_path / "c.ckpt"], tmp_path)
+    assert r.returncode == 0, r.stderr.decode()
+    err = r.stderr.decode()
+    assert re.search(r"^s4: S in \[1, 10000000\), 4 threads, chunk 1048576, ext=on, checkpointing$", err, re.M)
+    rows = re.findall(r"^\s+thread\s+(\d+): S \[(\d+),(\d+)\)", err, re.M)
+    assert [tuple(map(int, x)) for x in rows] == [(i, 1 + (10**7 - 1) * i // 4, 1 + (10**7 - 1) * (i + 1) // 4)
+                                                 for i in range(4)]
+    tot = re.findall(r"^TOTAL: (.*)$", err, re.M)
+    assert len(tot) == 1
+    kv = {k: int(v) for k, v in re.findall(r"([\w-]+)=(\d+)", tot[0])}
+    rec = next(x for x in golden["runs"] if (x["lo"], x["hi"]) == (1, 10**7))
+    for k, v in rec["counters"].items():
+        assert kv[k] == v
+    assert re.search(r"^CAPACITY: reps=24/512 divisors=252/16384 extensions=2/64$", err, re.M)
+    hist = re.search(r"^5-sets by log2\(largest element\):(.*)$", err, re.M).group(1)
+    assert dict(re.findall(r"(\d+):(\d+)", hist)) == {k: str(v) for k, v in rec["hist5"].items()}
+    lines = out.read_text().splitlines()
+    assert sha_sorted(lines) == rec["sorted_sha256"]
+    ck = (tmp_path / "c.ckpt").read_text().splitlines()
+    assert ck[0] == "s4ckpt v1" and ck[-1] == "DONE"
+    # a finished checkpoint: --resume exits at once
+    r2 = _run_cli([1, 10**7, 4, 20, "--out", out, "--ckpt", tmp_path / "c.ckpt", "--resume"], tmp_path)
+    assert r2.returncode == 0 and b"nothing to do" in r2.stderr
+    # an existing checkpoint without --resume is refused (exit 1)
+    r3 = _run_cli([1, 10**7, 4, 20, "--ckpt", tmp_path / "c.ckpt"], tmp_path)
+    assert r3.returncode == 1
+
+
+def _total(err):
+    return {k: int(v) for k, v in re.findall(r"([\w-]+)=(\d+)", re.search(r"^TOTAL: (.*)$", err, re.M).group(1))}
+
+
+def test_cli_resume_equivalence(tmp_path, golden):
+    """Interrupted-then-resumed equals one run.  The checkpoint is the state a kill
+    after the first checkpoint leaves (A::sources/exhaustive/test_preflight.py:49-66):
+    region 0 of 2 done up to a chunk boundary, region 1 untouched, results of the
+    finished part already in the output file."""
+    lo, hi, T, W = 1, 10**7, 2, 1 << 20
+    half = lo + (hi - lo) // 2
+    mid = lo + 3 * W
+    part = _run_cli([lo, mid, 1, 20, "--out", tmp_path / "r.out"], tmp_path)
+    assert part.returncode == 0
+    done = _total(part.stderr.decode())
+    ck = tmp_path / "r.ckpt"
+    ck.write_text("s4ckpt v1\n%d %d %d %d\n0 %d %d %s\n1 %d %d %s\n" % (
+        lo, hi, T, W, mid, half, " ".join(str(done[k]) for k in ps4.TOTAL_KEYS), half, hi, " ".join(["0"] * 10)))
+    r = _run_cli([lo, hi, T, 20, "--out", tmp_path / "r.out", "--ckpt", ck, "--resume"], tmp_path)
+    assert r.returncode == 0, r.stderr.decode()
+    err = r.stderr.decode()
+    assert "RESUMED from checkpoint" in err
+    rec = next(x for x in golden["runs"] if (x["lo"], x["hi"]) == (1, 10**7))
+    tot = _total(err)
+    for k, v in rec["counters"].items():
+        assert tot[k] == v, k
+    assert sha_sorted((tmp_path / "r.out").read_text().splitlines()) == rec["sorted_sha256"]
+    assert ck.read_text().splitlines()[-1] == "DONE"
+
+
+def test_cli_fatal_exit_code(tmp_path):
+    """Forced capacity failure through the CLI: FATAL line and exit code 2
+    (exhaustive/s4.c:56-59; the reference forces it with -DS4_MAX_DIVS=1)."""
+    env = dict(os.environ, PS4_CAP_DIVS="1")
+    say("cli capacity failure")
+    r = subprocess.run([ps4.CLI_PATH, "1", "1000000"], cwd=tmp_path, env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, timeout=300)
+    assert r.returncode == 2
+    assert b"FATAL: divisor capacity exceeded; this interval is NOT certified complete" in r.stderr
+
+
+def test_cli_dump4(tmp_path):
+    r = _run_cli([1, 10**6, 1, 20, "--noext", "--dump4", "5"], tmp_path)
+    assert r.returncode == 0
+    assert len([ln for ln in r.stdout.decode().splitlines() if ln.startswith("4:")]) == 5
+
+
+def test_fixtures_1e8_1e9(eng, golden):
+    for span in (10**8 - 1, 10**9 - 1):
+        rec = next(r for r in golden["runs"] if (r["lo"], r["hi"]) == (1, 1 + span))
+        say(f"fixture [1,{1 + span})")
+        res, lines = eng.search(1, 1 + span)
+        check(rec, res, lines)
+
+
+def test_high_s_fixtures(eng, golden):
+    for rec in golden["runs"]:
+        if rec["hi"] - rec["lo"] > 10**5 and rec["lo"] >= 9 * 10**11:
+            say(f"fixture [{rec['lo']},{rec['hi']})")
+            res, lines = eng.search(rec["lo"], rec["hi"])
+            check(rec, res, lines, (rec["lo"], rec["hi"]))
+
+
+def test_census_1e10(eng, golden):
+    """The headline range [1,1e10): every counter and all 54477 lines."""
+    rec = next(r for r in golden["runs"] if (r["lo"], r["hi"]) == (1, 10**10))
+    say("[1,1e10)")
+    res, lines = eng.search(1, 10**10)
+    check(rec, res, lines)
+
+
+def test_census_1e11_reference_log(eng, golden):
+    """[1,1e11) against the reference's own log (exhaustive/logs/s4d_1e11.err:68-69)."""
+    r = next(x for x in golden["reference_logs"] if x["source"].endswith("s4d_1e11.err"))
+    say("[1,1e11)")
+    res, lines = eng.search(r["lo"], r["hi"])
+    assert res["counters"] == r["counters"]
+    assert res["hist5"] == r["hist5"]
+    assert sum(ln.startswith("5:") for ln in lines) == r["counters"]["5-sets"]
+    assert sum(ln.startswith("M:") for ln in lines) == r["counters"]["multi-ext-4sets"]
