@@ -45,7 +45,10 @@ enum {
     ERR_CAP_DIVS   = 1u << 5,  /* d(D) > cap_divs (reference fatal)                  */
     ERR_VERIFY     = 1u << 6,  /* a 4-set failed its root check                      */
     ERR_ARENA      = 1u << 7,  /* one 4-set's divisor lists exceed the LDS arena     */
-    ERR_FACTORS    = 1u << 8   /* more distinct primes than the kernel holds         */
+    ERR_FACTORS    = 1u << 8,  /* more distinct primes than the kernel holds         */
+    ERR_BADQUAD    = 1u << 9,  /* a queued 4-set fails its root identities           */
+    ERR_BOUNDS     = 1u << 10  /* an index derived from data left its buffer (the
+                                  access was skipped)                                */
 };
 
 /* Work-queue control block (device memory, reset before every launch pair). */

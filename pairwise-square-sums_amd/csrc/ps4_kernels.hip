@@ -95,6 +95,21 @@ __device__ __forceinline__ u32 block_excl_scan(u32 v, u32 *s_w, u32 *total)
     return pre + incl - v;
 }
 
+// Block-wide OR of a predicate, uniform result (barriers only, no library call).
+template <int NT>
+__device__ __forceinline__ bool block_any(bool p, u32 *s_w)
+{
+    u32 tot;
+    (void)block_excl_scan<NT>(p ? 1u : 0u, s_w, &tot);
+    return uni(tot) != 0;
+}
+
+template <int NT>
+__device__ __forceinline__ bool block_all(bool p, u32 *s_w)
+{
+    return !block_any<NT>(!p, s_w);
+}
+
 __device__ __forceinline__ u64 shfl_xor_u64(u64 v, int m)
 {
     const u32 lo = __shfl_xor((u32)v, m, 64), hi = __shfl_xor((u32)(v >> 32), m, 64);
@@ -138,11 +153,11 @@ __device__ __forceinline__ u32 tri3(u32 k) { return (u32)(((u64)k * (k - 1) * (k
 // ----------------------------------------------------------------------------
 
 #define RQ_NT 256
-#define RQ_LOGW 14
+#define RQ_LOGW 13
 #define RQ_W (1u << RQ_LOGW)
-#define RQ_PV_CAP 5120
-#define RQ_QS_CAP 1536
-#define RQ_NB_MAX 512
+#define RQ_PV_CAP 2560
+#define RQ_QS_CAP 1024
+#define RQ_NB_MAX 1024
 #define RQ_K_MAX 2000u      // largest bucket whose triples a window indexes (C(2000,3) < 2^31)
 
 struct RqArgs {
@@ -157,6 +172,7 @@ struct RqArgs {
     u64 quad_cap;
     u32 cap_reps;
     u32 verify;
+    u32 n_rids;         // counter rows in ctr
 };
 
 // Walk the representation triples with flat indices [c0, c1) of one window
@@ -164,7 +180,7 @@ struct RqArgs {
 // of both classes (exhaustive/s4.c:324-339).  With WRITE false only counts.
 template <bool WRITE>
 __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u32 *s_tp, const u32 *s_pv,
-                       u32 nq, u64 B0, ps4_quad *out, u64 obase, u32 rid, u32 verify, u32 *err)
+                       u32 nq, u64 B0, ps4_quad *out, u64 obase, u64 ocap, u32 rid, u32 verify, u32 *err)
 {
     if (c0 >= c1) return 0;
     u32 lo = 0, hi = nq;
@@ -204,15 +220,23 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
                 const u32 qy = (u32)__builtin_rint(__builtin_sqrt(Sd - Py));
                 const u32 qz = (u32)__builtin_rint(__builtin_sqrt(Sd - Pz));
                 if (ca) {
-                    uint4 *o = (uint4 *)(out + obase + n);
-                    o[0] = make_uint4(qx, qy, qz, pz);
-                    o[1] = make_uint4(py, px, rid, 0u);
+                    if (obase + n < ocap) {
+                        uint4 *o = (uint4 *)(out + obase + n);
+                        o[0] = make_uint4(qx, qy, qz, pz);
+                        o[1] = make_uint4(py, px, rid, 0u);
+                    } else {
+                        *err |= ERR_BOUNDS;
+                    }
                     ++n;
                 }
                 if (cb) {
-                    uint4 *o = (uint4 *)(out + obase + n);
-                    o[0] = make_uint4(qx, qy, pz, qz);
-                    o[1] = make_uint4(py, px, rid, 0u);
+                    if (obase + n < ocap) {
+                        uint4 *o = (uint4 *)(out + obase + n);
+                        o[0] = make_uint4(qx, qy, pz, qz);
+                        o[1] = make_uint4(py, px, rid, 0u);
+                    } else {
+                        *err |= ERR_BOUNDS;
+                    }
                     ++n;
                 }
                 if (verify) {
@@ -279,9 +303,15 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
         __syncthreads();
         const u32 it = uni(s_item);
         if (it >= a.n_items) break;
-        const ps4_item item = a.items[it];
+        ps4_item item = a.items[it];
+        item.rid = uni(item.rid);
         const u64 X0 = item.lo, X1 = item.hi, len = X1 - X0;
         const u32 nb = (u32)((len + RQ_W - 1) >> RQ_LOGW);
+        // a malformed item (never produced by the host) is skipped, not walked
+        if (X1 <= X0 || nb > a.nb_alloc || nb > RQ_NB_MAX || item.rid >= a.n_rids) {
+            if (tid == 0) atomicOr(&a.ctl->err, (u32)ERR_BOUNDS);
+            continue;
+        }
         for (u32 i = tid; i < nb; i += RQ_NT) s_bin[i] = 0;
         __syncthreads();
 
@@ -308,6 +338,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                     u64 o = off, qq = q;
                     while (o < len) {
                         const u32 bin = (u32)(o >> RQ_LOGW);
+                        if (bin >= nb) break;     // o < len implies bin < nb
                         const u32 slot = atomicAdd(&s_bin[bin], 1u);
                         if (slot < a.bin_cap) scr[(u64)bin * a.bin_cap + slot] = (p << 16) | (o & (RQ_W - 1));
                         o += 2 * qq + 1;
@@ -375,7 +406,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             tot_nq = uni(tot_nq);
             tot_qr = uni(tot_qr);
             tot_tr = uni(tot_tr);
-            if (uni((u32)__syncthreads_or(big)) || tot_nq > RQ_QS_CAP || tot_qr > RQ_PV_CAP) {
+            if (block_any<RQ_NT>(big, s_w) || tot_nq > RQ_QS_CAP || tot_qr > RQ_PV_CAP) {
                 err |= ERR_LDS_OVF;
                 continue;
             }
@@ -409,7 +440,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                 const u32 s = (u32)(pt & 0xFFFFu), sh = (s & 1u) << 4;
                 if (((s_cnt[s >> 1] >> sh) & 0xFFFFu) != 0xFFFFu) {
                     const u32 old = atomicAdd(&s_cnt[s >> 1], 1u << sh);
-                    s_pv[(old >> sh) & 0xFFFFu] = (u32)(pt >> 16);
+                    const u32 slot = (old >> sh) & 0xFFFFu;
+                    if (slot < tot_qr) s_pv[slot] = (u32)(pt >> 16);
                 }
             }
             __syncthreads();
@@ -427,7 +459,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             // pass 4: triples -> 4-sets, count then write
             const u32 T = tot_tr;   // uniform
             const u32 c0 = (u32)(((u64)tid * T) / RQ_NT), c1 = (u32)(((u64)(tid + 1) * T) / RQ_NT);
-            const u32 n4 = rq_walk<false>(c0, c1, s_sk, s_st, s_tp, s_pv, tot_nq, B0, nullptr, 0, 0, 0, &err);
+            const u32 n4 = rq_walk<false>(c0, c1, s_sk, s_st, s_tp, s_pv, tot_nq, B0, nullptr, 0, 0, 0, 0, &err);
             u32 tot4;
             const u32 b4 = block_excl_scan<RQ_NT>(n4, s_w, &tot4);
             tot4 = uni(tot4);
@@ -439,7 +471,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             if (gb + tot4 > a.quad_cap) {
                 err |= ERR_QUAD_OVF;
             } else if (n4) {
-                rq_walk<true>(c0, c1, s_sk, s_st, s_tp, s_pv, tot_nq, B0, a.quads, gb + b4, item.rid, a.verify, &err);
+                rq_walk<true>(c0, c1, s_sk, s_st, s_tp, s_pv, tot_nq, B0, a.quads, gb + b4, a.quad_cap, item.rid, a.verify,
+                              &err);
             }
             __syncthreads();
         }
@@ -482,6 +515,7 @@ struct ExArgs {
     ps4_hit *hits;
     u32 hit_cap;
     u32 cap_divs;
+    u32 n_rids;             // counter rows in ctr
 };
 
 // x mod m for an integer-valued double |x| < 2^53; result in [0, m).
@@ -550,7 +584,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
     __shared__ u32 s_tile, s_rid0;
 
     const u32 tid = threadIdx.x;
-    const u64 qc = *(volatile u64 *)&a.ctl->quad_count;
+    const u64 qc = a.ctl->quad_count;   // written by the previous kernel on this stream
     u64 nquads = ((u64)uni((u32)(qc >> 32)) << 32) | uni((u32)qc);
     if (nquads > a.quad_cap) nquads = a.quad_cap;
     const u64 ntiles = (nquads + EX_NT - 1) / EX_NT;
@@ -563,12 +597,29 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
         const u64 tile = uni(s_tile);
         if (tile >= ntiles) break;
         const u64 qidx = tile * EX_NT + tid;
-        const bool valid = qidx < nquads;
+        bool valid = qidx < nquads;
         u32 err = 0, my_rid = 0;
         u64 my_cand = 0, my_maxdiv = 0;
         u32 nf = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0;
-        if (tid == 0) s_rid0 = a.quads[tile * EX_NT].rid;
+        if (tid == 0) s_rid0 = a.quads[tile * EX_NT].rid;   // tile < ntiles: entry exists
         s_esq[tid] = 0;
+        if (valid) {
+            const ps4_quad qd = a.quads[qidx];
+            const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
+            const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
+            const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
+            // every root difference and sum below indexes the tables: a queued
+            // 4-set must satisfy its identities and lie inside them, else it is
+            // reported (ERR_BADQUAD) and not extended
+            const bool good = qd.rid < a.n_rids && va > vb && vb > vc && vc > vd && vd > 0 &&
+                              vb + vc == rbc * rbc && vb + vd == rbd * rbd && vc + vd == rcd * rcd &&
+                              va + vd == rad * rad && rab + rac < a.tab_n && rac > rbc && rac > rad &&
+                              rab > rbc && rab > rad && rab > rbd && rab > rac;
+            if (!good) {
+                err |= ERR_BADQUAD;
+                valid = false;
+            }
+        }
         if (valid) {
             const ps4_quad qd = a.quads[qidx];
             my_rid = qd.rid;
@@ -694,7 +745,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
         // ---- rounds: divisor lists of as many 4-sets as the arena holds, then all
         //      their candidates flattened over the workgroup
         bool pending = ncand > 0;
-        while (uni((u32)__syncthreads_or(pending))) {
+        while (block_any<EX_NT>(pending, s_w)) {
             u32 ltot;
             const u32 loff = block_excl_scan<EX_NT>(pending ? lsize : 0u, s_w, &ltot);
             bool fits = pending && loff + lsize <= EX_ARENA;
@@ -802,7 +853,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
 
         // ---- counters of this tile (per interval)
         const u64 my_esq = valid ? (u64)s_esq[tid] : 0ull;
-        const bool uniform = uni((u32)__syncthreads_and(!valid || my_rid == s_rid0)) != 0;
+        const bool uniform = block_all<EX_NT>(!valid || my_rid == s_rid0, s_w) && s_rid0 < a.n_rids;
         if (uniform) {
             const u64 t_cand = block_sum_u64<EX_NT>(my_cand, s_w64);
             const u64 t_esq = block_sum_u64<EX_NT>(my_esq, s_w64);
@@ -829,10 +880,11 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
 // ----------------------------------------------------------------------------
 
 extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *items, u32 n_items, u32 bin_cap,
-                                    u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u64 *scratch, ps4_quad *quads,
+                                    u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u32 n_rids, u64 *scratch, ps4_quad *quads,
                                     u64 quad_cap, u32 cap_reps, u32 verify)
 {
     RqArgs a;
+    a.n_rids = n_rids;
     a.items = items; a.n_items = n_items; a.bin_cap = bin_cap; a.nb_alloc = nb_alloc; a.ctl = ctl; a.ctr = ctr;
     a.scratch = scratch;
     a.quads = quads; a.quad_cap = quad_cap; a.cap_reps = cap_reps; a.verify = verify;
@@ -842,9 +894,10 @@ extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *it
 
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
                                      const u64 *ftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
-                                     ps4_hit *hits, u32 hit_cap, u32 cap_divs)
+                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs)
 {
     ExArgs a;
+    a.n_rids = n_rids;
     a.quads = quads; a.quad_cap = quad_cap; a.ctl = ctl; a.ftab = ftab; a.tab_n = tab_n; a.ndiv = ndiv; a.qr = qr;
     a.ctr = ctr;
     a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs;

@@ -26,11 +26,11 @@ typedef uint32_t u32;
 typedef unsigned __int128 u128;
 
 extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *items, u32 n_items, u32 bin_cap,
-                                    u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u64 *scratch, ps4_quad *quads,
+                                    u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u32 n_rids, u64 *scratch, ps4_quad *quads,
                                     u64 quad_cap, u32 cap_reps, u32 verify);
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
                                      const u64 *ftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
-                                     ps4_hit *hits, u32 hit_cap, u32 cap_divs);
+                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs);
 extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u32 n_ctr);
 extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u64 d_max, const u64 *ftab,
                                         u64 tab_n, u64 *d_out, u32 cap, u32 *count, u32 *err);
@@ -180,6 +180,7 @@ struct ps4_ctx {
     u64 *d_tabc = nullptr, *d_tout = nullptr;
     u32 *d_tcnt = nullptr, *d_terr = nullptr;
     u32 tq_cap = 0, tq_out = 0;
+    bool debug_sync = false;
 };
 
 #define HIP_OK(call)                                                                     \
@@ -378,17 +379,33 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         }
         const u32 n_ctr = (rid_hi + 1) * CT_N;
         HIP_OK(hipMemcpyAsync(ctx->d_items, ctx->h_items, n_items * sizeof(ps4_item), hipMemcpyHostToDevice, ctx->st));
+        // PS4_DEBUG_SYNC=1: wait after every launch and name the kernel that failed
+        auto dbg = [&](const char *what) -> int {
+            if (!ctx->debug_sync) return 0;
+            fprintf(stderr, "ps4: %s (items %u)\n", what, n_items);
+            fflush(stderr);
+            const hipError_t e = hipStreamSynchronize(ctx->st);
+            if (e != hipSuccess) {
+                ctx->last_error = std::string("after ") + what + ": " + hipGetErrorString(e);
+                return PS4_E_HIP;
+            }
+            return 0;
+        };
         HIP_OK(ps4_launch_reset(ctx->st, ctx->d_ctl, ctx->d_ctr, n_ctr));
+        if (int rc = dbg("k_reset")) return rc;
         HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
         HIP_OK(ps4_launch_rq(ctx->st, ctx->rq_grid, ctx->d_items, n_items, ctx->bin_cap, ctx->nb_alloc, ctx->d_ctl,
-                             ctx->d_ctr, ctx->d_scratch, ctx->d_quads, ctx->quad_cap, ctx->cap_reps,
+                             ctx->d_ctr, rid_hi + 1, ctx->d_scratch, ctx->d_quads, ctx->quad_cap, ctx->cap_reps,
                              (ctx->flags & PS4_VERIFY) ? 1u : 0u));
         HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
+        if (int rc = dbg("k_rq")) return rc;
         if (!noext)
             HIP_OK(ps4_launch_ext(ctx->st, ctx->ext_grid, ctx->d_quads, ctx->quad_cap, ctx->d_ctl, ctx->d_ftab,
-                                  ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, ctx->d_hits, ctx->hit_cap,
-                                  ctx->cap_divs));
+                                  ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, rid_hi + 1, ctx->d_hits,
+                                  ctx->hit_cap, ctx->cap_divs));
         HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
+        if (!noext)
+            if (int rc = dbg("k_ext")) return rc;
         HIP_OK(hipMemcpyAsync(ctx->h_ctl, ctx->d_ctl, sizeof(ps4_ctl), hipMemcpyDeviceToHost, ctx->st));
         HIP_OK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, (size_t)n_ctr * sizeof(u64), hipMemcpyDeviceToHost, ctx->st));
         HIP_OK(hipStreamSynchronize(ctx->st));
@@ -402,7 +419,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         if (err & ERR_CAP_REPS) { ctx->last_error = "representation capacity exceeded"; return PS4_E_CAPACITY; }
         if (err & ERR_CAP_DIVS) { ctx->last_error = "divisor capacity exceeded"; return PS4_E_CAPACITY; }
         if (err & ERR_VERIFY) { ctx->last_error = "quadruple verification failed"; return PS4_E_VERIFY; }
-        if (err & (ERR_ARENA | ERR_FACTORS | ERR_LDS_OVF)) {
+        if (err & (ERR_ARENA | ERR_FACTORS | ERR_LDS_OVF | ERR_BADQUAD | ERR_BOUNDS)) {
             char buf[96];
             snprintf(buf, sizeof buf, "kernel capacity flag 0x%x", err);
             ctx->last_error = buf;
@@ -611,9 +628,10 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     ctx->rq_grid = (u32)env_u64("PS4_RQ_GRID", (u64)ctx->n_cu * (u64)std::max(1, b_rq));
     ctx->ext_grid = (u32)env_u64("PS4_EXT_GRID", (u64)ctx->n_cu * (u64)std::max(1, b_ext));
     ctx->W = (u32)ps4_rq_window();
-    // superwindows up to 2*sqrt(S_hi_max), at most 128 windows
+    ctx->debug_sync = env_u64("PS4_DEBUG_SYNC", 0) != 0;
+    // superwindows up to 2*sqrt(S_hi_max), at most 256 windows
     u32 nb = 1;
-    while ((u64)ctx->W * nb < 2 * isqrt64(ctx->S_hi_max) + 1 && nb < 128) nb <<= 1;
+    while ((u64)ctx->W * nb < 2 * isqrt64(ctx->S_hi_max) + 1 && nb < 256) nb <<= 1;
     ctx->nb_alloc = std::min<u32>(nb, (u32)ps4_rq_nb_max());
     ctx->bin_cap = ctx->W / 2 + 1024;
     // 4-set queue: enough for the whole range when it is small, else 2^28 entries (8 GiB)

@@ -1,0 +1,108 @@
+// emu_main.cpp -- runs the census library with its kernels emulated on the CPU
+// (hip_host_shim.h) and compares every result with the CPU restatement.
+// Test infrastructure only.
+//
+//   ps4_emu [--noext] [--dump4 N] [--smax S] lo hi [lo hi ...]
+//   ps4_emu --triple a b c dmax
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cpu_ref.h"
+#include "ps4.h"
+
+static int sink(void *user, const char *lines, size_t n)
+{
+    ((std::string *)user)->append(lines, n);
+    return 0;
+}
+static void ora_line(void *user, const char *line) { ((std::string *)user)->append(line); }
+
+static std::vector<std::string> split_sorted(const std::string &s)
+{
+    std::vector<std::string> v;
+    size_t i = 0;
+    while (i < s.size()) {
+        size_t j = s.find('\n', i);
+        if (j == std::string::npos) j = s.size();
+        v.push_back(s.substr(i, j - i));
+        i = j + 1;
+    }
+    std::sort(v.begin(), v.end());
+    return v;
+}
+
+int main(int argc, char **argv)
+{
+    unsigned flags = 0;
+    long long dump4 = 0;
+    unsigned long long smax = 0;
+    std::vector<unsigned long long> nums;
+    bool triple = false;
+    for (int i = 1; i < argc; ++i) {
+        if (!strcmp(argv[i], "--noext")) flags |= PS4_NOEXT;
+        else if (!strcmp(argv[i], "--dump4")) dump4 = atoll(argv[++i]);
+        else if (!strcmp(argv[i], "--smax")) smax = strtoull(argv[++i], 0, 10);
+        else if (!strcmp(argv[i], "--triple")) triple = true;
+        else nums.push_back(strtoull(argv[i], 0, 10));
+    }
+    ps4_config cfg;
+    memset(&cfg, 0, sizeof cfg);
+    unsigned long long top = 0;
+    for (auto x : nums) top = std::max(top, x);
+    cfg.S_hi_max = smax ? smax : std::max(top + 1, 100ull);
+    cfg.flags = flags;
+    cfg.dump4_limit = dump4;
+    ps4_ctx *ctx = nullptr;
+    int rc = ps4_open(&cfg, &ctx);
+    if (rc) { printf("open failed %d\n", rc); return 2; }
+    int bad = 0;
+    if (triple) {
+        uint64_t abc[3] = {nums[0], nums[1], nums[2]};
+        uint64_t out[4096], want[4096];
+        size_t n = 0;
+        rc = ps4_extend_triple(ctx, abc, nums[3], out, 4096, &n);
+        int64_t m = ora_extend_triple(abc, nums[3], want, 4096);
+        bool ok = rc == 0 && (int64_t)n == m && std::equal(out, out + n, want);
+        printf("triple %s n=%zu want=%lld rc=%d\n", ok ? "OK" : "MISMATCH", n, (long long)m, rc);
+        ps4_close(ctx);
+        return ok ? 0 : 1;
+    }
+    for (size_t i = 0; i + 1 < nums.size(); i += 2) {
+        const unsigned long long lo = nums[i], hi = nums[i + 1];
+        std::string got_text, want_text;
+        ps4_counters c;
+        rc = ps4_search(ctx, lo, hi, sink, &got_text, &c);
+        if (rc) { printf("[%llu,%llu) search rc=%d %s\n", lo, hi, rc, ps4_last_error(ctx)); bad = 1; continue; }
+        ora_counters w;
+        int orc = ora_census(lo, hi, (flags & PS4_NOEXT) ? ORA_NOEXT : 0u, dump4, ora_line, &want_text, &w);
+        if (orc) { printf("oracle rc=%d\n", orc); bad = 1; continue; }
+        const uint64_t g[] = {c.S_done, c.pairs, c.S_ge3, c.triples, c.sets4, c.sets5, c.sets6, c.multi, c.ext_cand,
+                              c.ext_sqrt, c.max_reps, c.max_divisors, c.max_extensions};
+        const uint64_t e[] = {w.S_done, w.pairs, w.S_ge3, w.triples, w.sets4, w.sets5, w.sets6, w.multi, w.ext_cand,
+                              w.ext_sqrt, w.max_reps, w.max_divisors, w.max_extensions};
+        const char *names[] = {"S_done", "pairs", "S_ge3", "triples", "4-sets", "5-sets", "6-sets", "multi",
+                               "ext_cand", "ext_sqrt", "max_reps", "max_divisors", "max_extensions"};
+        bool ok = true;
+        for (int k = 0; k < 13; ++k)
+            if (g[k] != e[k]) { printf("  %s got %llu want %llu\n", names[k], (unsigned long long)g[k], (unsigned long long)e[k]); ok = false; }
+        if (memcmp(c.hist5, w.hist5, sizeof c.hist5)) { printf("  hist5 differs\n"); ok = false; }
+        auto gl = split_sorted(got_text), wl = split_sorted(want_text);
+        if (gl != wl) {
+            printf("  lines differ: got %zu want %zu\n", gl.size(), wl.size());
+            std::vector<std::string> d;
+            std::set_symmetric_difference(gl.begin(), gl.end(), wl.begin(), wl.end(), std::back_inserter(d));
+            for (size_t k = 0; k < d.size() && k < 6; ++k) printf("    %s\n", d[k].c_str());
+            ok = false;
+        }
+        printf("[%llu,%llu) %s 4-sets=%llu ext_cand=%llu lines=%zu\n", lo, hi, ok ? "OK" : "MISMATCH",
+               (unsigned long long)c.sets4, (unsigned long long)c.ext_cand, gl.size());
+        fflush(stdout);
+        if (!ok) bad = 1;
+    }
+    ps4_close(ctx);
+    return bad;
+}

@@ -1,0 +1,200 @@
+// hip_host_shim.h -- runs the HIP sources of this repository on the CPU.
+//
+// Test infrastructure: lets the unmodified kernels (ps4_kernels.hip) and the
+// host library (ps4_lib.cpp) be compiled by g++ with AddressSanitizer, so
+// every global-memory and LDS access is bounds-checked on the host before the
+// code runs on a GPU.  Each workgroup is emulated by blockDim.x OS threads (one
+// per lane); workgroups of a launch run one after another.  Barriers,
+// wave shuffles, readfirstlane and atomics follow the GPU semantics the kernels
+// rely on (all lanes of a wave active at every cross-lane operation).
+#pragma once
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#include <math.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __noinline__ __attribute__((noinline))
+#define __launch_bounds__(...)
+// LDS: every __shared__ array lives in one section that is filled with garbage
+// at each workgroup launch (real LDS starts with arbitrary contents).
+#define __shared__ static __attribute__((section("emu_lds")))
+extern "C" char __start_emu_lds[], __stop_emu_lds[];
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct uint4 { unsigned x, y, z, w; };
+static inline uint4 make_uint4(unsigned a, unsigned b, unsigned c, unsigned d) { return uint4{a, b, c, d}; }
+
+namespace emu {
+struct Barrier {
+    std::mutex m;
+    std::condition_variable cv;
+    unsigned n = 0, count = 0, gen = 0;
+    void init(unsigned k) { n = k; count = 0; gen = 0; }
+    void wait() {
+        std::unique_lock<std::mutex> lk(m);
+        const unsigned g = gen;
+        if (++count == n) { count = 0; ++gen; cv.notify_all(); return; }
+        cv.wait(lk, [&] { return gen != g; });
+    }
+};
+struct Block {
+    unsigned nthreads = 0;
+    Barrier bar;
+    std::vector<Barrier> wave_bar;
+    std::vector<uint64_t> xchg;        // one slot per thread for cross-lane exchange
+    std::vector<int> red;              // block reduction scratch
+};
+extern Block *g_block;
+extern thread_local dim3 t_tid, t_bid, t_bdim, t_gdim;
+}  // namespace emu
+
+#define threadIdx (emu::t_tid)
+#define blockIdx (emu::t_bid)
+#define blockDim (emu::t_bdim)
+#define gridDim (emu::t_gdim)
+
+static inline void __syncthreads() { emu::g_block->bar.wait(); }
+
+static inline int emu_block_reduce(int v, bool is_or)
+{
+    emu::Block *b = emu::g_block;
+    b->red[threadIdx.x] = v;
+    b->bar.wait();
+    int r = is_or ? 0 : 1;
+    for (unsigned i = 0; i < b->nthreads; ++i) r = is_or ? (r | (b->red[i] != 0)) : (r & (b->red[i] != 0));
+    b->bar.wait();
+    return r;
+}
+static inline int __syncthreads_or(int p) { return emu_block_reduce(p, true); }
+static inline int __syncthreads_and(int p) { return emu_block_reduce(p, false); }
+
+static inline uint64_t emu_wave_exchange(uint64_t v, unsigned src_lane)
+{
+    emu::Block *b = emu::g_block;
+    const unsigned t = threadIdx.x, w = t / 64, base = w * 64;
+    b->xchg[t] = v;
+    b->wave_bar[w].wait();
+    const uint64_t r = b->xchg[base + (src_lane & 63)];
+    b->wave_bar[w].wait();
+    return r;
+}
+static inline unsigned __shfl_up(unsigned v, unsigned d, int width = 64)
+{
+    (void)width;
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned src = lane >= d ? lane - d : lane;
+    return (unsigned)emu_wave_exchange(v, src);
+}
+static inline unsigned __shfl_xor(unsigned v, int m, int width = 64)
+{
+    (void)width;
+    const unsigned lane = threadIdx.x & 63;
+    return (unsigned)emu_wave_exchange(v, lane ^ (unsigned)m);
+}
+static inline int __builtin_amdgcn_readfirstlane(int v) { return (int)emu_wave_exchange((uint32_t)v, 0); }
+static inline unsigned __umul24(unsigned a, unsigned b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }
+
+template <typename T, typename V> static inline T atomicAdd(T *p, V v) { return __atomic_fetch_add(p, (T)v, __ATOMIC_SEQ_CST); }
+template <typename T, typename V> static inline T atomicOr(T *p, V v) { return __atomic_fetch_or(p, (T)v, __ATOMIC_SEQ_CST); }
+template <typename T, typename V> static inline T atomicMax(T *p, V vv)
+{
+    const T v = (T)vv;
+    T old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (old < v && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
+    return old;
+}
+
+// ---- runtime API subset -------------------------------------------------------------
+typedef int hipError_t;
+#define hipSuccess 0
+#define hipErrorOutOfMemory 2
+typedef void *hipStream_t;
+typedef double *hipEvent_t;
+#define hipStreamNonBlocking 1
+#define hipMemcpyHostToDevice 1
+#define hipMemcpyDeviceToHost 2
+#define HIP_SYMBOL(x) (&(x))
+struct hipDeviceProp_t { int multiProcessorCount; char name[256]; };
+static inline const char *hipGetErrorString(hipError_t e) { return e ? "emulated error" : "no error"; }
+static inline hipError_t hipGetLastError() { return hipSuccess; }
+static inline hipError_t hipSetDevice(int) { return hipSuccess; }
+static inline hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int)
+{
+    p->multiProcessorCount = 2;
+    strcpy(p->name, "host-emulation");
+    return hipSuccess;
+}
+static inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = (void *)1; return hipSuccess; }
+static inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+static inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+static inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+static inline double emu_now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+static inline hipError_t hipEventCreate(hipEvent_t *e) { *e = new double(0); return hipSuccess; }
+static inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
+static inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { *e = emu_now_ms(); return hipSuccess; }
+static inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(*b - *a); return hipSuccess; }
+template <typename T> static inline hipError_t hipMalloc(T **p, size_t n)
+{
+    *p = (T *)malloc(n ? n : 1);
+    if (!*p) return hipErrorOutOfMemory;
+    memset(*p, 0xA5, n < ((size_t)64 << 20) ? n : ((size_t)64 << 20));   // device memory is not zero-initialised
+    return hipSuccess;
+}
+template <typename T> static inline hipError_t hipHostMalloc(T **p, size_t n) { return hipMalloc(p, n); }
+static inline hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+static inline hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
+static inline hipError_t hipMemcpy(void *d, const void *s, size_t n, int) { memcpy(d, s, n); return hipSuccess; }
+static inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, int, hipStream_t) { memcpy(d, s, n); return hipSuccess; }
+static inline hipError_t hipMemGetInfo(size_t *f, size_t *t) { *f = (size_t)8 << 30; *t = (size_t)16 << 30; return hipSuccess; }
+static inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, const void *, int, size_t) { *n = 1; return hipSuccess; }
+
+namespace emu {
+template <typename F, typename... Args>
+void launch(F kernel, dim3 grid, dim3 block, Args... args)
+{
+    const unsigned nt = block.x;
+    for (unsigned bx = 0; bx < grid.x; ++bx) {
+        Block b;
+        b.nthreads = nt;
+        b.bar.init(nt);
+        b.wave_bar = std::vector<Barrier>((nt + 63) / 64);
+        for (unsigned w = 0; w < b.wave_bar.size(); ++w) b.wave_bar[w].init(std::min(64u, nt - w * 64));
+        b.xchg.assign(nt, 0);
+        b.red.assign(nt, 0);
+        g_block = &b;
+        memset(__start_emu_lds, 0x5A ^ (int)(bx & 0xFF), (size_t)(__stop_emu_lds - __start_emu_lds));
+        std::vector<std::thread> th;
+        th.reserve(nt);
+        for (unsigned t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                t_tid = dim3(t, 0, 0);
+                t_bid = dim3(bx, 0, 0);
+                t_bdim = block;
+                t_gdim = grid;
+                kernel(args...);
+            });
+        for (auto &x : th) x.join();
+        g_block = nullptr;
+    }
+}
+}  // namespace emu
+
+#define hipLaunchKernelGGL(k, grid, block, shmem, stream, ...) emu::launch(k, dim3(grid), dim3(block), __VA_ARGS__)
