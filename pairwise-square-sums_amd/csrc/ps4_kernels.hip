@@ -534,14 +534,18 @@ __device__ __forceinline__ u32 fold4095(u32 r)
     u32 s = (r >> 12) + (r & 4095u);
     return s >= 4095u ? s - 4095u : s;
 }
-// floor(s/63), floor(s/65) for s < 4095 and floor(s/11..19) for s < 46189,
-// by 24-bit multiply-shift (checked exhaustively on the host).
-__device__ __forceinline__ u32 mod63(u32 s) { return s - 63u * (__umul24(s, 16645u) >> 20); }
-__device__ __forceinline__ u32 mod65(u32 s) { return s - 65u * (__umul24(s, 16132u) >> 20); }
-__device__ __forceinline__ u32 mod11(u32 s) { return s - 11u * (__umul24(s, 47663u) >> 19); }
-__device__ __forceinline__ u32 mod13(u32 s) { return s - 13u * (__umul24(s, 40330u) >> 19); }
-__device__ __forceinline__ u32 mod17(u32 s) { return s - 17u * (__umul24(s, 30841u) >> 19); }
-__device__ __forceinline__ u32 mod19(u32 s) { return s - 19u * (__umul24(s, 55189u) >> 20); }
+// s mod 63, s mod 65 for s < 4095 and s mod 11..19 for s < 46189 by a 32-bit
+// multiply and a logical shift (the products stay below 2^32; the constants are
+// checked exhaustively on the host).  Plain multiplies, not __umul24: with
+// __umul24 the compiler emitted an arithmetic shift of the product, which is
+// wrong once the product reaches 2^31 (the residue then came out near 90000 and
+// indexed far past the residue tables).
+__device__ __forceinline__ u32 mod63(u32 s) { return s - 63u * ((s * 16645u) >> 20); }
+__device__ __forceinline__ u32 mod65(u32 s) { return s - 65u * ((s * 16132u) >> 20); }
+__device__ __forceinline__ u32 mod11(u32 s) { return s - 11u * ((s * 47663u) >> 19); }
+__device__ __forceinline__ u32 mod13(u32 s) { return s - 13u * ((s * 40330u) >> 19); }
+__device__ __forceinline__ u32 mod17(u32 s) { return s - 17u * ((s * 30841u) >> 19); }
+__device__ __forceinline__ u32 mod19(u32 s) { return s - 19u * ((s * 55189u) >> 20); }
 
 #define M1 262080.0
 #define M2 46189.0
@@ -715,15 +719,22 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
             const u32 r1a = (u32)dmod(d1, M1, 1.0 / M1), r1b = (u32)dmod(d2, M1, 1.0 / M1);
             const u32 r2a = (u32)dmod(d1, M2, 1.0 / M2), r2b = (u32)dmod(d2, M2, 1.0 / M2);
             const u32 fa = fold4095(r1a), fb = fold4095(r1b);
-            s_m64[tid] = qr->t64[r1a & 63u] & qr->t64[r1b & 63u];
-            s_m63[tid] = qr->t63[mod63(fa)] & qr->t63[mod63(fb)];
-            const u32 i65a = mod65(fa), i65b = mod65(fb);
-            s_m65[tid] = qr->t65[i65a][0] & qr->t65[i65b][0];
-            const u64 m65hi = qr->t65[i65a][1] & qr->t65[i65b][1] & 1ull;
-            const u64 m11 = qr->t11[mod11(r2a)] & qr->t11[mod11(r2b)];
-            const u64 m13 = qr->t13[mod13(r2a)] & qr->t13[mod13(r2b)];
-            const u64 m17 = qr->t17[mod17(r2a)] & qr->t17[mod17(r2b)];
-            const u64 m19 = qr->t19[mod19(r2a)] & qr->t19[mod19(r2b)];
+            // residues index small tables: each index is checked against its table
+            // (a wrong residue is an error, never a load outside the table)
+            u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(fa), mod63(fb), mod65(fa), mod65(fb), mod11(r2a), mod11(r2b),
+                          mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
+            const u32 lim[7] = {64, 63, 65, 11, 13, 17, 19};
+#pragma unroll
+            for (int q = 0; q < 14; ++q)
+                if (ix[q] >= lim[q >> 1]) { err |= ERR_BOUNDS; ix[q] = 0; }
+            s_m64[tid] = qr->t64[ix[0]] & qr->t64[ix[1]];
+            s_m63[tid] = qr->t63[ix[2]] & qr->t63[ix[3]];
+            s_m65[tid] = qr->t65[ix[4]][0] & qr->t65[ix[5]][0];
+            const u64 m65hi = qr->t65[ix[4]][1] & qr->t65[ix[5]][1] & 1ull;
+            const u64 m11 = qr->t11[ix[6]] & qr->t11[ix[7]];
+            const u64 m13 = qr->t13[ix[8]] & qr->t13[ix[9]];
+            const u64 m17 = qr->t17[ix[10]] & qr->t17[ix[11]];
+            const u64 m19 = qr->t19[ix[12]] & qr->t19[ix[13]];
             s_msm[tid] = m65hi | (m11 << 1) | (m13 << 12) | (m17 << 25) | (m19 << 42);
             // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m, i.e. u is a
             // root of a pair sum containing A, or u^2 = 2A

@@ -56,7 +56,7 @@ def test_kernel_residue_constants_are_exact(name, m, shift, lim):
     """The multiply-shift divisions in the extension kernel equal true division
     over their whole input range (inputs are residues below lim)."""
     src = open(KERNELS).read()
-    mt = re.search(name + r"\(u32 s\) \{ return s - (\d+)u \* \(__umul24\(s, (\d+)u\) >> (\d+)\); \}", src)
+    mt = re.search(name + r"\(u32 s\) \{ return s - (\d+)u \* \(\(s \* (\d+)u\) >> (\d+)\); \}", src)
     assert mt, name
     mm, magic, sh = int(mt.group(1)), int(mt.group(2)), int(mt.group(3))
     assert mm == m and sh == shift and magic < 2**24
@@ -70,3 +70,24 @@ def test_fold4095_is_exact():
         s = (r >> 12) + (r & 4095)
         s = s - 4095 if s >= 4095 else s
         assert s == r % 4095
+
+
+def test_codegen_has_no_signed_shift_of_unsigned_products(tmp_path):
+    """Regression guard for a bring-up fault: the gfx950 code of the kernels must
+    not shift a multiply result arithmetically (with __umul24 the compiler emitted
+    v_mul_u32_u24 + v_ashrrev_i32, turning residues >= 2^31/magic into indices
+    ~90000 and reading far past the residue tables)."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    out = tmp_path / "k.s"
+    src = os.path.join(ps4.PKG_DIR, "csrc", "ps4_kernels.hip")
+    subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                    "-packed-fp32-ops", "-I" + os.path.join(ps4.PKG_DIR, "csrc"), "-I" + os.path.join(REPO, "include"),
+                    "--cuda-device-only", "-S", src, "-o", str(out)], check=True, stdout=subprocess.PIPE,
+                   stderr=subprocess.PIPE)
+    asm = out.read_text()
+    assert "v_ashrrev_i32" not in asm
+    assert "__umul24" not in open(src).read()
