@@ -90,4 +90,4 @@ def test_codegen_has_no_signed_shift_of_unsigned_products(tmp_path):
                    stderr=subprocess.PIPE)
     asm = out.read_text()
     assert "v_ashrrev_i32" not in asm
-    assert "__umul24" not in open(src).read()
+    assert "__umul24(" not in open(src).read()
