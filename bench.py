@@ -108,7 +108,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the census shards by S range "
-              f"across GPUs as described in DESIGN.md, but this build measures one device.", file=sys.stderr)
+              f"across GPUs (disjoint S intervals, one engine per device), but this bench measures one device.", file=sys.stderr)
         sys.exit(2)
     import ps4
     lo, hi = args.lo, args.hi

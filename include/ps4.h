@@ -61,7 +61,8 @@ typedef struct {
     int64_t dump4_limit;   /* emit the first dump4_limit 4-sets per search as "4:" lines
                               (--dump4 N, :237-241; 0 = none)                                   */
     uint32_t cap_reps, cap_divs, cap_ext;   /* capacity limits; 0 = the reference's 512 / 16384 / 64
-                                               (S4_MAX_REPS/DIVS/EXT, :45-54)                     */
+                                               (S4_MAX_REPS/DIVS/EXT, :45-54); cap_divs may not
+                                               exceed 16384 (PS4_E_ARG)                           */
 } ps4_config;
 
 /* Receives result text in the reference's line grammar ("5: [...] S=..",

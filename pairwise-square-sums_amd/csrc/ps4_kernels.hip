@@ -10,11 +10,11 @@
 //        every S with >= 3 representations, S != 2 mod 4, into 4-sets (pass 4,
 //        :311-341).  Triples are spread evenly over the threads; 4-sets go to a
 //        global queue in a count-then-write pair of sweeps (one atomic per window).
-// k_ext  one workgroup per tile of 128 queued 4-sets (persistent, work queue):
-//        per thread the pair choice (:163-171), the factorisation of
+// k_ext  one single-wave workgroup per tile of 64 queued 4-sets (persistent,
+//        work queue): per lane the pair choice (:163-171), the factorisation of
 //        D = (ra-rb)(ra+rb) by a resident smallest-prime table, the closed-form
 //        candidate count, and two half divisor lists in an LDS arena; then the
-//        divisor pairs of all 128 4-sets are flattened over the threads.  Each
+//        divisor pairs of all 64 4-sets are flattened over the lanes.  Each
 //        candidate is filtered by per-4-set quadratic-residue masks (mod 64, 63,
 //        65: the ext_sqrt rule of :192-198), by the exact e>0 / e not in the set
 //        conditions, then by residue masks mod 11, 13, 17, 19 and a size bound;
@@ -499,9 +499,10 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 // k_ext: extension of queued 4-sets
 // ----------------------------------------------------------------------------
 
-#define EX_NT 128
-#define EX_ARENA 3072
-#define EX_MAXF 16
+#define EX_NT 64          // one wave per workgroup: every barrier below is a wave's own
+#define EX_ARENA 1024     // doubles of divisor lists per pass; one 4-set with d(D) <= 16384
+                          // needs at most 918 (nL + nH over the best prefix split)
+#define EX_MAXF 12        // distinct odd primes of D < 2^51 (3*5*...*43 > 2^51)
 
 struct ExArgs {
     const ps4_quad *quads;
@@ -550,18 +551,45 @@ __device__ __forceinline__ u32 mod19(u32 s) { return s - 19u * ((s * 55189u) >> 
 #define M1 262080.0
 #define M2 46189.0
 
+// The chosen pair of a 4-set (exhaustive/s4.c:163-171 and the switch that
+// follows): A = v_i, the other two members C1, C2 in index order, the three
+// roots of pair sums containing A, and the pair roots ra > rb with
+// D = v_i - v_j = (ra - rb)(ra + rb).  Members a > b > c > d (index 0..3).
+struct ExPair {
+    u64 A, C1, C2, F1, F2, F3, ra, rb;
+};
+
+__device__ __forceinline__ ExPair ex_pair(u32 best, u64 va, u64 vb, u64 vc, u64 vd, u64 rab, u64 rac, u64 rad,
+                                          u64 rbc, u64 rbd, u64 rcd)
+{
+    ExPair p;
+    switch (best) {
+    case 0: p = {va, vc, vd, rab, rac, rad, rac, rbc}; break;   // (i,j,k) = (0,1,2)
+    case 1: p = {vc, va, vb, rac, rbc, rcd, rac, rad}; break;   // (2,3,0)
+    case 2: p = {va, vb, vd, rab, rac, rad, rab, rbc}; break;   // (0,2,1)
+    case 3: p = {vb, va, vc, rab, rbc, rbd, rab, rad}; break;   // (1,3,0)
+    case 4: p = {va, vb, vc, rab, rac, rad, rab, rbd}; break;   // (0,3,1)
+    default: p = {vb, va, vd, rab, rbc, rbd, rab, rac}; break;  // (1,2,0)
+    }
+    return p;
+}
 
 // One surviving candidate: exact e = u^2 - A and the two square tests of
-// exhaustive/s4.c:195-200.  Rare (after all residue filters), so out of line.
-__device__ __noinline__ u32 ext_exact(double u, u64 A, u64 C1, u64 C2, const ps4_quad *quads, u64 qidx,
-                                      u32 best, ps4_ctl *ctl, ps4_hit *hits, u32 hit_cap)
+// exhaustive/s4.c:195-200.  Rare (after all residue filters), so out of line;
+// A, C1, C2 are rederived from the queued 4-set.
+__device__ __noinline__ u32 ext_exact(double u, const ps4_quad *quads, u64 qidx, u32 best, ps4_ctl *ctl,
+                                      ps4_hit *hits, u32 hit_cap)
 {
+    const ps4_quad qd = quads[qidx];
+    const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
+    const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
+    const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
+    const ExPair P = ex_pair(best, va, vb, vc, vd, rab, rac, rad, rbc, rbd, rcd);
     const u64 uu = (u64)u;
-    const u128 e = (u128)uu * uu - A;
-    if (!d_is_square128(e + C1) || !d_is_square128(e + C2)) return 0;
+    const u128 e = (u128)uu * uu - P.A;
+    if (!d_is_square128(e + P.C1) || !d_is_square128(e + P.C2)) return 0;
     const u32 slot = atomicAdd(&ctl->hit_count, 1u);
     if (slot >= hit_cap) return ERR_HIT_OVF;
-    const ps4_quad qd = quads[qidx];
     ps4_hit h;
     h.r[0] = qd.r_ab; h.r[1] = qd.r_ac; h.r[2] = qd.r_ad;
     h.r[3] = qd.r_bc; h.r[4] = qd.r_bd; h.r[5] = qd.r_cd;
@@ -573,16 +601,49 @@ __device__ __noinline__ u32 ext_exact(double u, u64 A, u64 C1, u64 C2, const ps4
     return 0;
 }
 
+// Add p^e to a thread's factor list (slots [slot][thread] in s_fac), merging a
+// prime already present; `seen` is a 64-bit filter of the primes stored so far.
+__device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e, u32 &nf, u64 &seen, u32 &err)
+{
+    const u64 bit = 1ull << (p & 63u);
+    u32 sl = nf;
+    if (seen & bit)
+        for (u32 i = 0; i < nf; ++i)
+            if ((s_fac[i * EX_NT + tid] >> 5) == p) sl = i;
+    if (sl < nf) {
+        s_fac[sl * EX_NT + tid] += e;
+    } else if (nf < EX_MAXF) {
+        s_fac[nf * EX_NT + tid] = (p << 5) | e;
+        ++nf;
+        seen |= bit;
+    } else {
+        err |= ERR_FACTORS;
+    }
+}
+
+// Workgroup = one wave; a tile is 64 queued 4-sets, one per lane for the setup:
+//   validation, pair choice, factorisation of D by the smallest-prime table
+//   (the two factors ra - rb and ra + rb walked side by side), d(D), the
+//   candidate count ncand = d(D')/2 (exhaustive/s4.c:172-186), the split of
+//   the factor sequence into L and H parts, residue masks, e > 0 and e-member
+//   bounds.
+// Then passes: the lists L and H (all divisors are L[i]*H[j]) of as many
+// 4-sets as the arena holds, and their candidates spread evenly over the lanes.
+// Candidate t in [0, ncand) of a 4-set is the divisor pair
+//   f = L[i] * H[j],  g = D'/f = L[nL-1-i] * H[nH-1-j],   t = i + nL*j,
+// i.e. the index pair (t, d(D')-1-t) of the mixed-radix divisor list: every
+// unordered pair f != g exactly once, and f + g needs no ordering.
 __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
 {
-    __shared__ double s_ar[EX_ARENA];          // half divisor lists, exact doubles
-    __shared__ u32 s_fac[EX_MAXF * EX_NT];     // odd prime factors of D: p << 5 | e, [slot][thread]
-    __shared__ u32 s_pref[EX_NT + 1];          // candidate prefix of the current round
-    __shared__ u32 s_l0[EX_NT], s_l1[EX_NT];   // offL | nL<<16,  offH | nH<<16 | half<<31
+    __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
+    __shared__ u32 s_fac[EX_MAXF * EX_NT];     // odd prime factors of D: p << 5 | e, [slot][lane]
+    __shared__ u32 s_pref[EX_NT + 1];          // candidate prefix of the current pass
+    __shared__ u32 s_w0[EX_NT];                // offL | nL << 10 | offH << 20 | half << 30
+    __shared__ u32 s_w1[EX_NT];                // nH | ncand << 10 | best << 24
     __shared__ u64 s_m64[EX_NT], s_m63[EX_NT], s_m65[EX_NT], s_msm[EX_NT];
-    __shared__ double s_sqA[EX_NT], s_f1[EX_NT], s_f2[EX_NT], s_f3[EX_NT], s_f4[EX_NT], s_um[EX_NT];
-    __shared__ u64 s_A[EX_NT], s_C1[EX_NT], s_C2[EX_NT];
-    __shared__ u32 s_esq[EX_NT], s_best[EX_NT];
+    __shared__ u32 s_sqA[EX_NT], s_f1[EX_NT], s_f2[EX_NT], s_f3[EX_NT], s_f4[EX_NT];
+    __shared__ double s_um[EX_NT];
+    __shared__ u32 s_esq[EX_NT];
     __shared__ u32 s_w[EX_NT / 64];
     __shared__ u64 s_w64[EX_NT / 64];
     __shared__ u32 s_tile, s_rid0;
@@ -604,7 +665,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
         bool valid = qidx < nquads;
         u32 err = 0, my_rid = 0;
         u64 my_cand = 0, my_maxdiv = 0;
-        u32 nf = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0;
+        u32 nf = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0, best = 0;
         if (tid == 0) s_rid0 = a.quads[tile * EX_NT].rid;   // tile < ntiles: entry exists
         s_esq[tid] = 0;
         if (valid) {
@@ -623,143 +684,131 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                 err |= ERR_BADQUAD;
                 valid = false;
             }
-        }
-        if (valid) {
-            const ps4_quad qd = a.quads[qidx];
-            my_rid = qd.rid;
-            const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
-            const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
-            const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
-            // Pair choice: fewest estimated divisors d(ra-rb)*d(ra+rb) of D = v_i - v_j,
-            // first minimum in the scan order of exhaustive/s4.c:163-171.  Members are
-            // a > b > c > d, so the reference's swap never fires.
-            const u64 ra_[6] = {rac, rac, rab, rab, rab, rab};
-            const u64 rb_[6] = {rbc, rad, rbc, rad, rbd, rac};
-            u32 best = 0;
-            u64 bestv = ~0ull;
+            if (valid) {
+                my_rid = qd.rid;
+                // Pair choice: fewest estimated divisors d(ra-rb)*d(ra+rb) of
+                // D = v_i - v_j, first minimum in the scan order of
+                // exhaustive/s4.c:163-171 (members are a > b > c > d, so the
+                // reference's swap never fires).
+                const u64 ra_[6] = {rac, rac, rab, rab, rab, rab};
+                const u64 rb_[6] = {rbc, rad, rbc, rad, rbd, rac};
+                u32 nd_[12];
 #pragma unroll
-            for (int t = 0; t < 6; ++t) {
-                const u64 e = (u64)a.ndiv[ra_[t] - rb_[t]] * a.ndiv[ra_[t] + rb_[t]];
-                if (e < bestv) { bestv = e; best = (u32)t; }
-            }
-            // A = v_i, the other two members C1, C2 in index order, and the three
-            // roots of pairs containing A (u = one of them means e is a member).
-            u64 A, C1, C2, F1, F2, F3, ra, rb;
-            switch (best) {
-            case 0: A = va; C1 = vc; C2 = vd; F1 = rab; F2 = rac; F3 = rad; ra = rac; rb = rbc; break;
-            case 1: A = vc; C1 = va; C2 = vb; F1 = rac; F2 = rbc; F3 = rcd; ra = rac; rb = rad; break;
-            case 2: A = va; C1 = vb; C2 = vd; F1 = rab; F2 = rac; F3 = rad; ra = rab; rb = rbc; break;
-            case 3: A = vb; C1 = va; C2 = vc; F1 = rab; F2 = rbc; F3 = rbd; ra = rab; rb = rad; break;
-            case 4: A = va; C1 = vb; C2 = vc; F1 = rab; F2 = rac; F3 = rad; ra = rab; rb = rbd; break;
-            default: A = vb; C1 = va; C2 = vd; F1 = rab; F2 = rbc; F3 = rbd; ra = rab; rb = rac; break;
-            }
-            s_best[tid] = best;
-            // D = m1 * m2: strip the twos, then walk the odd parts through the table
-            u64 m1 = ra - rb, m2 = ra + rb;
-            const u32 z1 = (u32)__builtin_ctzll(m1), z2 = (u32)__builtin_ctzll(m2);
-            const u32 e2 = z1 + z2;
-            m1 >>= z1;
-            m2 >>= z2;
-            u32 n1 = 0;
-            for (int pass = 0; pass < 2; ++pass) {
-                u64 m = pass ? m2 : m1;
-                for (int guard = 0; m > 1; ++guard) {
-                    if (m >= a.tab_n || guard >= 64) { err |= ERR_FACTORS; break; }   // never for a valid 4-set
-                    const u64 en = a.ftab[m];
-                    const u32 p = (u32)(en >> 32) & 0x7FFFFFFu, e = (u32)(en >> 59);
-                    m = en & 0xFFFFFFFFull;
-                    u32 sl = nf;
-                    if (pass)
-                        for (u32 i = 0; i < n1; ++i)
-                            if ((s_fac[i * EX_NT + tid] >> 5) == p) sl = i;
-                    if (sl < nf) {
-                        s_fac[sl * EX_NT + tid] += e;
-                    } else if (nf < EX_MAXF) {
-                        s_fac[nf * EX_NT + tid] = (p << 5) | e;
-                        ++nf;
-                    } else {
-                        err |= ERR_FACTORS;
+                for (int t = 0; t < 6; ++t) {
+                    nd_[2 * t] = a.ndiv[ra_[t] - rb_[t]];
+                    nd_[2 * t + 1] = a.ndiv[ra_[t] + rb_[t]];
+                }
+                u32 bestv = 0xFFFFFFFFu;
+#pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    const u32 e = nd_[2 * t] * nd_[2 * t + 1];
+                    if (e < bestv) { bestv = e; best = (u32)t; }
+                }
+                const ExPair P = ex_pair(best, va, vb, vc, vd, rab, rac, rad, rbc, rbd, rcd);
+                // D = m1 * m2: strip the twos, then walk both odd parts through
+                // the table side by side (two independent load chains)
+                u64 m1 = P.ra - P.rb, m2 = P.ra + P.rb;
+                const u32 z1 = (u32)__builtin_ctzll(m1), z2 = (u32)__builtin_ctzll(m2);
+                const u32 e2 = z1 + z2;
+                m1 >>= z1;
+                m2 >>= z2;
+                u64 seen = 0;
+                for (int guard = 0; m1 > 1 || m2 > 1; ++guard) {
+                    if (m1 >= a.tab_n || m2 >= a.tab_n || guard >= 64) { err |= ERR_FACTORS; break; }   // never for a valid 4-set
+                    const u64 en1 = m1 > 1 ? a.ftab[m1] : 0ull;
+                    const u64 en2 = m2 > 1 ? a.ftab[m2] : 0ull;
+                    if (m1 > 1) {
+                        ex_add_factor(s_fac, tid, (u32)(en1 >> 32) & 0x7FFFFFFu, (u32)(en1 >> 59), nf, seen, err);
+                        m1 = en1 & 0xFFFFFFFFull;
+                    }
+                    if (m2 > 1) {
+                        ex_add_factor(s_fac, tid, (u32)(en2 >> 32) & 0x7FFFFFFu, (u32)(en2 >> 59), nf, seen, err);
+                        m2 = en2 & 0xFFFFFFFFull;
                     }
                 }
-                if (!pass) n1 = nf;
-            }
-            u64 ndodd = 1;
-            for (u32 i = 0; i < nf; ++i) ndodd *= (s_fac[i * EX_NT + tid] & 31u) + 1;
-            const u64 ndD = (u64)(e2 + 1) * ndodd;
-            my_maxdiv = ndD;
-            if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
-            // Divisor pairs f < g of D' = D (D odd, u = (f+g)/2) or D/4 (4 | D, u = f+g);
-            // none when D = 2 mod 4.  Their number is the reference's ext_cand.
-            if (e2 != 1 && ndD <= (1ull << 20)) {
-                half = (e2 == 0);
-                e2p = half ? 0u : e2 - 2;
-                const u64 ndp = (u64)(e2p + 1) * ndodd;
-                my_cand = ndp / 2;
-                if (ndp > 1) {
-                    // split the factor sequence (2 first when e2' > 0, then the odd
-                    // primes) into a prefix L and the rest H, minimising nL + nH
-                    const u32 nfa = nf + (e2p ? 1u : 0u);
-                    u64 nl = 1, bestc = 1 + ndp, bestnl = 1;
-                    for (u32 s = 1; s <= nfa; ++s) {
-                        const u32 ee = e2p ? (s == 1 ? e2p : (s_fac[(s - 2) * EX_NT + tid] & 31u))
-                                           : (s_fac[(s - 1) * EX_NT + tid] & 31u);
-                        nl *= ee + 1;
-                        const u64 c = nl + ndp / nl;
-                        if (c < bestc) { bestc = c; sL = s; bestnl = nl; }
+                u64 ndodd = 1;
+                for (u32 i = 0; i < nf; ++i) ndodd *= (s_fac[i * EX_NT + tid] & 31u) + 1;
+                const u64 ndD = (u64)(e2 + 1) * ndodd;
+                my_maxdiv = ndD;
+                if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
+                // Divisor pairs f < g of D' = D (D odd, u = (f+g)/2) or D/4 (4 | D,
+                // u = f+g); none when D = 2 mod 4.  Their number is the
+                // reference's ext_cand.
+                if (e2 != 1 && ndD <= a.cap_divs) {
+                    half = (e2 == 0);
+                    e2p = half ? 0u : e2 - 2;
+                    const u32 ndp = (u32)((u64)(e2p + 1) * ndodd);
+                    ncand = ndp / 2;
+                    my_cand = ncand;
+                    if (ncand > 0) {
+                        // split the factor sequence (2 first when e2' > 0, then the
+                        // odd primes) into a prefix L and the rest H, minimising
+                        // nL + nH; ndp / nl is exact (nl divides ndp < 2^24) by a
+                        // rounded single-precision quotient
+                        const u32 nfa = nf + (e2p ? 1u : 0u);
+                        const float fndp = (float)ndp;
+                        u32 nl = 1, bestc = 1 + ndp, bestnl = 1;
+                        for (u32 s = 1; s <= nfa; ++s) {
+                            const u32 ee = e2p ? (s == 1 ? e2p : (s_fac[(s - 2) * EX_NT + tid] & 31u))
+                                               : (s_fac[(s - 1) * EX_NT + tid] & 31u);
+                            nl *= ee + 1;
+                            const u32 nh = (u32)__builtin_rintf(fndp * __builtin_amdgcn_rcpf((float)nl));
+                            const u32 c = nl + nh;
+                            if (c < bestc) { bestc = c; sL = s; bestnl = nl; }
+                        }
+                        nL = bestnl;
+                        nH = bestc - bestnl;
+                        lsize = bestc;
+                        if (nL * nH != ndp) { err |= ERR_FACTORS; ncand = 0; }   // never: the quotient is exact
                     }
-                    nL = (u32)bestnl;
-                    nH = (u32)(ndp / bestnl);
-                    lsize = nL + nH;
-                    ncand = (u32)ndp;
                 }
-            }
-            // quadratic-residue masks of this 4-set: bit r of M_m set iff r^2 + (C1-A)
-            // and r^2 + (C2-A) are both squares mod m (the filters of s4.c:192-194)
-            const double d1 = (double)(i64)(C1 - A), d2 = (double)(i64)(C2 - A);
-            const u32 r1a = (u32)dmod(d1, M1, 1.0 / M1), r1b = (u32)dmod(d2, M1, 1.0 / M1);
-            const u32 r2a = (u32)dmod(d1, M2, 1.0 / M2), r2b = (u32)dmod(d2, M2, 1.0 / M2);
-            const u32 fa = fold4095(r1a), fb = fold4095(r1b);
-            // residues index small tables: each index is checked against its table
-            // (a wrong residue is an error, never a load outside the table)
-            u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(fa), mod63(fb), mod65(fa), mod65(fb), mod11(r2a), mod11(r2b),
-                          mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
-            const u32 lim[7] = {64, 63, 65, 11, 13, 17, 19};
+                // quadratic-residue masks of this 4-set: bit r of M_m set iff
+                // r^2 + (C1-A) and r^2 + (C2-A) are both squares mod m (the
+                // filters of exhaustive/s4.c:192-194)
+                const double d1 = (double)(i64)(P.C1 - P.A), d2 = (double)(i64)(P.C2 - P.A);
+                const u32 r1a = (u32)dmod(d1, M1, 1.0 / M1), r1b = (u32)dmod(d2, M1, 1.0 / M1);
+                const u32 r2a = (u32)dmod(d1, M2, 1.0 / M2), r2b = (u32)dmod(d2, M2, 1.0 / M2);
+                const u32 fa = fold4095(r1a), fb = fold4095(r1b);
+                // residues index small tables: each index is checked against its
+                // table (a wrong residue is an error, never a load outside it)
+                u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(fa), mod63(fb), mod65(fa), mod65(fb), mod11(r2a),
+                              mod11(r2b), mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
+                const u32 lim[7] = {64, 63, 65, 11, 13, 17, 19};
 #pragma unroll
-            for (int q = 0; q < 14; ++q)
-                if (ix[q] >= lim[q >> 1]) { err |= ERR_BOUNDS; ix[q] = 0; }
-            s_m64[tid] = qr->t64[ix[0]] & qr->t64[ix[1]];
-            s_m63[tid] = qr->t63[ix[2]] & qr->t63[ix[3]];
-            s_m65[tid] = qr->t65[ix[4]][0] & qr->t65[ix[5]][0];
-            const u64 m65hi = qr->t65[ix[4]][1] & qr->t65[ix[5]][1] & 1ull;
-            const u64 m11 = qr->t11[ix[6]] & qr->t11[ix[7]];
-            const u64 m13 = qr->t13[ix[8]] & qr->t13[ix[9]];
-            const u64 m17 = qr->t17[ix[10]] & qr->t17[ix[11]];
-            const u64 m19 = qr->t19[ix[12]] & qr->t19[ix[13]];
-            s_msm[tid] = m65hi | (m11 << 1) | (m13 << 12) | (m17 << 25) | (m19 << 42);
-            // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m, i.e. u is a
-            // root of a pair sum containing A, or u^2 = 2A
-            const u64 sa = d_isqrt64(A), s2a = d_isqrt64(2 * A);
-            s_sqA[tid] = (double)sa;
-            s_f1[tid] = (double)F1;
-            s_f2[tid] = (double)F2;
-            s_f3[tid] = (double)F3;
-            s_f4[tid] = (s2a * s2a == 2 * A) ? (double)s2a : -1.0;
-            // u^2 + (C - A) can be square only if u <= (|C - A| + 1) / 2
-            const u64 ad1 = C1 > A ? C1 - A : A - C1, ad2 = C2 > A ? C2 - A : A - C2;
-            s_um[tid] = (double)(((ad1 < ad2 ? ad1 : ad2) + 1) >> 1);
-            s_A[tid] = A;
-            s_C1[tid] = C1;
-            s_C2[tid] = C2;
+                for (int q = 0; q < 14; ++q)
+                    if (ix[q] >= lim[q >> 1]) { err |= ERR_BOUNDS; ix[q] = 0; }
+                s_m64[tid] = qr->t64[ix[0]] & qr->t64[ix[1]];
+                s_m63[tid] = qr->t63[ix[2]] & qr->t63[ix[3]];
+                s_m65[tid] = qr->t65[ix[4]][0] & qr->t65[ix[5]][0];
+                const u64 m65hi = qr->t65[ix[4]][1] & qr->t65[ix[5]][1] & 1ull;
+                const u64 m11 = qr->t11[ix[6]] & qr->t11[ix[7]];
+                const u64 m13 = qr->t13[ix[8]] & qr->t13[ix[9]];
+                const u64 m17 = qr->t17[ix[10]] & qr->t17[ix[11]];
+                const u64 m19 = qr->t19[ix[12]] & qr->t19[ix[13]];
+                s_msm[tid] = m65hi | (m11 << 1) | (m13 << 12) | (m17 << 25) | (m19 << 42);
+                // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m,
+                // i.e. u is a root of a pair sum containing A, or u^2 = 2A (every
+                // root is below 2^26 for S < 2^50)
+                const u64 sa = d_isqrt64(P.A), s2a = d_isqrt64(2 * P.A);
+                s_sqA[tid] = (u32)sa;
+                s_f1[tid] = (u32)P.F1;
+                s_f2[tid] = (u32)P.F2;
+                s_f3[tid] = (u32)P.F3;
+                s_f4[tid] = (s2a * s2a == 2 * P.A) ? (u32)s2a : 0u;   // u >= 2: 0 never matches
+                // u^2 + (C - A) can be square only if u <= (|C - A| + 1) / 2
+                const u64 ad1 = P.C1 > P.A ? P.C1 - P.A : P.A - P.C1, ad2 = P.C2 > P.A ? P.C2 - P.A : P.A - P.C2;
+                s_um[tid] = (double)(((ad1 < ad2 ? ad1 : ad2) + 1) >> 1);
+            }
         }
         __syncthreads();
 
-        // ---- rounds: divisor lists of as many 4-sets as the arena holds, then all
-        //      their candidates flattened over the workgroup
+        // ---- passes: divisor lists of as many 4-sets as the arena holds, then
+        //      all their candidates spread over the lanes
         bool pending = ncand > 0;
         while (block_any<EX_NT>(pending, s_w)) {
             u32 ltot;
             const u32 loff = block_excl_scan<EX_NT>(pending ? lsize : 0u, s_w, &ltot);
-            bool fits = pending && loff + lsize <= EX_ARENA;
+            const bool fits = pending && loff + lsize <= EX_ARENA;
             if (pending && loff == 0 && lsize > EX_ARENA) { err |= ERR_ARENA; pending = false; }
             const u32 mine = fits ? ncand : 0u;
             u32 ctot;
@@ -768,9 +817,11 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
             s_pref[tid] = cpre;
             if (tid == EX_NT - 1) s_pref[EX_NT] = cpre + mine;
             if (fits) {
-                s_l0[tid] = loff | (nL << 16);
-                s_l1[tid] = (loff + nL) | (nH << 16) | (half << 31);
-                // mixed-radix lists over the factor sequence
+                s_w0[tid] = loff | (nL << 10) | ((loff + nL) << 20) | (half << 30);
+                s_w1[tid] = nH | (ncand << 10) | (best << 24);
+                // mixed-radix lists over the factor sequence: stage s multiplies
+                // the list so far by p_s^1 .. p_s^e (earlier stages are the less
+                // significant digits, so index n and n_max - n are complements)
                 const u32 nfa = nf + (e2p ? 1u : 0u);
                 double *lst = s_ar + loff;
                 u32 n = 1;
@@ -794,7 +845,8 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                 if (sL == nfa) s_ar[loff + nL] = 1.0;   // empty H part: the list {1}
             }
             __syncthreads();
-            const u32 g0 = (u32)(((u64)tid * ctot) / EX_NT), g1 = (u32)(((u64)(tid + 1) * ctot) / EX_NT);
+            // lane l takes the candidates [l*ctot/64, (l+1)*ctot/64) of the pass
+            const u32 g0 = (tid * ctot) >> 6, g1 = ((tid + 1) * ctot) >> 6;
             if (g0 < g1) {
                 u32 lo = 0, hi = EX_NT;
                 while (hi - lo > 1) {
@@ -802,57 +854,57 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                     if (s_pref[mid] <= g0) lo = mid; else hi = mid;
                 }
                 u32 k = lo;
-                u32 offL, cnL, offH, cnH;
-                bool hf;
+                u32 offL, cnL, offH, cnH, nc, kbest;
+                double scale, sqA, f1, f2, f3, f4, um;
                 u64 m64, m63, m65, msm;
-                double sqA, f1, f2, f3, f4, um;
 #define EX_LOAD_PARAMS()                                                                  \
     do {                                                                                  \
-        const u32 L0 = s_l0[k], L1 = s_l1[k];                                             \
-        offL = L0 & 0xFFFFu; cnL = L0 >> 16; offH = L1 & 0xFFFFu; cnH = (L1 >> 16) & 0x7FFFu; \
-        hf = (L1 >> 31) != 0;                                                             \
+        const u32 W0 = s_w0[k], W1 = s_w1[k];                                             \
+        offL = W0 & 1023u; cnL = (W0 >> 10) & 1023u; offH = (W0 >> 20) & 1023u;           \
+        scale = (W0 >> 30) ? 0.5 : 1.0;                                                   \
+        cnH = W1 & 1023u; nc = (W1 >> 10) & 0x3FFFu; kbest = W1 >> 24;                   \
         m64 = s_m64[k]; m63 = s_m63[k]; m65 = s_m65[k]; msm = s_msm[k];                   \
-        sqA = s_sqA[k]; f1 = s_f1[k]; f2 = s_f2[k]; f3 = s_f3[k]; f4 = s_f4[k]; um = s_um[k]; \
+        sqA = (double)s_sqA[k]; f1 = (double)s_f1[k]; f2 = (double)s_f2[k];               \
+        f3 = (double)s_f3[k]; f4 = (double)s_f4[k]; um = s_um[k];                         \
     } while (0)
                 EX_LOAD_PARAMS();
-                const u32 loc = g0 - s_pref[k];
-                u32 jj = loc / cnL, ii = loc - jj * cnL;
-                double hj = s_ar[offH + jj], hjc = s_ar[offH + cnH - 1 - jj];
+                u32 t = g0 - s_pref[k];
+                u32 j = t / cnL, i = t - j * cnL;
+                double hj = s_ar[offH + j], hjc = s_ar[offH + cnH - 1 - j];
                 u32 esq = 0;
                 for (u32 g = g0; g < g1; ++g) {
-                    const double f = s_ar[offL + ii] * hj, gg = s_ar[offL + cnL - 1 - ii] * hjc;
-                    if (f < gg) {
-                        const double u = hf ? (f + gg) * 0.5 : f + gg;
-                        const u32 r1 = (u32)dmod(u, M1, 1.0 / M1);
-                        const u32 r4095 = fold4095(r1);
-                        const u32 r63 = mod63(r4095), r65 = mod65(r4095);
-                        const u64 b65 = r65 < 64u ? (m65 >> r65) : msm;
-                        const bool pass = ((m64 >> (r1 & 63u)) & (m63 >> r63) & b65 & 1ull) != 0;
-                        if (pass && u > sqA && u != f1 && u != f2 && u != f3 && u != f4) {
-                            ++esq;
-                            if (u <= um) {
-                                const u32 r2 = (u32)dmod(u, M2, 1.0 / M2);
-                                const u64 t = (msm >> (1u + mod11(r2))) & (msm >> (12u + mod13(r2))) &
-                                              (msm >> (25u + mod17(r2))) & (msm >> (42u + mod19(r2)));
-                                if (t & 1ull)
-                                    err |= ext_exact(u, s_A[k], s_C1[k], s_C2[k], a.quads, tile * EX_NT + k, s_best[k],
-                                                     a.ctl, a.hits, a.hit_cap);
-                            }
+                    const double f = s_ar[offL + i] * hj;
+                    const double u = __builtin_fma(s_ar[offL + cnL - 1 - i], hjc, f) * scale;
+                    const u32 r1 = (u32)dmod(u, M1, 1.0 / M1);
+                    const u32 r4095 = fold4095(r1);
+                    const u32 r63 = mod63(r4095), r65 = mod65(r4095);
+                    const u64 b65 = r65 < 64u ? (m65 >> r65) : msm;
+                    const bool pass = ((m64 >> (r1 & 63u)) & (m63 >> r63) & b65 & 1ull) != 0;
+                    if (pass && u > sqA && u != f1 && u != f2 && u != f3 && u != f4) {
+                        ++esq;
+                        if (u <= um) {
+                            const u32 r2 = (u32)dmod(u, M2, 1.0 / M2);
+                            const u64 tt = (msm >> (1u + mod11(r2))) & (msm >> (12u + mod13(r2))) &
+                                           (msm >> (25u + mod17(r2))) & (msm >> (42u + mod19(r2)));
+                            if (tt & 1ull)
+                                err |= ext_exact(u, a.quads, tile * EX_NT + k, kbest, a.ctl, a.hits, a.hit_cap);
                         }
                     }
-                    if (++ii == cnL) {
-                        ii = 0;
-                        if (++jj == cnH) {
-                            jj = 0;
-                            if (g + 1 < g1) {
-                                atomicAdd(&s_esq[k], esq);
-                                esq = 0;
-                                do { ++k; } while (k < EX_NT - 1 && s_pref[k + 1] == s_pref[k]);
-                                EX_LOAD_PARAMS();
-                            }
+                    if (++t == nc) {   // this 4-set is done: on to the next one with candidates
+                        if (g + 1 < g1) {
+                            atomicAdd(&s_esq[k], esq);
+                            esq = 0;
+                            do { ++k; } while (k < EX_NT - 1 && s_pref[k + 1] == s_pref[k]);
+                            EX_LOAD_PARAMS();
+                            t = 0; i = 0; j = 0;
+                            hj = s_ar[offH];
+                            hjc = s_ar[offH + cnH - 1];
                         }
-                        hj = s_ar[offH + jj];
-                        hjc = s_ar[offH + cnH - 1 - jj];
+                    } else if (++i == cnL) {
+                        i = 0;
+                        ++j;
+                        hj = s_ar[offH + j];
+                        hjc = s_ar[offH + cnH - 1 - j];
                     }
                 }
 #undef EX_LOAD_PARAMS

@@ -107,6 +107,7 @@ static inline unsigned __shfl_xor(unsigned v, int m, int width = 64)
     return (unsigned)emu_wave_exchange(v, lane ^ (unsigned)m);
 }
 static inline int __builtin_amdgcn_readfirstlane(int v) { return (int)emu_wave_exchange((uint32_t)v, 0); }
+static inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 static inline unsigned __umul24(unsigned a, unsigned b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }
 
 template <typename T, typename V> static inline T atomicAdd(T *p, V v) { return __atomic_fetch_add(p, (T)v, __ATOMIC_SEQ_CST); }

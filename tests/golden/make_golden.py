@@ -14,6 +14,13 @@ and bruteforce-scratchpad/dotnet/1mln.txt.
 
 Usage:  python3 tests/golden/make_golden.py [--big]
         --big adds [1,1e9) and [1,1e10) (about 4 minutes on 8 cores).
+        python3 tests/golden/make_golden.py --add LO HI
+        runs one more range (8 threads) and adds it to the existing fixture;
+        [1, 6250000000) is there this way (about 2.5 minutes on 8 cores): it is
+        the first thread's range of exhaustive/logs/s4d_1e11.err, whose
+        4-set and extension counters come from an earlier build of the program
+        than exhaustive/s4.c (its TOTAL line carries "nonprimitive skipped=",
+        which s4.c does not print).
 """
 import hashlib
 import json
@@ -83,7 +90,19 @@ def reference_logs():
     return recs
 
 
+def add_run(lo, hi):
+    data = json.load(open(OUT))
+    rec, _ = run_s4(lo, hi, 8)
+    data["runs"] = [r for r in data["runs"] if (r["lo"], r["hi"]) != (lo, hi)] + [rec]
+    with open(OUT, "w") as f:
+        json.dump(data, f, indent=1)
+    print(lo, hi, rec["counters"]["4-sets"], rec["n_lines"], "added to", OUT)
+
+
 def main():
+    if "--add" in sys.argv:
+        i = sys.argv.index("--add")
+        return add_run(int(sys.argv[i + 1]), int(sys.argv[i + 2]))
     big = "--big" in sys.argv
     if not os.path.exists(S4):
         sys.exit("build the reference first: make -C oracle ref")

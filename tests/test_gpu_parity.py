@@ -47,16 +47,19 @@ def check(rec, res, lines, what=""):
     assert sha_sorted(lines) == rec["sorted_sha256"], what
 
 
-# The single-S search [4205,4206) took a GPU memory fault during bring-up; it
-# stays off the GPU until the next round ("Bring-up faults").
-# Every wider range below still covers S = 4205.
-SKIP_THIS_ROUND = {(4205, 4206)}
+# The single-S search [4205,4206) was running when the GPU took a memory fault
+# during bring-up (a residue multiply-shift compiled to an arithmetic shift
+# indexed past a residue table; fixed, with a code-generation test in
+# test_abi.py).  The test of the first fix faulted again, so the case stays off
+# the GPU; the host emulation (tests/emu) runs it, and every wider range below
+# still covers S = 4205.
+SKIPPED_ON_GPU = {(4205, 4206)}
 
 
 def test_tiny_ranges(eng, golden):
     """Single S values and tiny windows (including S = 1..7 edge cases)."""
     for rec in golden["runs"]:
-        if (rec["lo"], rec["hi"]) in SKIP_THIS_ROUND:
+        if (rec["lo"], rec["hi"]) in SKIPPED_ON_GPU:
             continue
         if rec["hi"] - rec["lo"] <= 8:
             say(f"tiny [{rec['lo']},{rec['hi']})")
@@ -324,12 +327,31 @@ def test_census_1e10(eng, golden):
     check(rec, res, lines)
 
 
+def test_census_first_thread_range_of_1e11_log(eng, golden):
+    """[1, 6.25e9), the first thread's range of exhaustive/logs/s4d_1e11.err:
+    every counter and all 38019 lines against exhaustive/s4.c."""
+    rec = next(r for r in golden["runs"] if (r["lo"], r["hi"]) == (1, 6250000000))
+    say("[1,6.25e9)")
+    res, lines = eng.search(1, 6250000000)
+    check(rec, res, lines)
+
+
 def test_census_1e11_reference_log(eng, golden):
-    """[1,1e11) against the reference's own log (exhaustive/logs/s4d_1e11.err:68-69)."""
+    """[1,1e11) against the reference's own log (exhaustive/logs/s4d_1e11.err:68-69).
+
+    The log was written by an earlier build of the program than
+    exhaustive/s4.c: its TOTAL line carries "nonprimitive skipped=", which s4.c
+    does not print, and on the log's first thread range [1, 6.25e9) s4.c finds
+    759664239 4-sets and 37818 5-sets where the log has 759675299 and 37820
+    (the golden run of that range).  The counters up to the representation
+    triples do not depend on that difference and must agree; the 4-set and
+    extension counters are checked against s4.c itself on [1, 6.25e9) and
+    [1, 1e10)."""
     r = next(x for x in golden["reference_logs"] if x["source"].endswith("s4d_1e11.err"))
     say("[1,1e11)")
     res, lines = eng.search(r["lo"], r["hi"])
-    assert res["counters"] == r["counters"]
-    assert res["hist5"] == r["hist5"]
-    assert sum(ln.startswith("5:") for ln in lines) == r["counters"]["5-sets"]
-    assert sum(ln.startswith("M:") for ln in lines) == r["counters"]["multi-ext-4sets"]
+    for k in ("S_done", "pairs", "S_ge3", "triples", "6-sets"):
+        assert res["counters"][k] == r["counters"][k], k
+    assert sum(ln.startswith("5:") for ln in lines) == res["counters"]["5-sets"]
+    assert sum(ln.startswith("M:") for ln in lines) == res["counters"]["multi-ext-4sets"]
+    assert sum(res["hist5"].values()) == res["counters"]["5-sets"]
