@@ -503,6 +503,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 #define EX_ARENA 1024     // doubles of divisor lists per pass; one 4-set with d(D) <= 16384
                           // needs at most 918 (nL + nH over the best prefix split)
 #define EX_MAXF 12        // distinct odd primes of D < 2^51 (3*5*...*43 > 2^51)
+#define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
+#define EX_CHUNK 4        // tiles claimed per work-queue atomic
 
 struct ExArgs {
     const ps4_quad *quads;
@@ -519,6 +521,18 @@ struct ExArgs {
     u32 n_rids;             // counter rows in ctr
 };
 
+// Per-4-set parameters of the candidate loop (one record per 4-set of a pass,
+// 64 bytes, read as four 16-byte words).
+struct ExPar {
+    u32 w0;                 // offL | nL << 10 | offH << 20 | half << 30
+    u32 w1;                 // nH | lane << 10 | best << 16
+    float um;               // >= (min |C - A| + 1) / 2
+    u32 f4;                 // isqrt(2A) when 2A is a square, else 0
+    u64 m64, m63;           // residue masks mod 64, 63
+    u64 m65, msm;           // mod 65 (bits 0..63); bit 64 of mod 65 and the masks mod 11..19
+    u32 sqA, f1, f2, f3;    // isqrt(A) and the roots of pair sums containing A
+};
+
 // x mod m for an integer-valued double |x| < 2^53; result in [0, m).
 __device__ __forceinline__ double dmod(double x, double m, double inv_m)
 {
@@ -529,7 +543,8 @@ __device__ __forceinline__ double dmod(double x, double m, double inv_m)
     return r;
 }
 
-// Residues of a value mod 4095 = 63*65 given its residue mod 262080 = 64*4095.
+// Residues of a value mod 4095 = 63*65 given a representative below 2^20 of
+// its class mod 262080 = 64*4095.
 __device__ __forceinline__ u32 fold4095(u32 r)
 {
     u32 s = (r >> 12) + (r & 4095u);
@@ -550,6 +565,16 @@ __device__ __forceinline__ u32 mod19(u32 s) { return s - 19u * ((s * 55189u) >> 
 
 #define M1 262080.0
 #define M2 46189.0
+
+// A representative in [0, 3*M1) of x mod M1 for an integer-valued double
+// 0 <= x < 2^52: the floor of the rounded quotient is within one of the true
+// quotient, so x - q*M1 (exact by the fused multiply-add) lies in [-M1, 2*M1).
+// Residues mod 64, 63 and 65 do not depend on which representative.
+__device__ __forceinline__ u32 rep_m1(double x)
+{
+    const double q = __builtin_floor(x * (1.0 / M1));
+    return (u32)(__builtin_fma(-q, M1, x) + M1);
+}
 
 // The chosen pair of a 4-set (exhaustive/s4.c:163-171 and the switch that
 // follows): A = v_i, the other two members C1, C2 in index order, the three
@@ -601,7 +626,7 @@ __device__ __noinline__ u32 ext_exact(double u, const ps4_quad *quads, u64 qidx,
     return 0;
 }
 
-// Add p^e to a thread's factor list (slots [slot][thread] in s_fac), merging a
+// Add p^e to a lane's factor list (slots [slot][lane] in s_fac), merging a
 // prime already present; `seen` is a 64-bit filter of the primes stored so far.
 __device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e, u32 &nf, u64 &seen, u32 &err)
 {
@@ -621,6 +646,38 @@ __device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e,
     }
 }
 
+__device__ __forceinline__ u64 wave_sum_u64(u64 v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_u64(v, m);
+    return v;
+}
+__device__ __forceinline__ u64 wave_max_u64(u64 v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) { const u64 o = shfl_xor_u64(v, m); v = o > v ? o : v; }
+    return v;
+}
+// exclusive prefix sum over the wave; *total gets the sum (uniform)
+__device__ __forceinline__ u32 wave_excl_scan(u32 v, u32 *total)
+{
+    const u32 incl = wave_incl_scan(v);
+    *total = uni((u32)__shfl((int)incl, 63, 64));
+    return incl - v;
+}
+
+// Add a wave's accumulated counters of interval `rid` to the device counters.
+__device__ __forceinline__ void ex_flush(u64 *ctr, u32 n_rids, u32 rid, u64 cand, u64 esq, u64 md)
+{
+    const u64 tc = wave_sum_u64(cand), te = wave_sum_u64(esq), tm = wave_max_u64(md);
+    if (threadIdx.x == 0 && rid < n_rids) {
+        unsigned long long *c = (unsigned long long *)(ctr + (u64)rid * CT_N);
+        atomicAdd(c + CT_EXT_CAND, (unsigned long long)tc);
+        atomicAdd(c + CT_EXT_SQRT, (unsigned long long)te);
+        atomicMax(c + CT_MAX_DIVS, (unsigned long long)tm);
+    }
+}
+
 // Workgroup = one wave; a tile is 64 queued 4-sets, one per lane for the setup:
 //   validation, pair choice, factorisation of D by the smallest-prime table
 //   (the two factors ra - rb and ra + rb walked side by side), d(D), the
@@ -628,314 +685,324 @@ __device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e,
 //   the factor sequence into L and H parts, residue masks, e > 0 and e-member
 //   bounds.
 // Then passes: the lists L and H (all divisors are L[i]*H[j]) of as many
-// 4-sets as the arena holds, and their candidates spread evenly over the lanes.
-// Candidate t in [0, ncand) of a 4-set is the divisor pair
+// 4-sets as the arena holds, and their candidates spread over the lanes with
+// stride 64 (lane l takes candidates l, l+64, ...; the 4-set of a candidate
+// comes from a bitmap of 4-set starts).  Candidate t in [0, ncand) of a 4-set
+// is the divisor pair
 //   f = L[i] * H[j],  g = D'/f = L[nL-1-i] * H[nH-1-j],   t = i + nL*j,
 // i.e. the index pair (t, d(D')-1-t) of the mixed-radix divisor list: every
 // unordered pair f != g exactly once, and f + g needs no ordering.
+// Counters are summed per wave over consecutive tiles of one interval and
+// added to the device counters when the interval changes and at the end.
 __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
 {
     __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
     __shared__ u32 s_fac[EX_MAXF * EX_NT];     // odd prime factors of D: p << 5 | e, [slot][lane]
-    __shared__ u32 s_pref[EX_NT + 1];          // candidate prefix of the current pass
-    __shared__ u32 s_w0[EX_NT];                // offL | nL << 10 | offH << 20 | half << 30
-    __shared__ u32 s_w1[EX_NT];                // nH | ncand << 10 | best << 24
-    __shared__ u64 s_m64[EX_NT], s_m63[EX_NT], s_m65[EX_NT], s_msm[EX_NT];
-    __shared__ u32 s_sqA[EX_NT], s_f1[EX_NT], s_f2[EX_NT], s_f3[EX_NT], s_f4[EX_NT];
-    __shared__ double s_um[EX_NT];
-    __shared__ u32 s_esq[EX_NT];
-    __shared__ u32 s_w[EX_NT / 64];
-    __shared__ u64 s_w64[EX_NT / 64];
-    __shared__ u32 s_tile, s_rid0;
+    __shared__ ExPar s_par[EX_NT];             // parameters of the 4-sets of the current pass
+    __shared__ u32 s_bits[EX_CMAX / 32];       // bit g: a 4-set's candidates start at g
 
     const u32 tid = threadIdx.x;
     const u64 qc = a.ctl->quad_count;   // written by the previous kernel on this stream
     u64 nquads = ((u64)uni((u32)(qc >> 32)) << 32) | uni((u32)qc);
     if (nquads > a.quad_cap) nquads = a.quad_cap;
     const u64 ntiles = (nquads + EX_NT - 1) / EX_NT;
+    const u64 nchunks = (ntiles + EX_CHUNK - 1) / EX_CHUNK;
     const ps4_qrtab *qr = a.qr;
 
-    for (;;) {
-        __syncthreads();   // keeps this claim apart from the previous tile's tid == 0 work
-        if (tid == 0) s_tile = atomicAdd(&a.ctl->tile_next, 1u);
-        __syncthreads();
-        const u64 tile = uni(s_tile);
-        if (tile >= ntiles) break;
-        const u64 qidx = tile * EX_NT + tid;
-        bool valid = qidx < nquads;
-        u32 err = 0, my_rid = 0;
-        u64 my_cand = 0, my_maxdiv = 0;
-        u32 nf = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0, best = 0;
-        if (tid == 0) s_rid0 = a.quads[tile * EX_NT].rid;   // tile < ntiles: entry exists
-        s_esq[tid] = 0;
-        if (valid) {
-            const ps4_quad qd = a.quads[qidx];
-            const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
-            const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
-            const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
-            // every root difference and sum below indexes the tables: a queued
-            // 4-set must satisfy its identities and lie inside them, else it is
-            // reported (ERR_BADQUAD) and not extended
-            const bool good = qd.rid < a.n_rids && va > vb && vb > vc && vc > vd && vd > 0 &&
-                              vb + vc == rbc * rbc && vb + vd == rbd * rbd && vc + vd == rcd * rcd &&
-                              va + vd == rad * rad && rab + rac < a.tab_n && rac > rbc && rac > rad &&
-                              rab > rbc && rab > rad && rab > rbd && rab > rac;
-            if (!good) {
-                err |= ERR_BADQUAD;
-                valid = false;
+    // this wave's counters of interval acc_rid (lane partial sums)
+    u32 acc_rid = 0xFFFFFFFFu;
+    u64 acc_cand = 0, acc_esq = 0, acc_md = 0;
+    u32 err = 0;
+
+    u32 chunk = 0;
+    if (tid == 0) chunk = atomicAdd(&a.ctl->tile_next, 1u);
+    chunk = uni(chunk);   // lane 0's value (every lane is active here)
+    while (chunk < nchunks) {
+        // claim the next chunk now; its latency overlaps this chunk's work
+        u32 next = 0;
+        if (tid == 0) next = atomicAdd(&a.ctl->tile_next, 1u);
+        for (u32 sub = 0; sub < EX_CHUNK; ++sub) {
+            const u64 tile = (u64)chunk * EX_CHUNK + sub;
+            if (tile >= ntiles) break;
+            const u64 qidx = tile * EX_NT + tid;
+            bool valid = qidx < nquads;
+            const ps4_quad qd = a.quads[valid ? qidx : tile * EX_NT];   // lane 0's entry exists
+            const u32 rid0 = uni(qd.rid);
+            const bool uniform = __ballot(valid && qd.rid != rid0) == 0ull && rid0 < a.n_rids;
+            if (!uniform || rid0 != acc_rid) {
+                ex_flush(a.ctr, a.n_rids, acc_rid, acc_cand, acc_esq, acc_md);
+                acc_cand = acc_esq = acc_md = 0;
+                acc_rid = uniform ? rid0 : 0xFFFFFFFFu;
             }
+            u64 my_cand = 0, my_maxdiv = 0;
+            u32 my_esq = 0;
+            u32 nf = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0, best = 0;
+            u64 pm64 = 0, pm63 = 0, pm65 = 0, pmsm = 0;
+            u32 psqA = 0, pf1 = 0, pf2 = 0, pf3 = 0, pf4 = 0;
+            float pum = 0.f;
             if (valid) {
-                my_rid = qd.rid;
-                // Pair choice: fewest estimated divisors d(ra-rb)*d(ra+rb) of
-                // D = v_i - v_j, first minimum in the scan order of
-                // exhaustive/s4.c:163-171 (members are a > b > c > d, so the
-                // reference's swap never fires).
-                const u64 ra_[6] = {rac, rac, rab, rab, rab, rab};
-                const u64 rb_[6] = {rbc, rad, rbc, rad, rbd, rac};
-                u32 nd_[12];
+                const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
+                const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
+                const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
+                // every root difference and sum below indexes the tables: a queued
+                // 4-set must satisfy its identities and lie inside them, else it is
+                // reported (ERR_BADQUAD) and not extended
+                const bool good = qd.rid < a.n_rids && va > vb && vb > vc && vc > vd && vd > 0 &&
+                                  vb + vc == rbc * rbc && vb + vd == rbd * rbd && vc + vd == rcd * rcd &&
+                                  va + vd == rad * rad && rab + rac < a.tab_n && rac > rbc && rac > rad &&
+                                  rab > rbc && rab > rad && rab > rbd && rab > rac;
+                if (!good) {
+                    err |= ERR_BADQUAD;
+                    valid = false;
+                }
+                if (valid) {
+                    // Pair choice: fewest estimated divisors d(ra-rb)*d(ra+rb) of
+                    // D = v_i - v_j, first minimum in the scan order of
+                    // exhaustive/s4.c:163-171 (members are a > b > c > d, so the
+                    // reference's swap never fires).
+                    const u64 ra_[6] = {rac, rac, rab, rab, rab, rab};
+                    const u64 rb_[6] = {rbc, rad, rbc, rad, rbd, rac};
+                    u32 nd_[12];
 #pragma unroll
-                for (int t = 0; t < 6; ++t) {
-                    nd_[2 * t] = a.ndiv[ra_[t] - rb_[t]];
-                    nd_[2 * t + 1] = a.ndiv[ra_[t] + rb_[t]];
-                }
-                u32 bestv = 0xFFFFFFFFu;
+                    for (int t = 0; t < 6; ++t) {
+                        nd_[2 * t] = a.ndiv[ra_[t] - rb_[t]];
+                        nd_[2 * t + 1] = a.ndiv[ra_[t] + rb_[t]];
+                    }
+                    u32 bestv = 0xFFFFFFFFu;
 #pragma unroll
-                for (int t = 0; t < 6; ++t) {
-                    const u32 e = nd_[2 * t] * nd_[2 * t + 1];
-                    if (e < bestv) { bestv = e; best = (u32)t; }
-                }
-                const ExPair P = ex_pair(best, va, vb, vc, vd, rab, rac, rad, rbc, rbd, rcd);
-                // D = m1 * m2: strip the twos, then walk both odd parts through
-                // the table side by side (two independent load chains)
-                u64 m1 = P.ra - P.rb, m2 = P.ra + P.rb;
-                const u32 z1 = (u32)__builtin_ctzll(m1), z2 = (u32)__builtin_ctzll(m2);
-                const u32 e2 = z1 + z2;
-                m1 >>= z1;
-                m2 >>= z2;
-                u64 seen = 0;
-                for (int guard = 0; m1 > 1 || m2 > 1; ++guard) {
-                    if (m1 >= a.tab_n || m2 >= a.tab_n || guard >= 64) { err |= ERR_FACTORS; break; }   // never for a valid 4-set
-                    const u64 en1 = m1 > 1 ? a.ftab[m1] : 0ull;
-                    const u64 en2 = m2 > 1 ? a.ftab[m2] : 0ull;
-                    if (m1 > 1) {
-                        ex_add_factor(s_fac, tid, (u32)(en1 >> 32) & 0x7FFFFFFu, (u32)(en1 >> 59), nf, seen, err);
-                        m1 = en1 & 0xFFFFFFFFull;
+                    for (int t = 0; t < 6; ++t) {
+                        const u32 e = nd_[2 * t] * nd_[2 * t + 1];
+                        if (e < bestv) { bestv = e; best = (u32)t; }
                     }
-                    if (m2 > 1) {
-                        ex_add_factor(s_fac, tid, (u32)(en2 >> 32) & 0x7FFFFFFu, (u32)(en2 >> 59), nf, seen, err);
-                        m2 = en2 & 0xFFFFFFFFull;
-                    }
-                }
-                u64 ndodd = 1;
-                for (u32 i = 0; i < nf; ++i) ndodd *= (s_fac[i * EX_NT + tid] & 31u) + 1;
-                const u64 ndD = (u64)(e2 + 1) * ndodd;
-                my_maxdiv = ndD;
-                if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
-                // Divisor pairs f < g of D' = D (D odd, u = (f+g)/2) or D/4 (4 | D,
-                // u = f+g); none when D = 2 mod 4.  Their number is the
-                // reference's ext_cand.
-                if (e2 != 1 && ndD <= a.cap_divs) {
-                    half = (e2 == 0);
-                    e2p = half ? 0u : e2 - 2;
-                    const u32 ndp = (u32)((u64)(e2p + 1) * ndodd);
-                    ncand = ndp / 2;
-                    my_cand = ncand;
-                    if (ncand > 0) {
-                        // split the factor sequence (2 first when e2' > 0, then the
-                        // odd primes) into a prefix L and the rest H, minimising
-                        // nL + nH; ndp / nl is exact (nl divides ndp < 2^24) by a
-                        // rounded single-precision quotient
-                        const u32 nfa = nf + (e2p ? 1u : 0u);
-                        const float fndp = (float)ndp;
-                        u32 nl = 1, bestc = 1 + ndp, bestnl = 1;
-                        for (u32 s = 1; s <= nfa; ++s) {
-                            const u32 ee = e2p ? (s == 1 ? e2p : (s_fac[(s - 2) * EX_NT + tid] & 31u))
-                                               : (s_fac[(s - 1) * EX_NT + tid] & 31u);
-                            nl *= ee + 1;
-                            const u32 nh = (u32)__builtin_rintf(fndp * __builtin_amdgcn_rcpf((float)nl));
-                            const u32 c = nl + nh;
-                            if (c < bestc) { bestc = c; sL = s; bestnl = nl; }
+                    const ExPair P = ex_pair(best, va, vb, vc, vd, rab, rac, rad, rbc, rbd, rcd);
+                    // D = m1 * m2: strip the twos, then walk both odd parts through
+                    // the table side by side (two independent load chains)
+                    u64 m1 = P.ra - P.rb, m2 = P.ra + P.rb;
+                    const u32 z1 = (u32)__builtin_ctzll(m1), z2 = (u32)__builtin_ctzll(m2);
+                    const u32 e2 = z1 + z2;
+                    m1 >>= z1;
+                    m2 >>= z2;
+                    u64 seen = 0;
+                    for (int guard = 0; m1 > 1 || m2 > 1; ++guard) {
+                        if (m1 >= a.tab_n || m2 >= a.tab_n || guard >= 64) { err |= ERR_FACTORS; break; }   // never for a valid 4-set
+                        const u64 en1 = m1 > 1 ? a.ftab[m1] : 0ull;
+                        const u64 en2 = m2 > 1 ? a.ftab[m2] : 0ull;
+                        if (m1 > 1) {
+                            ex_add_factor(s_fac, tid, (u32)(en1 >> 32) & 0x7FFFFFFu, (u32)(en1 >> 59), nf, seen, err);
+                            m1 = en1 & 0xFFFFFFFFull;
                         }
-                        nL = bestnl;
-                        nH = bestc - bestnl;
-                        lsize = bestc;
-                        if (nL * nH != ndp) { err |= ERR_FACTORS; ncand = 0; }   // never: the quotient is exact
+                        if (m2 > 1) {
+                            ex_add_factor(s_fac, tid, (u32)(en2 >> 32) & 0x7FFFFFFu, (u32)(en2 >> 59), nf, seen, err);
+                            m2 = en2 & 0xFFFFFFFFull;
+                        }
                     }
-                }
-                // quadratic-residue masks of this 4-set: bit r of M_m set iff
-                // r^2 + (C1-A) and r^2 + (C2-A) are both squares mod m (the
-                // filters of exhaustive/s4.c:192-194)
-                const double d1 = (double)(i64)(P.C1 - P.A), d2 = (double)(i64)(P.C2 - P.A);
-                const u32 r1a = (u32)dmod(d1, M1, 1.0 / M1), r1b = (u32)dmod(d2, M1, 1.0 / M1);
-                const u32 r2a = (u32)dmod(d1, M2, 1.0 / M2), r2b = (u32)dmod(d2, M2, 1.0 / M2);
-                const u32 fa = fold4095(r1a), fb = fold4095(r1b);
-                // residues index small tables: each index is checked against its
-                // table (a wrong residue is an error, never a load outside it)
-                u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(fa), mod63(fb), mod65(fa), mod65(fb), mod11(r2a),
-                              mod11(r2b), mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
-                const u32 lim[7] = {64, 63, 65, 11, 13, 17, 19};
+                    u64 ndodd = 1;
+                    for (u32 i = 0; i < nf; ++i) ndodd *= (s_fac[i * EX_NT + tid] & 31u) + 1;
+                    const u64 ndD = (u64)(e2 + 1) * ndodd;
+                    my_maxdiv = ndD;
+                    if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
+                    // Divisor pairs f < g of D' = D (D odd, u = (f+g)/2) or D/4
+                    // (4 | D, u = f+g); none when D = 2 mod 4.  Their number is the
+                    // reference's ext_cand.
+                    if (e2 != 1 && ndD <= a.cap_divs) {
+                        half = (e2 == 0);
+                        e2p = half ? 0u : e2 - 2;
+                        const u32 ndp = (u32)((u64)(e2p + 1) * ndodd);
+                        ncand = ndp / 2;
+                        my_cand = ncand;
+                        if (ncand > 0) {
+                            // split the factor sequence (2 first when e2' > 0, then
+                            // the odd primes) into a prefix L and the rest H,
+                            // minimising nL + nH; ndp / nl is exact (nl divides
+                            // ndp < 2^24) by a rounded single-precision quotient
+                            const u32 nfa = nf + (e2p ? 1u : 0u);
+                            const float fndp = (float)ndp;
+                            u32 nl = 1, bestc = 1 + ndp, bestnl = 1;
+                            for (u32 s = 1; s <= nfa; ++s) {
+                                const u32 ee = e2p ? (s == 1 ? e2p : (s_fac[(s - 2) * EX_NT + tid] & 31u))
+                                                   : (s_fac[(s - 1) * EX_NT + tid] & 31u);
+                                nl *= ee + 1;
+                                const u32 nh = (u32)__builtin_rintf(fndp * __builtin_amdgcn_rcpf((float)nl));
+                                const u32 c = nl + nh;
+                                if (c < bestc) { bestc = c; sL = s; bestnl = nl; }
+                            }
+                            nL = bestnl;
+                            nH = bestc - bestnl;
+                            lsize = bestc;
+                            if (nL * nH != ndp || ncand > EX_CMAX) { err |= ERR_FACTORS; ncand = 0; }   // never
+                        }
+                    }
+                    // quadratic-residue masks of this 4-set: bit r of M_m set iff
+                    // r^2 + (C1-A) and r^2 + (C2-A) are both squares mod m (the
+                    // filters of exhaustive/s4.c:192-194)
+                    const double d1 = (double)(i64)(P.C1 - P.A), d2 = (double)(i64)(P.C2 - P.A);
+                    const u32 r1a = (u32)dmod(d1, M1, 1.0 / M1), r1b = (u32)dmod(d2, M1, 1.0 / M1);
+                    const u32 r2a = (u32)dmod(d1, M2, 1.0 / M2), r2b = (u32)dmod(d2, M2, 1.0 / M2);
+                    const u32 fa = fold4095(r1a), fb = fold4095(r1b);
+                    // residues index small tables: each index is checked against
+                    // its table (a wrong residue is an error, never a load outside it)
+                    u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(fa), mod63(fb), mod65(fa), mod65(fb), mod11(r2a),
+                                  mod11(r2b), mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
+                    const u32 lim[7] = {64, 63, 65, 11, 13, 17, 19};
 #pragma unroll
-                for (int q = 0; q < 14; ++q)
-                    if (ix[q] >= lim[q >> 1]) { err |= ERR_BOUNDS; ix[q] = 0; }
-                s_m64[tid] = qr->t64[ix[0]] & qr->t64[ix[1]];
-                s_m63[tid] = qr->t63[ix[2]] & qr->t63[ix[3]];
-                s_m65[tid] = qr->t65[ix[4]][0] & qr->t65[ix[5]][0];
-                const u64 m65hi = qr->t65[ix[4]][1] & qr->t65[ix[5]][1] & 1ull;
-                const u64 m11 = qr->t11[ix[6]] & qr->t11[ix[7]];
-                const u64 m13 = qr->t13[ix[8]] & qr->t13[ix[9]];
-                const u64 m17 = qr->t17[ix[10]] & qr->t17[ix[11]];
-                const u64 m19 = qr->t19[ix[12]] & qr->t19[ix[13]];
-                s_msm[tid] = m65hi | (m11 << 1) | (m13 << 12) | (m17 << 25) | (m19 << 42);
-                // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m,
-                // i.e. u is a root of a pair sum containing A, or u^2 = 2A (every
-                // root is below 2^26 for S < 2^50)
-                const u64 sa = d_isqrt64(P.A), s2a = d_isqrt64(2 * P.A);
-                s_sqA[tid] = (u32)sa;
-                s_f1[tid] = (u32)P.F1;
-                s_f2[tid] = (u32)P.F2;
-                s_f3[tid] = (u32)P.F3;
-                s_f4[tid] = (s2a * s2a == 2 * P.A) ? (u32)s2a : 0u;   // u >= 2: 0 never matches
-                // u^2 + (C - A) can be square only if u <= (|C - A| + 1) / 2
-                const u64 ad1 = P.C1 > P.A ? P.C1 - P.A : P.A - P.C1, ad2 = P.C2 > P.A ? P.C2 - P.A : P.A - P.C2;
-                s_um[tid] = (double)(((ad1 < ad2 ? ad1 : ad2) + 1) >> 1);
+                    for (int q = 0; q < 14; ++q)
+                        if (ix[q] >= lim[q >> 1]) { err |= ERR_BOUNDS; ix[q] = 0; }
+                    pm64 = qr->t64[ix[0]] & qr->t64[ix[1]];
+                    pm63 = qr->t63[ix[2]] & qr->t63[ix[3]];
+                    pm65 = qr->t65[ix[4]][0] & qr->t65[ix[5]][0];
+                    const u64 m65hi = qr->t65[ix[4]][1] & qr->t65[ix[5]][1] & 1ull;
+                    const u64 m11 = qr->t11[ix[6]] & qr->t11[ix[7]];
+                    const u64 m13 = qr->t13[ix[8]] & qr->t13[ix[9]];
+                    const u64 m17 = qr->t17[ix[10]] & qr->t17[ix[11]];
+                    const u64 m19 = qr->t19[ix[12]] & qr->t19[ix[13]];
+                    pmsm = m65hi | (m11 << 1) | (m13 << 12) | (m17 << 25) | (m19 << 42);
+                    // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m,
+                    // i.e. u is a root of a pair sum containing A, or u^2 = 2A
+                    // (every root is below 2^26 for S < 2^50)
+                    const u64 sa = d_isqrt64(P.A), s2a = d_isqrt64(2 * P.A);
+                    psqA = (u32)sa;
+                    pf1 = (u32)P.F1;
+                    pf2 = (u32)P.F2;
+                    pf3 = (u32)P.F3;
+                    pf4 = (s2a * s2a == 2 * P.A) ? (u32)s2a : 0u;   // u >= 2: 0 never matches
+                    // u^2 + (C - A) can be square only if u <= (|C - A| + 1) / 2;
+                    // kept as a float rounded up (a weaker bound is still a bound)
+                    const u64 ad1 = P.C1 > P.A ? P.C1 - P.A : P.A - P.C1, ad2 = P.C2 > P.A ? P.C2 - P.A : P.A - P.C2;
+                    pum = (float)((double)(((ad1 < ad2 ? ad1 : ad2) + 1) >> 1) * (1.0 + 1.0 / 65536.0) + 1.0);
+                }
             }
-        }
-        __syncthreads();
 
-        // ---- passes: divisor lists of as many 4-sets as the arena holds, then
-        //      all their candidates spread over the lanes
-        bool pending = ncand > 0;
-        while (block_any<EX_NT>(pending, s_w)) {
-            u32 ltot;
-            const u32 loff = block_excl_scan<EX_NT>(pending ? lsize : 0u, s_w, &ltot);
-            const bool fits = pending && loff + lsize <= EX_ARENA;
-            if (pending && loff == 0 && lsize > EX_ARENA) { err |= ERR_ARENA; pending = false; }
-            const u32 mine = fits ? ncand : 0u;
-            u32 ctot;
-            const u32 cpre = block_excl_scan<EX_NT>(mine, s_w, &ctot);
-            ctot = uni(ctot);
-            s_pref[tid] = cpre;
-            if (tid == EX_NT - 1) s_pref[EX_NT] = cpre + mine;
-            if (fits) {
-                s_w0[tid] = loff | (nL << 10) | ((loff + nL) << 20) | (half << 30);
-                s_w1[tid] = nH | (ncand << 10) | (best << 24);
-                // mixed-radix lists over the factor sequence: stage s multiplies
-                // the list so far by p_s^1 .. p_s^e (earlier stages are the less
-                // significant digits, so index n and n_max - n are complements)
-                const u32 nfa = nf + (e2p ? 1u : 0u);
-                double *lst = s_ar + loff;
-                u32 n = 1;
-                lst[0] = 1.0;
-                for (u32 s = 0; s < nfa; ++s) {
-                    if (s == sL) {   // start the H list
-                        lst = s_ar + loff + nL;
-                        n = 1;
-                        lst[0] = 1.0;
+            // ---- passes: divisor lists of as many 4-sets as the arena holds,
+            //      then all their candidates spread over the lanes
+            bool pending = ncand > 0;
+            while (__ballot(pending) != 0ull) {
+                u32 ltot, ptot, ctot;
+                const u32 loff = wave_excl_scan(pending ? lsize : 0u, &ltot);
+                const u32 coff = wave_excl_scan(pending ? ncand : 0u, &ptot);
+                bool fits = pending && loff + lsize <= EX_ARENA && coff + ncand <= EX_CMAX;
+                if (pending && loff == 0 && !fits) { err |= ERR_ARENA; pending = false; }   // never (bounds above)
+                const u64 fmask = __ballot(fits);
+                const u32 slot = (u32)__popcll(fmask & ((1ull << tid) - 1ull));
+                (void)wave_excl_scan(fits ? ncand : 0u, &ctot);
+                // fitting 4-sets are a prefix of the pending ones: coff is each one's
+                // first candidate in the pass
+                for (u32 w = tid; w < (ctot + 31) / 32; w += EX_NT) s_bits[w] = 0u;
+                if (((ctot + 31) / 32) & 1u) s_bits[(ctot + 31) / 32] = 0u;   // the loop reads word pairs
+                __syncthreads();
+                if (fits) {
+                    ExPar P;
+                    P.w0 = loff | (nL << 10) | ((loff + nL) << 20) | (half << 30);
+                    P.w1 = nH | (tid << 10) | (best << 16);
+                    P.um = pum;
+                    P.f4 = pf4;
+                    P.m64 = pm64; P.m63 = pm63; P.m65 = pm65; P.msm = pmsm;
+                    P.sqA = psqA; P.f1 = pf1; P.f2 = pf2; P.f3 = pf3;
+                    s_par[slot] = P;
+                    atomicOr(&s_bits[coff >> 5], 1u << (coff & 31u));
+                    // mixed-radix lists over the factor sequence: stage s multiplies
+                    // the list so far by p_s^1 .. p_s^e (earlier stages are the less
+                    // significant digits, so index n and n_max - n are complements)
+                    const u32 nfa = nf + (e2p ? 1u : 0u);
+                    double *lst = s_ar + loff;
+                    u32 n = 1;
+                    lst[0] = 1.0;
+                    for (u32 s = 0; s < nfa; ++s) {
+                        if (s == sL) {   // start the H list
+                            lst = s_ar + loff + nL;
+                            n = 1;
+                            lst[0] = 1.0;
+                        }
+                        u32 p, e;
+                        if (e2p && s == 0) { p = 2; e = e2p; }
+                        else { const u32 w = s_fac[(s - (e2p ? 1u : 0u)) * EX_NT + tid]; p = w >> 5; e = w & 31u; }
+                        const u32 cur = n;
+                        double pk = 1.0;
+                        for (u32 ee = 0; ee < e; ++ee) {
+                            pk *= (double)p;
+                            u32 j = 0;
+                            for (; j + 4 <= cur; j += 4) {   // four loads in flight
+                                const double x0 = lst[j], x1 = lst[j + 1], x2 = lst[j + 2], x3 = lst[j + 3];
+                                lst[n] = x0 * pk; lst[n + 1] = x1 * pk; lst[n + 2] = x2 * pk; lst[n + 3] = x3 * pk;
+                                n += 4;
+                            }
+                            for (; j < cur; ++j) lst[n++] = lst[j] * pk;
+                        }
                     }
-                    u32 p, e;
-                    if (e2p && s == 0) { p = 2; e = e2p; }
-                    else { const u32 w = s_fac[(s - (e2p ? 1u : 0u)) * EX_NT + tid]; p = w >> 5; e = w & 31u; }
-                    const u32 cur = n;
-                    double pk = 1.0;
-                    for (u32 ee = 0; ee < e; ++ee) {
-                        pk *= (double)p;
-                        for (u32 j = 0; j < cur; ++j) lst[n++] = lst[j] * pk;
+                    if (sL == nfa) s_ar[loff + nL] = 1.0;   // empty H part: the list {1}
+                }
+                __syncthreads();
+                // lane l takes candidates l, l+64, ... of the pass
+                u32 kp = 0, sprev = 0;   // 4-set starts before base; start of the 4-set holding base-1
+                for (u32 base = 0; base < ctot; base += 64) {
+                    const u32 wi = base >> 5;
+                    const u64 word = (u64)s_bits[wi] | ((u64)s_bits[wi + 1] << 32);
+                    const u32 g = base + tid;
+                    const u64 below = word & ((2ull << tid) - 1ull);   // starts in [base, g]
+                    const u32 c = (u32)__popcll(below);
+                    const u32 k = kp + c - 1u;
+                    const u32 st = c ? base + 63u - (u32)__clzll((long long)below) : sprev;
+                    kp += (u32)__popcll(word);
+                    if (word) sprev = base + 63u - (u32)__clzll((long long)word);
+                    if (g < ctot) {
+                        const ExPar &P = s_par[k];
+                        const u32 W0 = P.w0, W1 = P.w1;
+                        const u32 offL = W0 & 1023u, cnL = (W0 >> 10) & 1023u, offH = (W0 >> 20) & 1023u;
+                        const u32 cnH = W1 & 1023u;
+                        const u32 t = g - st;
+                        int j = (int)((float)t * __builtin_amdgcn_rcpf((float)cnL));   // within one of t / nL
+                        int i = (int)t - j * (int)cnL;
+                        if (i < 0) { --j; i += (int)cnL; }
+                        else if (i >= (int)cnL) { ++j; i -= (int)cnL; }
+                        const double f = s_ar[offL + i] * s_ar[offH + j];
+                        const double uu2 = __builtin_fma(s_ar[offL + cnL - 1 - i], s_ar[offH + cnH - 1 - j], f);
+                        const double u = (W0 >> 30) ? uu2 * 0.5 : uu2;
+                        const u32 r1 = rep_m1(u);
+                        const u32 r4095 = fold4095(r1);
+                        const u32 r63 = mod63(r4095), r65 = mod65(r4095);
+                        const u64 b65 = r65 < 64u ? (P.m65 >> r65) : P.msm;
+                        const bool pass = ((P.m64 >> (r1 & 63u)) & (P.m63 >> r63) & b65 & 1ull) != 0;
+                        if (pass) {
+                            const u32 ui = (u32)__builtin_fmin(u, 4294967295.0);   // roots are < 2^26
+                            if (ui > P.sqA && ui != P.f1 && ui != P.f2 && ui != P.f3 && ui != P.f4) {
+                                if (uniform) {
+                                    ++my_esq;
+                                } else {   // a tile with several intervals (rare): count where it belongs
+                                    const u32 r = a.quads[tile * EX_NT + ((W1 >> 10) & 63u)].rid;
+                                    atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
+                                }
+                                if (u <= (double)P.um) {
+                                    const u32 r2 = (u32)dmod(u, M2, 1.0 / M2);
+                                    const u64 msm = P.msm;
+                                    const u64 tt = (msm >> (1u + mod11(r2))) & (msm >> (12u + mod13(r2))) &
+                                                   (msm >> (25u + mod17(r2))) & (msm >> (42u + mod19(r2)));
+                                    if (tt & 1ull)
+                                        err |= ext_exact(u, a.quads, tile * EX_NT + ((W1 >> 10) & 63u), (W1 >> 16) & 7u,
+                                                         a.ctl, a.hits, a.hit_cap);
+                                }
+                            }
+                        }
                     }
                 }
-                if (sL == nfa) s_ar[loff + nL] = 1.0;   // empty H part: the list {1}
+                __syncthreads();
+                if (fits) pending = false;
+            }
+
+            // ---- counters of this tile
+            if (uniform) {
+                acc_cand += my_cand;
+                acc_esq += my_esq;
+                acc_md = my_maxdiv > acc_md ? my_maxdiv : acc_md;
+            } else if (valid) {
+                unsigned long long *c = (unsigned long long *)(a.ctr + (u64)qd.rid * CT_N);
+                atomicAdd(c + CT_EXT_CAND, (unsigned long long)my_cand);
+                atomicMax(c + CT_MAX_DIVS, (unsigned long long)my_maxdiv);
             }
             __syncthreads();
-            // lane l takes the candidates [l*ctot/64, (l+1)*ctot/64) of the pass
-            const u32 g0 = (tid * ctot) >> 6, g1 = ((tid + 1) * ctot) >> 6;
-            if (g0 < g1) {
-                u32 lo = 0, hi = EX_NT;
-                while (hi - lo > 1) {
-                    const u32 mid = (lo + hi) >> 1;
-                    if (s_pref[mid] <= g0) lo = mid; else hi = mid;
-                }
-                u32 k = lo;
-                u32 offL, cnL, offH, cnH, nc, kbest;
-                double scale, sqA, f1, f2, f3, f4, um;
-                u64 m64, m63, m65, msm;
-#define EX_LOAD_PARAMS()                                                                  \
-    do {                                                                                  \
-        const u32 W0 = s_w0[k], W1 = s_w1[k];                                             \
-        offL = W0 & 1023u; cnL = (W0 >> 10) & 1023u; offH = (W0 >> 20) & 1023u;           \
-        scale = (W0 >> 30) ? 0.5 : 1.0;                                                   \
-        cnH = W1 & 1023u; nc = (W1 >> 10) & 0x3FFFu; kbest = W1 >> 24;                   \
-        m64 = s_m64[k]; m63 = s_m63[k]; m65 = s_m65[k]; msm = s_msm[k];                   \
-        sqA = (double)s_sqA[k]; f1 = (double)s_f1[k]; f2 = (double)s_f2[k];               \
-        f3 = (double)s_f3[k]; f4 = (double)s_f4[k]; um = s_um[k];                         \
-    } while (0)
-                EX_LOAD_PARAMS();
-                u32 t = g0 - s_pref[k];
-                u32 j = t / cnL, i = t - j * cnL;
-                double hj = s_ar[offH + j], hjc = s_ar[offH + cnH - 1 - j];
-                u32 esq = 0;
-                for (u32 g = g0; g < g1; ++g) {
-                    const double f = s_ar[offL + i] * hj;
-                    const double u = __builtin_fma(s_ar[offL + cnL - 1 - i], hjc, f) * scale;
-                    const u32 r1 = (u32)dmod(u, M1, 1.0 / M1);
-                    const u32 r4095 = fold4095(r1);
-                    const u32 r63 = mod63(r4095), r65 = mod65(r4095);
-                    const u64 b65 = r65 < 64u ? (m65 >> r65) : msm;
-                    const bool pass = ((m64 >> (r1 & 63u)) & (m63 >> r63) & b65 & 1ull) != 0;
-                    if (pass && u > sqA && u != f1 && u != f2 && u != f3 && u != f4) {
-                        ++esq;
-                        if (u <= um) {
-                            const u32 r2 = (u32)dmod(u, M2, 1.0 / M2);
-                            const u64 tt = (msm >> (1u + mod11(r2))) & (msm >> (12u + mod13(r2))) &
-                                           (msm >> (25u + mod17(r2))) & (msm >> (42u + mod19(r2)));
-                            if (tt & 1ull)
-                                err |= ext_exact(u, a.quads, tile * EX_NT + k, kbest, a.ctl, a.hits, a.hit_cap);
-                        }
-                    }
-                    if (++t == nc) {   // this 4-set is done: on to the next one with candidates
-                        if (g + 1 < g1) {
-                            atomicAdd(&s_esq[k], esq);
-                            esq = 0;
-                            do { ++k; } while (k < EX_NT - 1 && s_pref[k + 1] == s_pref[k]);
-                            EX_LOAD_PARAMS();
-                            t = 0; i = 0; j = 0;
-                            hj = s_ar[offH];
-                            hjc = s_ar[offH + cnH - 1];
-                        }
-                    } else if (++i == cnL) {
-                        i = 0;
-                        ++j;
-                        hj = s_ar[offH + j];
-                        hjc = s_ar[offH + cnH - 1 - j];
-                    }
-                }
-#undef EX_LOAD_PARAMS
-                atomicAdd(&s_esq[k], esq);
-            }
-            __syncthreads();
-            if (fits) pending = false;
         }
-
-        // ---- counters of this tile (per interval)
-        const u64 my_esq = valid ? (u64)s_esq[tid] : 0ull;
-        const bool uniform = block_all<EX_NT>(!valid || my_rid == s_rid0, s_w) && s_rid0 < a.n_rids;
-        if (uniform) {
-            const u64 t_cand = block_sum_u64<EX_NT>(my_cand, s_w64);
-            const u64 t_esq = block_sum_u64<EX_NT>(my_esq, s_w64);
-            const u64 t_md = block_max_u64<EX_NT>(my_maxdiv, s_w64);
-            if (tid == 0) {
-                unsigned long long *c = (unsigned long long *)(a.ctr + (u64)s_rid0 * CT_N);
-                atomicAdd(c + CT_EXT_CAND, (unsigned long long)t_cand);
-                atomicAdd(c + CT_EXT_SQRT, (unsigned long long)t_esq);
-                atomicMax(c + CT_MAX_DIVS, (unsigned long long)t_md);
-            }
-        } else if (valid) {
-            unsigned long long *c = (unsigned long long *)(a.ctr + (u64)my_rid * CT_N);
-            atomicAdd(c + CT_EXT_CAND, (unsigned long long)my_cand);
-            atomicAdd(c + CT_EXT_SQRT, (unsigned long long)my_esq);
-            atomicMax(c + CT_MAX_DIVS, (unsigned long long)my_maxdiv);
-        }
-        if (err) atomicOr(&a.ctl->err, err);
-        __syncthreads();
+        chunk = uni(next);
     }
+    ex_flush(a.ctr, a.n_rids, acc_rid, acc_cand, acc_esq, acc_md);
+    if (err) atomicOr(&a.ctl->err, err);
 }
 
 // ----------------------------------------------------------------------------

@@ -2,7 +2,8 @@
 // (hip_host_shim.h) and compares every result with the CPU restatement.
 // Test infrastructure only.
 //
-//   ps4_emu [--noext] [--dump4 N] [--smax S] lo hi [lo hi ...]
+//   ps4_emu [--noext] [--dump4 N] [--smax S] [--batch] lo hi [lo hi ...]
+//   (--batch: all ranges in one ps4_search_batch call, so tiles mix intervals)
 //   ps4_emu --triple a b c dmax
 #include <algorithm>
 #include <cstdio>
@@ -41,12 +42,13 @@ int main(int argc, char **argv)
     long long dump4 = 0;
     unsigned long long smax = 0;
     std::vector<unsigned long long> nums;
-    bool triple = false;
+    bool triple = false, batch = false;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--noext")) flags |= PS4_NOEXT;
         else if (!strcmp(argv[i], "--dump4")) dump4 = atoll(argv[++i]);
         else if (!strcmp(argv[i], "--smax")) smax = strtoull(argv[++i], 0, 10);
         else if (!strcmp(argv[i], "--triple")) triple = true;
+        else if (!strcmp(argv[i], "--batch")) batch = true;
         else nums.push_back(strtoull(argv[i], 0, 10));
     }
     ps4_config cfg;
@@ -71,12 +73,27 @@ int main(int argc, char **argv)
         ps4_close(ctx);
         return ok ? 0 : 1;
     }
+    const size_t nr = nums.size() / 2;
+    std::vector<std::string> btext(nr);
+    std::vector<ps4_counters> bc(nr);
+    if (batch) {
+        std::vector<uint64_t> los(nr), his(nr);
+        std::vector<void *> users(nr);
+        for (size_t r = 0; r < nr; ++r) { los[r] = nums[2 * r]; his[r] = nums[2 * r + 1]; users[r] = &btext[r]; }
+        rc = ps4_search_batch(ctx, nr, los.data(), his.data(), sink, users.data(), bc.data());
+        if (rc) { printf("batch search rc=%d %s\n", rc, ps4_last_error(ctx)); ps4_close(ctx); return 1; }
+    }
     for (size_t i = 0; i + 1 < nums.size(); i += 2) {
         const unsigned long long lo = nums[i], hi = nums[i + 1];
         std::string got_text, want_text;
         ps4_counters c;
-        rc = ps4_search(ctx, lo, hi, sink, &got_text, &c);
-        if (rc) { printf("[%llu,%llu) search rc=%d %s\n", lo, hi, rc, ps4_last_error(ctx)); bad = 1; continue; }
+        if (batch) {
+            got_text = btext[i / 2];
+            c = bc[i / 2];
+        } else {
+            rc = ps4_search(ctx, lo, hi, sink, &got_text, &c);
+            if (rc) { printf("[%llu,%llu) search rc=%d %s\n", lo, hi, rc, ps4_last_error(ctx)); bad = 1; continue; }
+        }
         ora_counters w;
         int orc = ora_census(lo, hi, (flags & PS4_NOEXT) ? ORA_NOEXT : 0u, dump4, ora_line, &want_text, &w);
         if (orc) { printf("oracle rc=%d\n", orc); bad = 1; continue; }

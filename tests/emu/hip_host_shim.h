@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <math.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -106,6 +107,26 @@ static inline unsigned __shfl_xor(unsigned v, int m, int width = 64)
     const unsigned lane = threadIdx.x & 63;
     return (unsigned)emu_wave_exchange(v, lane ^ (unsigned)m);
 }
+static inline int __shfl(int v, int src, int width = 64)
+{
+    (void)width;
+    return (int)(uint32_t)emu_wave_exchange((uint32_t)v, (unsigned)src);
+}
+static inline unsigned long long __ballot(int p)
+{
+    emu::Block *b = emu::g_block;
+    const unsigned t = threadIdx.x, w = t / 64, base = w * 64;
+    const unsigned n = std::min(64u, b->nthreads - base);
+    b->xchg[t] = p ? 1u : 0u;
+    b->wave_bar[w].wait();
+    unsigned long long m = 0;
+    for (unsigned i = 0; i < n; ++i)
+        if (b->xchg[base + i]) m |= 1ull << i;
+    b->wave_bar[w].wait();
+    return m;
+}
+static inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+static inline int __clzll(long long x) { return x ? __builtin_clzll((unsigned long long)x) : 64; }
 static inline int __builtin_amdgcn_readfirstlane(int v) { return (int)emu_wave_exchange((uint32_t)v, 0); }
 static inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 static inline unsigned __umul24(unsigned a, unsigned b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }
