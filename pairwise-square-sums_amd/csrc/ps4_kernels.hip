@@ -505,6 +505,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 #define EX_MAXF 12        // distinct odd primes of D < 2^51 (3*5*...*43 > 2^51)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
 #define EX_CHUNK 4        // tiles claimed per work-queue atomic
+#define EX_QCAP 192       // exact-test queue entries per workgroup (drained at >= 64; an
+                          // iteration adds at most 128)
 
 struct ExArgs {
     const ps4_quad *quads;
@@ -516,22 +518,32 @@ struct ExArgs {
     const ps4_qrtab *qr;
     u64 *ctr;
     ps4_hit *hits;
+    u64 *exq;               // EX_QCAP entries per workgroup: u | record << 52
     u32 hit_cap;
     u32 cap_divs;
     u32 n_rids;             // counter rows in ctr
 };
 
-// Per-4-set parameters of the candidate loop (one record per 4-set of a pass,
-// 64 bytes, read as four 16-byte words).
-struct ExPar {
-    u32 w0;                 // offL | nL << 10 | offH << 20 | half << 30
-    u32 w1;                 // nH | lane << 10 | best << 16
-    float um;               // >= (min |C - A| + 1) / 2
-    u32 f4;                 // isqrt(2A) when 2A is a square, else 0
-    u64 m64, m63;           // residue masks mod 64, 63
-    u64 m65, msm;           // mod 65 (bits 0..63); bit 64 of mod 65 and the masks mod 11..19
-    u32 sqA, f1, f2, f3;    // isqrt(A) and the roots of pair sums containing A
-};
+// Per-4-set parameters of the candidate loop: one record of four 16-byte
+// words per 4-set of a pass (read together, before any branch):
+//   q0 = {offL | nL << 10 | offH << 20 | half << 30,  nH | lane << 10 | best << 16,
+//         um as float (>= (min |C - A| + 1) / 2),  isqrt(2A) when 2A is a square, else 0}
+//   q1 = {mask mod 64 (lo, hi),  mask mod 63 (lo, hi)}
+//   q2 = {mask mod 65 bits 0..63 (lo, hi),  msm (lo, hi)}: msm bit 0 is bit 64 of
+//         the mod-65 mask, then the masks mod 11 (bit 1), 13 (12), 17 (25), 19 (42)
+//   q3 = {isqrt(A), F1, F2, F3}: the roots of the pair sums containing A
+#define EX_PAR_WORDS 4
+
+// Takes values in registers at this point: an empty asm statement with them as
+// inputs, so the compiler issues their loads before it and cannot sink them
+// into branches of their own.  (The host emulation of these sources defines it
+// as nothing.)
+#ifndef PS4_KEEP
+#define PS4_KEEP(...) asm volatile("" ::__VA_ARGS__)
+#endif
+
+// bit r (< 64) of the 64-bit value hi:lo, as 0 or 1
+__device__ __forceinline__ u32 bit64(u32 lo, u32 hi, u32 r) { return ((r & 32u) ? hi : lo) >> (r & 31u) & 1u; }
 
 // x mod m for an integer-valued double |x| < 2^53; result in [0, m).
 __device__ __forceinline__ double dmod(double x, double m, double inv_m)
@@ -565,6 +577,17 @@ __device__ __forceinline__ u32 mod19(u32 s) { return s - 19u * ((s * 55189u) >> 
 
 #define M1 262080.0
 #define M2 46189.0
+
+// x mod M2 for an integer-valued double 0 <= x < 2^52 (the representative of
+// rep_m1's kind, then two conditional subtractions as unsigned minima).
+__device__ __forceinline__ u32 mod_m2(double x)
+{
+    const double q = __builtin_floor(x * (1.0 / M2));
+    u32 r = (u32)(__builtin_fma(-q, M2, x) + M2);   // [0, 3*M2)
+    r = r < r - 46189u ? r : r - 46189u;
+    r = r < r - 46189u ? r : r - 46189u;
+    return r;
+}
 
 // A representative in [0, 3*M1) of x mod M1 for an integer-valued double
 // 0 <= x < 2^52: the floor of the rounded quotient is within one of the true
@@ -646,6 +669,73 @@ __device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e,
     }
 }
 
+// One candidate: 4-set record k of the pass, candidate index t of that 4-set.
+// Straight-line: the record and the four list entries are loaded before the
+// tests (ex_rec, ex_lists), and every test is a select.  ok: the candidate counts for ext_sqrt (the residue
+// filters mod 64, 63, 65, e > 0, e not a member); go: it also passes the bound
+// u <= um and the residue filters mod 11, 13, 17, 19, so it gets the exact test.
+struct ExCand {
+    double u;
+    u32 w1;
+    bool ok, go;
+};
+
+struct ExRec {
+    uint4 q0, q1, q2, q3;
+};
+
+__device__ __forceinline__ ExRec ex_rec(const uint4 *s_par, u32 k)
+{
+    const uint4 *pw = s_par + k * EX_PAR_WORDS;
+    return ExRec{pw[0], pw[1], pw[2], pw[3]};
+}
+
+// The four list entries of candidate t: L[i], H[j], L[nL-1-i], H[nH-1-j].
+struct ExLists {
+    double l0, h0, l1, h1;
+};
+
+__device__ __forceinline__ ExLists ex_lists(const ExRec &R, const double *s_ar, u32 t)
+{
+    const u32 W0 = R.q0.x, W1 = R.q0.y;
+    const u32 offL = W0 & 1023u, cnL = (W0 >> 10) & 1023u, offH = (W0 >> 20) & 1023u;
+    const u32 cnH = W1 & 1023u;
+    // j = t / nL from a single-precision estimate within one of it (unsigned
+    // corrections: no signed arithmetic in the hot loop)
+    u32 j = (u32)((float)t * __builtin_amdgcn_rcpf((float)cnL));
+    u32 jn = j * cnL;
+    const bool over = jn > t;
+    j -= over ? 1u : 0u;
+    jn -= over ? cnL : 0u;
+    u32 i = t - jn;
+    const bool under = i >= cnL;
+    j += under ? 1u : 0u;
+    i -= under ? cnL : 0u;
+    return ExLists{s_ar[offL + i], s_ar[offH + j], s_ar[offL + cnL - 1 - i], s_ar[offH + cnH - 1 - j]};
+}
+
+__device__ __forceinline__ ExCand ex_cand(const ExRec &R, const ExLists &Ls)
+{
+    const uint4 q0 = R.q0, q1 = R.q1, q2 = R.q2, q3 = R.q3;
+    const u32 W0 = q0.x, W1 = q0.y;
+    const double s2 = __builtin_fma(Ls.l1, Ls.h1, Ls.l0 * Ls.h0);
+    ExCand c;
+    c.u = s2 * ((W0 >> 30) ? 0.5 : 1.0);
+    c.w1 = W1;
+    const u32 r1 = rep_m1(c.u);
+    const u32 r4095 = fold4095(r1);
+    const u32 r63 = mod63(r4095), r65 = mod65(r4095);
+    const u32 qrb = bit64(q1.x, q1.y, r1 & 63u) & bit64(q1.z, q1.w, r63) &
+                    (r65 < 64u ? bit64(q2.x, q2.y, r65) : (q2.z & 1u));
+    const u32 ui = (u32)__builtin_fmin(c.u, 4294967295.0);   // roots are < 2^26
+    c.ok = (qrb != 0u) & (ui > q3.x) & (ui != q3.y) & (ui != q3.z) & (ui != q3.w) & (ui != q0.w);
+    const u32 r2 = mod_m2(c.u);
+    const u32 m2b = bit64(q2.z, q2.w, 1u + mod11(r2)) & bit64(q2.z, q2.w, 12u + mod13(r2)) &
+                    bit64(q2.z, q2.w, 25u + mod17(r2)) & bit64(q2.z, q2.w, 42u + mod19(r2));
+    c.go = c.ok & (m2b != 0u) & (c.u <= (double)__uint_as_float(q0.z));
+    return c;
+}
+
 __device__ __forceinline__ u64 wave_sum_u64(u64 v)
 {
 #pragma unroll
@@ -698,8 +788,9 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
 {
     __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
     __shared__ u32 s_fac[EX_MAXF * EX_NT];     // odd prime factors of D: p << 5 | e, [slot][lane]
-    __shared__ ExPar s_par[EX_NT];             // parameters of the 4-sets of the current pass
-    __shared__ u32 s_bits[EX_CMAX / 32];       // bit g: a 4-set's candidates start at g
+    __shared__ uint4 s_par[EX_NT * EX_PAR_WORDS];   // parameters of the 4-sets of the current pass
+    __shared__ uint4 s_bits4[EX_CMAX / 128];   // bit g: a 4-set's candidates start at g
+    u32 *s_bits = (u32 *)s_bits4;
 
     const u32 tid = threadIdx.x;
     const u64 qc = a.ctl->quad_count;   // written by the previous kernel on this stream
@@ -708,6 +799,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
     const u64 ntiles = (nquads + EX_NT - 1) / EX_NT;
     const u64 nchunks = (ntiles + EX_CHUNK - 1) / EX_CHUNK;
     const ps4_qrtab *qr = a.qr;
+    u64 *exq = a.exq + (u64)blockIdx.x * EX_QCAP;
 
     // this wave's counters of interval acc_rid (lane partial sums)
     u32 acc_rid = 0xFFFFFFFFu;
@@ -886,18 +978,15 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                 (void)wave_excl_scan(fits ? ncand : 0u, &ctot);
                 // fitting 4-sets are a prefix of the pending ones: coff is each one's
                 // first candidate in the pass
-                for (u32 w = tid; w < (ctot + 31) / 32; w += EX_NT) s_bits[w] = 0u;
-                if (((ctot + 31) / 32) & 1u) s_bits[(ctot + 31) / 32] = 0u;   // the loop reads word pairs
+                for (u32 w = tid; w < (ctot + 127) / 128 * 4; w += EX_NT) s_bits[w] = 0u;   // whole 128-bit words
                 __syncthreads();
                 if (fits) {
-                    ExPar P;
-                    P.w0 = loff | (nL << 10) | ((loff + nL) << 20) | (half << 30);
-                    P.w1 = nH | (tid << 10) | (best << 16);
-                    P.um = pum;
-                    P.f4 = pf4;
-                    P.m64 = pm64; P.m63 = pm63; P.m65 = pm65; P.msm = pmsm;
-                    P.sqA = psqA; P.f1 = pf1; P.f2 = pf2; P.f3 = pf3;
-                    s_par[slot] = P;
+                    uint4 *pw = s_par + slot * EX_PAR_WORDS;
+                    pw[0] = make_uint4(loff | (nL << 10) | ((loff + nL) << 20) | (half << 30), nH | (tid << 10) | (best << 16),
+                                       __float_as_uint(pum), pf4);
+                    pw[1] = make_uint4((u32)pm64, (u32)(pm64 >> 32), (u32)pm63, (u32)(pm63 >> 32));
+                    pw[2] = make_uint4((u32)pm65, (u32)(pm65 >> 32), (u32)pmsm, (u32)(pmsm >> 32));
+                    pw[3] = make_uint4(psqA, pf1, pf2, pf3);
                     atomicOr(&s_bits[coff >> 5], 1u << (coff & 31u));
                     // mixed-radix lists over the factor sequence: stage s multiplies
                     // the list so far by p_s^1 .. p_s^e (earlier stages are the less
@@ -931,56 +1020,75 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                     if (sL == nfa) s_ar[loff + nL] = 1.0;   // empty H part: the list {1}
                 }
                 __syncthreads();
-                // lane l takes candidates l, l+64, ... of the pass
+                // lane l takes candidates l, l+64, l+128, ... of the pass, two per
+                // iteration (g and g + 64: two independent load/compute chains)
                 u32 kp = 0, sprev = 0;   // 4-set starts before base; start of the 4-set holding base-1
-                for (u32 base = 0; base < ctot; base += 64) {
-                    const u32 wi = base >> 5;
-                    const u64 word = (u64)s_bits[wi] | ((u64)s_bits[wi + 1] << 32);
-                    const u32 g = base + tid;
-                    const u64 below = word & ((2ull << tid) - 1ull);   // starts in [base, g]
-                    const u32 c = (u32)__popcll(below);
-                    const u32 k = kp + c - 1u;
-                    const u32 st = c ? base + 63u - (u32)__clzll((long long)below) : sprev;
-                    kp += (u32)__popcll(word);
-                    if (word) sprev = base + 63u - (u32)__clzll((long long)word);
-                    if (g < ctot) {
-                        const ExPar &P = s_par[k];
-                        const u32 W0 = P.w0, W1 = P.w1;
-                        const u32 offL = W0 & 1023u, cnL = (W0 >> 10) & 1023u, offH = (W0 >> 20) & 1023u;
-                        const u32 cnH = W1 & 1023u;
-                        const u32 t = g - st;
-                        int j = (int)((float)t * __builtin_amdgcn_rcpf((float)cnL));   // within one of t / nL
-                        int i = (int)t - j * (int)cnL;
-                        if (i < 0) { --j; i += (int)cnL; }
-                        else if (i >= (int)cnL) { ++j; i -= (int)cnL; }
-                        const double f = s_ar[offL + i] * s_ar[offH + j];
-                        const double uu2 = __builtin_fma(s_ar[offL + cnL - 1 - i], s_ar[offH + cnH - 1 - j], f);
-                        const double u = (W0 >> 30) ? uu2 * 0.5 : uu2;
-                        const u32 r1 = rep_m1(u);
-                        const u32 r4095 = fold4095(r1);
-                        const u32 r63 = mod63(r4095), r65 = mod65(r4095);
-                        const u64 b65 = r65 < 64u ? (P.m65 >> r65) : P.msm;
-                        const bool pass = ((P.m64 >> (r1 & 63u)) & (P.m63 >> r63) & b65 & 1ull) != 0;
-                        if (pass) {
-                            const u32 ui = (u32)__builtin_fmin(u, 4294967295.0);   // roots are < 2^26
-                            if (ui > P.sqA && ui != P.f1 && ui != P.f2 && ui != P.f3 && ui != P.f4) {
-                                if (uniform) {
-                                    ++my_esq;
-                                } else {   // a tile with several intervals (rare): count where it belongs
-                                    const u32 r = a.quads[tile * EX_NT + ((W1 >> 10) & 63u)].rid;
-                                    atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
-                                }
-                                if (u <= (double)P.um) {
-                                    const u32 r2 = (u32)dmod(u, M2, 1.0 / M2);
-                                    const u64 msm = P.msm;
-                                    const u64 tt = (msm >> (1u + mod11(r2))) & (msm >> (12u + mod13(r2))) &
-                                                   (msm >> (25u + mod17(r2))) & (msm >> (42u + mod19(r2)));
-                                    if (tt & 1ull)
-                                        err |= ext_exact(u, a.quads, tile * EX_NT + ((W1 >> 10) & 63u), (W1 >> 16) & 7u,
-                                                         a.ctl, a.hits, a.hit_cap);
-                                }
+                const u64 lane_mask = (2ull << tid) - 1ull;   // bits 0..tid
+                u32 qn = 0;                                   // entries in the exact-test queue
+                for (u32 base = 0; base < ctot; base += 128) {
+                    const uint4 wv = s_bits4[base >> 7];
+                    const u64 wa = (u64)wv.x | ((u64)wv.y << 32), wb = (u64)wv.z | ((u64)wv.w << 32);
+                    const u32 ga = base + tid, gb = ga + 64u;
+                    const u64 ba = wa & lane_mask, bb = wb & lane_mask;   // starts in [base, ga], [base+64, gb]
+                    const u32 ca = (u32)__popcll(ba), cb = (u32)__popcll(bb);
+                    const u32 ka = kp + ca - 1u;
+                    const u32 sa = ca ? base + 63u - (u32)__clzll((long long)ba) : sprev;
+                    kp += (u32)__popcll(wa);
+                    if (wa) sprev = base + 63u - (u32)__clzll((long long)wa);
+                    const u32 kb = kp + cb - 1u;
+                    const u32 sb = cb ? base + 127u - (u32)__clzll((long long)bb) : sprev;
+                    kp += (u32)__popcll(wb);
+                    if (wb) sprev = base + 127u - (u32)__clzll((long long)wb);
+                    // past the end of the pass a lane evaluates candidate 0 of record 0
+                    // (in range) and discards it
+                    const bool va = ga < ctot, vb = gb < ctot;
+                    const ExRec RA = ex_rec(s_par, va ? ka : 0u), RB = ex_rec(s_par, vb ? kb : 0u);
+                    PS4_KEEP("v"(RA.q1.x), "v"(RA.q1.y), "v"(RA.q1.z), "v"(RA.q1.w), "v"(RA.q2.x), "v"(RA.q2.y),
+                             "v"(RA.q2.z), "v"(RA.q2.w), "v"(RA.q3.x), "v"(RA.q3.y), "v"(RA.q3.z), "v"(RA.q3.w),
+                             "v"(RA.q0.z), "v"(RA.q0.w), "v"(RB.q1.x), "v"(RB.q1.y), "v"(RB.q1.z), "v"(RB.q1.w),
+                             "v"(RB.q2.x), "v"(RB.q2.y), "v"(RB.q2.z), "v"(RB.q2.w), "v"(RB.q3.x), "v"(RB.q3.y),
+                             "v"(RB.q3.z), "v"(RB.q3.w), "v"(RB.q0.z), "v"(RB.q0.w));   // both records
+                    const ExLists LA = ex_lists(RA, s_ar, va ? ga - sa : 0u), LB = ex_lists(RB, s_ar, vb ? gb - sb : 0u);
+                    PS4_KEEP("v"(LA.l0), "v"(LA.h0), "v"(LA.l1), "v"(LA.h1), "v"(LB.l0), "v"(LB.h0), "v"(LB.l1),
+                             "v"(LB.h1));   // both candidates' list loads in flight together
+                    const ExCand A = ex_cand(RA, LA);
+                    const ExCand B = ex_cand(RB, LB);
+                    const bool oka = va & A.ok, okb = vb & B.ok;
+                    if (uniform) {
+                        my_esq += (oka ? 1u : 0u) + (okb ? 1u : 0u);
+                    } else {   // a tile with several intervals (rare): count where each belongs
+                        if (oka) {
+                            const u32 r = a.quads[tile * EX_NT + ((A.w1 >> 10) & 63u)].rid;
+                            atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
+                        }
+                        if (okb) {
+                            const u32 r = a.quads[tile * EX_NT + ((B.w1 >> 10) & 63u)].rid;
+                            atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
+                        }
+                    }
+                    // candidates that passed every filter wait in the queue for the
+                    // exact test, so that it runs with all lanes busy
+                    const bool goa = va & A.go, gob = vb & B.go;
+                    const u64 lane_lt = lane_mask >> 1;   // bits 0..tid-1
+                    const u64 bga = __ballot(goa), bgb = __ballot(gob);
+                    if (goa) exq[qn + (u32)__popcll(bga & lane_lt)] = (u64)A.u | ((u64)ka << 52);
+                    qn += (u32)__popcll(bga);
+                    if (gob) exq[qn + (u32)__popcll(bgb & lane_lt)] = (u64)B.u | ((u64)kb << 52);
+                    qn += (u32)__popcll(bgb);
+                    if (qn >= 64u || (base + 128u >= ctot && qn > 0u)) {
+                        __syncthreads();   // the queue's stores are visible to every lane
+                        for (u32 q0 = 0; q0 < qn; q0 += EX_NT) {
+                            if (q0 + tid < qn) {
+                                const u64 ent = exq[q0 + tid];
+                                const u32 kq = (u32)(ent >> 52);
+                                const u32 W1 = s_par[kq * EX_PAR_WORDS].y;
+                                err |= ext_exact((double)(ent & ((1ull << 52) - 1ull)), a.quads,
+                                                 tile * EX_NT + ((W1 >> 10) & 63u), (W1 >> 16) & 7u, a.ctl, a.hits,
+                                                 a.hit_cap);
                             }
                         }
+                        __syncthreads();   // read before the next stores
+                        qn = 0;
                     }
                 }
                 __syncthreads();
@@ -1024,13 +1132,13 @@ extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *it
 
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
                                      const u64 *ftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
-                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs)
+                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq)
 {
     ExArgs a;
     a.n_rids = n_rids;
     a.quads = quads; a.quad_cap = quad_cap; a.ctl = ctl; a.ftab = ftab; a.tab_n = tab_n; a.ndiv = ndiv; a.qr = qr;
     a.ctr = ctr;
-    a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs;
+    a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs; a.exq = exq;
     hipLaunchKernelGGL(k_ext, dim3(grid), dim3(EX_NT), 0, st, a);
     return hipGetLastError();
 }
@@ -1056,6 +1164,7 @@ extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u
 
 extern "C" int ps4_rq_threads(void) { return RQ_NT; }
 extern "C" int ps4_ext_threads(void) { return EX_NT; }
+extern "C" int ps4_ext_queue_cap(void) { return EX_QCAP; }
 extern "C" int ps4_rq_window(void) { return (int)RQ_W; }
 extern "C" int ps4_rq_nb_max(void) { return RQ_NB_MAX; }
 

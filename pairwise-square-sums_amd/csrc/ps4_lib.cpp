@@ -30,13 +30,14 @@ extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *it
                                     u64 quad_cap, u32 cap_reps, u32 verify);
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
                                      const u64 *ftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
-                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs);
+                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq);
 extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u32 n_ctr);
 extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u64 d_max, const u64 *ftab,
                                         u64 tab_n, u64 *d_out, u32 cap, u32 *count, u32 *err);
 extern "C" hipError_t ps4_launch_fill_u32(hipStream_t st, u32 *p, u32 n, u32 v);
 extern "C" int ps4_rq_threads(void);
 extern "C" int ps4_ext_threads(void);
+extern "C" int ps4_ext_queue_cap(void);
 extern "C" int ps4_rq_window(void);
 extern "C" int ps4_rq_nb_max(void);
 extern "C" void *ps4_k_rq_ptr(void);
@@ -162,6 +163,7 @@ struct ps4_ctx {
     u64 *d_scratch = nullptr;
     ps4_quad *d_quads = nullptr;
     ps4_hit *d_hits = nullptr;
+    u64 *d_exq = nullptr;      // per-workgroup exact-test queues of the extension kernel
     // pinned host
     ps4_ctl *h_ctl = nullptr;
     u64 *h_ctr = nullptr;
@@ -402,7 +404,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         if (!noext)
             HIP_OK(ps4_launch_ext(ctx->st, ctx->ext_grid, ctx->d_quads, ctx->quad_cap, ctx->d_ctl, ctx->d_ftab,
                                   ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, rid_hi + 1, ctx->d_hits,
-                                  ctx->hit_cap, ctx->cap_divs));
+                                  ctx->hit_cap, ctx->cap_divs, ctx->d_exq));
         HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
         if (!noext)
             if (int rc = dbg("k_ext")) return rc;
@@ -594,7 +596,7 @@ extern "C" void ps4_close(ps4_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     void *dev[] = {ctx->d_ftab, ctx->d_ndiv, ctx->d_qr, ctx->d_ctl, ctx->d_ctr, ctx->d_items, ctx->d_scratch,
-                   ctx->d_quads, ctx->d_hits, ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
+                   ctx->d_quads, ctx->d_hits, ctx->d_exq, ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
     for (void *p : dev) if (p) (void)hipFree(p);
     void *host[] = {ctx->h_ctl, ctx->h_ctr, ctx->h_items, ctx->h_hits};
     for (void *p : host) if (p) (void)hipHostFree(p);
@@ -653,6 +655,7 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     HIP_OK(hipHostMalloc(&ctx->h_items, (size_t)ctx->items_cap * sizeof(ps4_item)));
     HIP_OK(hipMalloc(&ctx->d_quads, (size_t)ctx->quad_cap * sizeof(ps4_quad)));
     HIP_OK(hipMalloc(&ctx->d_hits, (size_t)ctx->hit_cap * sizeof(ps4_hit)));
+    HIP_OK(hipMalloc(&ctx->d_exq, (size_t)ctx->ext_grid * (size_t)ps4_ext_queue_cap() * sizeof(u64)));
     HIP_OK(hipHostMalloc(&ctx->h_hits, (size_t)ctx->hit_cap * sizeof(ps4_hit)));
     if (int rc = alloc_scratch(ctx)) return rc;
     if (int rc = ensure_ctr(ctx, 64)) return rc;

@@ -28,6 +28,7 @@
 #define __forceinline__ inline
 #define __noinline__ __attribute__((noinline))
 #define __launch_bounds__(...)
+#define PS4_KEEP(...) ((void)0)
 // LDS: every __shared__ array lives in one section that is filled with garbage
 // at each workgroup launch (real LDS starts with arbitrary contents).
 #define __shared__ static __attribute__((section("emu_lds")))
@@ -125,6 +126,8 @@ static inline unsigned long long __ballot(int p)
     b->wave_bar[w].wait();
     return m;
 }
+static inline unsigned __float_as_uint(float f) { unsigned u; memcpy(&u, &f, 4); return u; }
+static inline float __uint_as_float(unsigned u) { float f; memcpy(&f, &u, 4); return f; }
 static inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 static inline int __clzll(long long x) { return x ? __builtin_clzll((unsigned long long)x) : 64; }
 static inline int __builtin_amdgcn_readfirstlane(int v) { return (int)emu_wave_exchange((uint32_t)v, 0); }
