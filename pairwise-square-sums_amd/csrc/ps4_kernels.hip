@@ -64,14 +64,18 @@ __device__ __forceinline__ u32 lane_id() { return threadIdx.x & 63u; }
 // scalar and the compiler cannot split them across lanes.
 __device__ __forceinline__ u32 uni(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
 
+// Inclusive prefix sum over the 64 lanes of a wave (all lanes active): four
+// row_shr steps inside each 16-lane row, then the row totals by row_bcast:15
+// (into rows 1 and 3) and row_bcast:31 (into rows 2 and 3).  Each step is one
+// VALU add with a DPP source; a lane without a source adds 0.
 __device__ __forceinline__ u32 wave_incl_scan(u32 x)
 {
-    const u32 lane = lane_id();
-#pragma unroll
-    for (u32 d = 1; d < 64; d <<= 1) {
-        const u32 y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);   // row_shr:1
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);   // row_shr:2
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);   // row_shr:4
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);   // row_shr:8
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);   // row_bcast:15
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);   // row_bcast:31
     return x;
 }
 
@@ -752,7 +756,7 @@ __device__ __forceinline__ u64 wave_max_u64(u64 v)
 __device__ __forceinline__ u32 wave_excl_scan(u32 v, u32 *total)
 {
     const u32 incl = wave_incl_scan(v);
-    *total = uni((u32)__shfl((int)incl, 63, 64));
+    *total = (u32)__builtin_amdgcn_readlane((int)incl, 63);
     return incl - v;
 }
 

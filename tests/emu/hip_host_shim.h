@@ -130,6 +130,29 @@ static inline unsigned __float_as_uint(float f) { unsigned u; memcpy(&u, &f, 4);
 static inline float __uint_as_float(unsigned u) { float f; memcpy(&f, &u, 4); return f; }
 static inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 static inline int __clzll(long long x) { return x ? __builtin_clzll((unsigned long long)x) : 64; }
+// DPP moves used by the kernels: row_shr:1..15 (0x111..0x11F) inside 16-lane
+// rows, row_bcast:15 (0x142) and row_bcast:31 (0x143); rows outside row_mask and
+// lanes without a source return `old` (the bank mask is always 0xF here).
+static inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl)
+{
+    (void)bank_mask; (void)bound_ctrl;
+    const unsigned lane = threadIdx.x & 63, row = lane >> 4;
+    int from = -1;
+    if (ctrl >= 0x111 && ctrl <= 0x11F) {
+        const unsigned n = (unsigned)ctrl - 0x110;
+        if ((lane & 15) >= n) from = (int)(lane - n);
+    } else if (ctrl == 0x142) {
+        if (row >= 1) from = (int)(row * 16 - 1);
+    } else if (ctrl == 0x143) {
+        if (row >= 2) from = 31;
+    } else {
+        abort();
+    }
+    const int v = (int)(uint32_t)emu_wave_exchange((uint32_t)src, from < 0 ? lane : (unsigned)from);
+    if (!((row_mask >> row) & 1) || from < 0) return old;
+    return v;
+}
+static inline int __builtin_amdgcn_readlane(int v, int l) { return (int)(uint32_t)emu_wave_exchange((uint32_t)v, (unsigned)l); }
 static inline int __builtin_amdgcn_readfirstlane(int v) { return (int)emu_wave_exchange((uint32_t)v, 0); }
 static inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 static inline unsigned __umul24(unsigned a, unsigned b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }
