@@ -509,8 +509,11 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 #define EX_MAXF 12        // distinct odd primes of D < 2^51 (3*5*...*43 > 2^51)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
 #define EX_CHUNK 4        // tiles claimed per work-queue atomic
-#define EX_QCAP 192       // exact-test queue entries per workgroup (drained at >= 64; an
-                          // iteration adds at most 128)
+#define EX_Q1 192         // queue of candidates counted for ext_sqrt, per workgroup (drained
+                          // at >= 64; an iteration adds at most 128)
+#define EX_Q2 128         // queue of candidates for the exact test (drained at >= 64; a
+                          // drain round of the first queue adds at most 64)
+#define EX_QCAP (EX_Q1 + EX_Q2)
 
 struct ExArgs {
     const ps4_quad *quads;
@@ -522,7 +525,7 @@ struct ExArgs {
     const ps4_qrtab *qr;
     u64 *ctr;
     ps4_hit *hits;
-    u64 *exq;               // EX_QCAP entries per workgroup: u | record << 52
+    u64 *exq;               // EX_QCAP entries per workgroup (both queues): u | record << 52
     u32 hit_cap;
     u32 cap_divs;
     u32 n_rids;             // counter rows in ctr
@@ -547,7 +550,7 @@ struct ExArgs {
 #endif
 
 // bit r (< 64) of the 64-bit value hi:lo, as 0 or 1
-__device__ __forceinline__ u32 bit64(u32 lo, u32 hi, u32 r) { return ((r & 32u) ? hi : lo) >> (r & 31u) & 1u; }
+__device__ __forceinline__ u32 bit64(u32 lo, u32 hi, u32 r) { return (u32)((((u64)hi << 32) | lo) >> r) & 1u; }
 
 // x mod m for an integer-valued double |x| < 2^53; result in [0, m).
 __device__ __forceinline__ double dmod(double x, double m, double inv_m)
@@ -675,13 +678,13 @@ __device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e,
 
 // One candidate: 4-set record k of the pass, candidate index t of that 4-set.
 // Straight-line: the record and the four list entries are loaded before the
-// tests (ex_rec, ex_lists), and every test is a select.  ok: the candidate counts for ext_sqrt (the residue
-// filters mod 64, 63, 65, e > 0, e not a member); go: it also passes the bound
-// u <= um and the residue filters mod 11, 13, 17, 19, so it gets the exact test.
+// tests (ex_rec, ex_lists), and every test is a select.  ok: the candidate
+// counts for ext_sqrt (the residue filters mod 64, 63, 65, e > 0, e not a
+// member).  The later filters (ex_late) run on the queued ok candidates.
 struct ExCand {
     double u;
     u32 w1;
-    bool ok, go;
+    bool ok;
 };
 
 struct ExRec {
@@ -733,11 +736,18 @@ __device__ __forceinline__ ExCand ex_cand(const ExRec &R, const ExLists &Ls)
                     (r65 < 64u ? bit64(q2.x, q2.y, r65) : (q2.z & 1u));
     const u32 ui = (u32)__builtin_fmin(c.u, 4294967295.0);   // roots are < 2^26
     c.ok = (qrb != 0u) & (ui > q3.x) & (ui != q3.y) & (ui != q3.z) & (ui != q3.w) & (ui != q0.w);
-    const u32 r2 = mod_m2(c.u);
+    return c;
+}
+
+// The later filters of a counted candidate u of record word q0 and q2: the
+// size bound u <= (min |C - A| + 1) / 2 and the residue masks mod 11, 13, 17,
+// 19 (e + C1 and e + C2 both squares mod each).  True: it needs the exact test.
+__device__ __forceinline__ bool ex_late(double u, const uint4 &q0, const uint4 &q2)
+{
+    const u32 r2 = mod_m2(u);
     const u32 m2b = bit64(q2.z, q2.w, 1u + mod11(r2)) & bit64(q2.z, q2.w, 12u + mod13(r2)) &
                     bit64(q2.z, q2.w, 25u + mod17(r2)) & bit64(q2.z, q2.w, 42u + mod19(r2));
-    c.go = c.ok & (m2b != 0u) & (c.u <= (double)__uint_as_float(q0.z));
-    return c;
+    return (m2b != 0u) & (u <= (double)__uint_as_float(q0.z));
 }
 
 __device__ __forceinline__ u64 wave_sum_u64(u64 v)
@@ -803,7 +813,8 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
     const u64 ntiles = (nquads + EX_NT - 1) / EX_NT;
     const u64 nchunks = (ntiles + EX_CHUNK - 1) / EX_CHUNK;
     const ps4_qrtab *qr = a.qr;
-    u64 *exq = a.exq + (u64)blockIdx.x * EX_QCAP;
+    u64 *exq1 = a.exq + (u64)blockIdx.x * EX_QCAP;   // counted candidates
+    u64 *exq2 = exq1 + EX_Q1;                        // candidates for the exact test
 
     // this wave's counters of interval acc_rid (lane partial sums)
     u32 acc_rid = 0xFFFFFFFFu;
@@ -1028,7 +1039,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                 // iteration (g and g + 64: two independent load/compute chains)
                 u32 kp = 0, sprev = 0;   // 4-set starts before base; start of the 4-set holding base-1
                 const u64 lane_mask = (2ull << tid) - 1ull;   // bits 0..tid
-                u32 qn = 0;                                   // entries in the exact-test queue
+                u32 qn1 = 0, qn2 = 0;                         // entries in the two queues
                 for (u32 base = 0; base < ctot; base += 128) {
                     const uint4 wv = s_bits4[base >> 7];
                     const u64 wa = (u64)wv.x | ((u64)wv.y << 32), wb = (u64)wv.z | ((u64)wv.w << 32);
@@ -1070,29 +1081,49 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                             atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
                         }
                     }
-                    // candidates that passed every filter wait in the queue for the
-                    // exact test, so that it runs with all lanes busy
-                    const bool goa = va & A.go, gob = vb & B.go;
+                    // counted candidates wait in a queue for the later filters, and
+                    // the survivors in a second one for the exact test, so that both
+                    // run with all lanes busy
                     const u64 lane_lt = lane_mask >> 1;   // bits 0..tid-1
-                    const u64 bga = __ballot(goa), bgb = __ballot(gob);
-                    if (goa) exq[qn + (u32)__popcll(bga & lane_lt)] = (u64)A.u | ((u64)ka << 52);
-                    qn += (u32)__popcll(bga);
-                    if (gob) exq[qn + (u32)__popcll(bgb & lane_lt)] = (u64)B.u | ((u64)kb << 52);
-                    qn += (u32)__popcll(bgb);
-                    if (qn >= 64u || (base + 128u >= ctot && qn > 0u)) {
+                    const u64 boa = __ballot(oka), bob = __ballot(okb);
+                    if (oka) exq1[qn1 + (u32)__popcll(boa & lane_lt)] = (u64)A.u | ((u64)ka << 52);
+                    qn1 += (u32)__popcll(boa);
+                    if (okb) exq1[qn1 + (u32)__popcll(bob & lane_lt)] = (u64)B.u | ((u64)kb << 52);
+                    qn1 += (u32)__popcll(bob);
+                    const bool last = base + 128u >= ctot;
+                    // exact tests of every entry of the second queue
+                    auto drain2 = [&]() {
                         __syncthreads();   // the queue's stores are visible to every lane
-                        for (u32 q0 = 0; q0 < qn; q0 += EX_NT) {
-                            if (q0 + tid < qn) {
-                                const u64 ent = exq[q0 + tid];
-                                const u32 kq = (u32)(ent >> 52);
-                                const u32 W1 = s_par[kq * EX_PAR_WORDS].y;
-                                err |= ext_exact((double)(ent & ((1ull << 52) - 1ull)), a.quads,
+                        for (u32 q = 0; q < qn2; q += EX_NT) {
+                            if (q + tid < qn2) {
+                                const u64 e2 = exq2[q + tid];
+                                const u32 W1 = s_par[(u32)(e2 >> 52) * EX_PAR_WORDS].y;
+                                err |= ext_exact((double)(e2 & ((1ull << 52) - 1ull)), a.quads,
                                                  tile * EX_NT + ((W1 >> 10) & 63u), (W1 >> 16) & 7u, a.ctl, a.hits,
                                                  a.hit_cap);
                             }
                         }
                         __syncthreads();   // read before the next stores
-                        qn = 0;
+                        qn2 = 0;
+                    };
+                    if (qn1 >= 64u || (last && (qn1 | qn2) != 0u)) {
+                        __syncthreads();   // the queue's stores are visible to every lane
+                        for (u32 q0 = 0; q0 < qn1; q0 += EX_NT) {
+                            bool go = false;
+                            u64 ent = 0;
+                            if (q0 + tid < qn1) {
+                                ent = exq1[q0 + tid];
+                                const uint4 *pw = s_par + (u32)(ent >> 52) * EX_PAR_WORDS;
+                                go = ex_late((double)(ent & ((1ull << 52) - 1ull)), pw[0], pw[2]);
+                            }
+                            const u64 bg = __ballot(go);
+                            if (go) exq2[qn2 + (u32)__popcll(bg & lane_lt)] = ent;
+                            qn2 += (u32)__popcll(bg);
+                            if (qn2 >= 64u) drain2();
+                        }
+                        if (last && qn2 > 0u) drain2();
+                        __syncthreads();   // read before the next stores
+                        qn1 = 0;
                     }
                 }
                 __syncthreads();
