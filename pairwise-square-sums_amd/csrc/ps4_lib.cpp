@@ -77,7 +77,9 @@ std::string dec(u128 v)
 {
     char tmp[48];
     int n = 0;
-    do { tmp[n++] = (char)('0' + (int)(v % 10)); v /= 10; } while (v);
+    while (v >> 64) { tmp[n++] = (char)('0' + (int)(v % 10)); v /= 10; }   // 128-bit digits (rare)
+    u64 w = (u64)v;
+    do { tmp[n++] = (char)('0' + (int)(w % 10)); w /= 10; } while (w);     // 64-bit digits
     std::string s;
     while (n) s.push_back(tmp[--n]);
     return s;
@@ -639,9 +641,10 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     while ((u64)ctx->W * nb < 2 * isqrt64(ctx->S_hi_max) + 1 && nb < 256) nb <<= 1;
     ctx->nb_alloc = std::min<u32>(nb, (u32)ps4_rq_nb_max());
     ctx->bin_cap = ctx->W / 2 + 1024;
-    // 4-set queue: enough for the whole range when it is small, else 2^28 entries (8 GiB)
+    // 4-set queue: enough for the whole range when it is small, else 2^30 entries
+    // (32 GiB, a few batches per 10^10 S), at most 40% of the free device memory
     const double whole = quad_density(ctx->S_hi_max) * (double)ctx->S_hi_max * 1.5 + 65536.0;
-    u64 qcap = env_u64("PS4_QUAD_CAP", 1ull << 28);
+    u64 qcap = env_u64("PS4_QUAD_CAP", 1ull << 30);
     if ((double)qcap > whole) qcap = (u64)whole;
     size_t fr = 0, tot = 0;
     HIP_OK(hipMemGetInfo(&fr, &tot));
