@@ -8,6 +8,22 @@ is the input; nothing is cached between steps.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
                     [--lo 1] [--hi 10000000000] [--no-cpu] [--no-latency]
+    python bench.py --workload batched|band|serving
+
+The default workload is the plain census above.  The others print their own
+JSON line:
+  batched  4096 subranges of [1,1e11], lengths log-uniform in [1e4,1e9]
+           (splitmix64 seed 42): all in one ps4_search_batch against one
+           ps4_search per subrange; per-subrange results must be identical.
+  band     [9e11, 1e12) as 1000 subranges of 1e8 in one batched call; whole-
+           band counters against the reference's own logs
+           (exhaustive/logs/r12_9.err + boundary_9.err), and a seeded sample of
+           subranges bit-exact against the reference s4 run on this host.
+  serving  an open-loop Poisson stream (200 req/s, seed 7; 70% range queries
+           lo in [1,1e12], length log-uniform in [1e3,1e8]; 30% triple queries
+           c <= 1e9, d <= 1e12) through the continuous-batching request loop;
+           p50/p99 latency, throughput, and a sample of requests checked
+           against the same requests run alone.
 
 Also reported:
   roofline      the extension kernel (the dominant one) against the integer
@@ -22,6 +38,7 @@ Also reported:
                 answered one at a time.
 """
 import argparse
+import hashlib
 import json
 import math
 import os
@@ -96,6 +113,190 @@ def latency(eng, hi, n=100, seed=7):
             "lengths": "log-uniform 1e3..1e8", "lo": f"uniform in [1,{hi})"}
 
 
+def lines_sha(lines):
+    return hashlib.sha256("".join(sorted(x + "\n" for x in lines)).encode()).hexdigest()
+
+
+def s4_ref_run(lo, hi, threads):
+    """The reference s4 (oracle/_ref/s4, built from exhaustive/s4.c) on [lo, hi):
+    (counters dict, sorted-lines sha) or None when it is not built."""
+    s4 = os.path.join(REPO, "oracle", "_ref", "s4")
+    if not os.path.exists(s4):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        outp = os.path.join(tmp, "o.out")
+        r = subprocess.run([s4, str(lo), str(hi), str(threads), "20", "--out", outp], cwd=tmp,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, timeout=900)
+        if r.returncode != 0:
+            return None
+        tot = re.search(rb"^TOTAL: (.*)$", r.stderr, re.M).group(1).decode()
+        kv = {k: int(v) for k, v in re.findall(r"([\w-]+)=(\d+)", tot)}
+        import ps4
+        counters = {k: kv[k] for k in ps4.TOTAL_KEYS}
+        lines = open(outp).read().splitlines()
+    return counters, lines_sha(lines)
+
+
+def add_counters(acc, c):
+    for k, v in c.items():
+        acc[k] = acc.get(k, 0) + v
+    return acc
+
+
+def bench_batched(args):
+    import ps4
+    from ps4 import workloads
+    ranges = workloads.batched_ranges(4096, seed=42, top=10**11)
+    los, his = [a for a, _ in ranges], [b for _, b in ranges]
+    total = sum(b - a for a, b in ranges)
+    eng = ps4.Engine(S_hi_max=10**11 + 1)
+    eng.search(1, 10**6)                     # warm the device
+    log(f"batched: {len(ranges)} subranges, {total:.3e} S in one ps4_search_batch")
+    eng.kernel_time(reset=True)
+    t0 = time.perf_counter()
+    res_b = eng.search_batch(los, his)
+    t_b = time.perf_counter() - t0
+    kb = eng.kernel_time(reset=True)
+    log(f"batched: {t_b:.2f} s; plain: one ps4_search per subrange")
+    t0 = time.perf_counter()
+    res_p = [eng.search(a, b) for a, b in ranges]
+    t_p = time.perf_counter() - t0
+    kp = eng.kernel_time(reset=True)
+    eng.close()
+    same = all(x[0] == y[0] and sorted(x[1]) == sorted(y[1]) for x, y in zip(res_b, res_p))
+    tot = {}
+    for c, _ in res_b:
+        add_counters(tot, c["counters"])
+    return {
+        "metric": "elements/sec, 4096 uneven subranges of [1,1e11] in one batched call vs one call per subrange",
+        "value": total / t_b, "unit": "S/s", "n_gpus": 1, "higher_is_better": True, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "batched: 4096 subranges [lo,lo+len) in [1,1e11], len = floor(10^(4+5U)), "
+                               "lo = 1+floor(U'(1e11-len)), U,U' from splitmix64 seed 42",
+                   "subranges": len(ranges), "S_total": total},
+        "seconds_batched": t_b, "seconds_plain": t_p, "plain_value": total / t_p, "batched_over_plain": t_p / t_b,
+        "per_subrange_identical": same, "counters": tot,
+        "kernel_ms_batched": {"k_rq": kb[0], "k_ext": kb[1], "launch_pairs": kb[2]},
+        "kernel_ms_plain": {"k_rq": kp[0], "k_ext": kp[1], "launch_pairs": kp[2]},
+    }
+
+
+def bench_band(args):
+    import ps4
+    from ps4 import workloads
+    lo, hi, width = 9 * 10**11, 10**12, 10**8
+    sub = workloads.band_ranges(lo, hi, width)
+    eng = ps4.Engine(S_hi_max=hi)
+    eng.search(lo, lo + 10**6)
+    eng.kernel_time(reset=True)
+    log(f"band: [{lo},{hi}) as {len(sub)} subranges in one ps4_search_batch")
+    t0 = time.perf_counter()
+    res = eng.search_batch([a for a, _ in sub], [b for _, b in sub])
+    t = time.perf_counter() - t0
+    kt = eng.kernel_time(reset=True)
+    eng.close()
+    tot = {}
+    n5_le = 0
+    for c, lines in res:
+        add_counters(tot, c["counters"])
+        for ln in lines:   # the fifth element e <= 1e12 (members of a 5-set line, largest first)
+            if ln.startswith("5:") and int(ln[4:].split(",", 1)[0]) <= 10**12:
+                n5_le += 1
+    out = {
+        "metric": "elements/sec over the band [9e11,1e12) in batched form",
+        "value": (hi - lo) / t, "unit": "S/s", "n_gpus": 1, "higher_is_better": True, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "band [9e11,1e12) as 1000 subranges of 1e8, one batched call", "S_lo": lo, "S_hi": hi},
+        "seconds": t, "counters": tot, "five_sets_with_largest_le_1e12": n5_le,
+        "kernel_ms": {"k_rq": kt[0], "k_ext": kt[1], "launch_pairs": kt[2]},
+    }
+    if os.path.exists(GOLDEN):
+        gold = json.load(open(GOLDEN))
+        logs = {r["source"].rsplit("/", 1)[-1]: r for r in gold["reference_logs"]}
+        if "r12_9.err" in logs and "boundary_9.err" in logs:
+            ref = add_counters(dict(logs["r12_9.err"]["counters"]), logs["boundary_9.err"]["counters"])
+            # the logs come from an earlier build of s4 than exhaustive/s4.c: the counters up
+            # to the triples must agree exactly, the rest are reported beside them
+            out["vs_reference_logs"] = {
+                "exact_through_triples": all(ref[k] == tot[k] for k in ("S_done", "pairs", "S_ge3", "triples")),
+                "log": ref, "difference": {k: tot[k] - ref[k] for k in ref}}
+    if not args.no_cpu:
+        rng = random.Random(20261203)
+        pick = sorted(rng.sample(range(len(sub)), 3))
+        threads = min(16, len(os.sched_getaffinity(0)))
+        checks = []
+        for i in pick:
+            a, b = sub[i]
+            log(f"band: s4 on [{a},{b}) with {threads} threads")
+            t0 = time.perf_counter()
+            r = s4_ref_run(a, b, threads)
+            dt = time.perf_counter() - t0
+            if r is None:
+                checks = None
+                break
+            checks.append({"lo": a, "hi": b, "s4_seconds": dt, "identical": r[0] == res[i][0]["counters"]
+                           and r[1] == lines_sha(res[i][1])})
+        out["sample_vs_s4"] = checks
+    return out
+
+
+def bench_serving(args):
+    import ps4
+    from ps4 import serve, workloads
+    n = args.requests
+    stream = workloads.serving_stream(n, rate=200.0, seed=7)
+    eng = ps4.Engine(S_hi_max=10**12 + 10**8 + 1)
+    eng.search(10**12 - 10**6, 10**12)   # warm the device
+    srv = serve.Server(eng)
+    log(f"serving: {n} requests at 200/s (open loop)")
+    t_start = time.perf_counter()
+    reqs = []
+    for t_arr, kind, a in stream:
+        now = time.perf_counter() - t_start
+        if t_arr > now:
+            time.sleep(t_arr - now)
+        reqs.append(srv.submit(kind, a, t_arrive=t_start + t_arr))
+    for r in reqs:
+        r.wait()
+    t_end = time.perf_counter()
+    srv.close()
+    errors = [repr(r.error) for r in reqs if r.error is not None]
+
+    def pct(xs, q):
+        xs = sorted(xs)
+        return xs[min(len(xs) - 1, int(q * len(xs)))] * 1e3 if xs else None
+    lat = [r.latency for r in reqs]
+    lat_r = [r.latency for r in reqs if r.kind == "range"]
+    lat_t = [r.latency for r in reqs if r.kind == "triple"]
+    s_served = sum(r.args[1] - r.args[0] for r in reqs if r.kind == "range")
+    # the same requests alone: a seeded sample re-run one by one
+    rng = random.Random(11)
+    sample = rng.sample(range(n), min(40, n))
+    same = True
+    for i in sample:
+        r = reqs[i]
+        if r.kind == "range":
+            alone = eng.search(*r.args)
+            same &= alone[0] == r.result[0] and sorted(alone[1]) == sorted(r.result[1])
+        else:
+            same &= eng.extend_triple(*r.args) == r.result
+    eng.close()
+    p50, p99 = pct(lat, 0.5), pct(lat, 0.99)
+    return {
+        "metric": "p50/p99 request latency at 200 req/s (range and triple queries), continuous batching",
+        "value": p99, "unit": "ms (p99)", "n_gpus": 1, "higher_is_better": False, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "serving: Poisson 200 req/s seed 7, 70% ranges lo~U[1,1e12] len~logU[1e3,1e8], "
+                               "30% triples c<=1e9 d<=1e12", "requests": n},
+        "latency_ms": {"p50": p50, "p99": p99, "p99_over_p50": p99 / p50 if p50 else None,
+                       "range": {"p50": pct(lat_r, 0.5), "p99": pct(lat_r, 0.99)},
+                       "triple": {"p50": pct(lat_t, 0.5), "p99": pct(lat_t, 0.99)}},
+        "throughput": {"requests_per_s": n / (t_end - t_start), "range_S_per_s": s_served / (t_end - t_start)},
+        "steps": len(srv.steps), "mean_requests_per_step": n / max(1, len(srv.steps)),
+        "same_alone_and_in_mix": same, "sample_checked": len(sample), "errors": errors[:5],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,11 +306,17 @@ def main():
     ap.add_argument("--hi", type=int, default=10**10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
+    ap.add_argument("--workload", choices=["plain", "batched", "band", "serving"], default="plain")
+    ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the census shards by S range "
               f"across GPUs (disjoint S intervals, one engine per device), but this bench measures one device.", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "plain":
+        fn = {"batched": bench_batched, "band": bench_band, "serving": bench_serving}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return
     import ps4
     lo, hi = args.lo, args.hi
     eng = ps4.Engine(S_hi_max=hi)

@@ -22,6 +22,10 @@ typedef struct {
     uint32_t rid, pad;
 } ps4_quad;
 
+/* rid of a null entry: each window's block of the queue is padded to a whole
+ * number of 64-entry tiles with entries carrying this rid (and zero roots). */
+#define PS4_RID_NONE 0xFFFFFFFFu
+
 /* An extension e of a 4-set (e + each member is square). */
 typedef struct {
     uint32_t r[6];          /* the 4-set's roots, order of ps4_quad */

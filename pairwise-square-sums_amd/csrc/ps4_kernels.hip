@@ -467,16 +467,25 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             u32 tot4;
             const u32 b4 = block_excl_scan<RQ_NT>(n4, s_w, &tot4);
             tot4 = uni(tot4);
+            // the window's block of the queue is a whole number of k_ext tiles (64),
+            // padded with null entries, so that no tile mixes intervals
+            const u32 tot4p = (tot4 + 63u) & ~63u;
             if (tid == 0)
-                s_gbase = tot4 ? (u64)atomicAdd((unsigned long long *)&a.ctl->quad_count, (unsigned long long)tot4) : 0ull;
+                s_gbase = tot4 ? (u64)atomicAdd((unsigned long long *)&a.ctl->quad_count, (unsigned long long)tot4p) : 0ull;
             __syncthreads();
             const u64 gb = ((u64)uni((u32)(s_gbase >> 32)) << 32) | uni((u32)s_gbase);
             c_sets4 += n4;
-            if (gb + tot4 > a.quad_cap) {
+            if (gb + tot4p > a.quad_cap) {
                 err |= ERR_QUAD_OVF;
-            } else if (n4) {
-                rq_walk<true>(c0, c1, s_sk, s_st, s_tp, s_pv, tot_nq, B0, a.quads, gb + b4, a.quad_cap, item.rid, a.verify,
-                              &err);
+            } else {
+                if (n4)
+                    rq_walk<true>(c0, c1, s_sk, s_st, s_tp, s_pv, tot_nq, B0, a.quads, gb + b4, a.quad_cap, item.rid,
+                                  a.verify, &err);
+                if (tid < tot4p - tot4) {
+                    uint4 *o = (uint4 *)(a.quads + gb + tot4 + tid);
+                    o[0] = make_uint4(0u, 0u, 0u, 0u);
+                    o[1] = make_uint4(0u, 0u, PS4_RID_NONE, 0u);
+                }
             }
             __syncthreads();
         }
@@ -834,6 +843,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
             const u64 qidx = tile * EX_NT + tid;
             bool valid = qidx < nquads;
             const ps4_quad qd = a.quads[valid ? qidx : tile * EX_NT];   // lane 0's entry exists
+            valid = valid && qd.rid != PS4_RID_NONE;   // padding of a window's block
             const u32 rid0 = uni(qd.rid);
             const bool uniform = __ballot(valid && qd.rid != rid0) == 0ull && rid0 < a.n_rids;
             if (!uniform || rid0 != acc_rid) {

@@ -362,10 +362,16 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         const double quads_budget = 0.8 * (double)ctx->quad_cap * budget_scale;
         u32 n_items = 0;
         u32 rid_lo = iv[next_iv].rid, rid_hi = rid_lo;
+        // little work left (small queries): shorter superwindows, so that the
+        // representation kernel still has about one per workgroup
+        u64 remaining = 0;
+        for (size_t i = next_iv; i < n; ++i) remaining += iv[i].hi - iv[i].cur;
+        u64 item_max = ctx->W;
+        while (item_max * ctx->rq_grid < remaining && item_max < ((u64)1 << 40)) item_max <<= 1;
         for (size_t i = next_iv; i < n && n_items < ctx->items_cap && quads_expected < quads_budget; ++i) {
             Interval &v = iv[i];
             while (v.cur < v.hi && n_items < ctx->items_cap && quads_expected < quads_budget) {
-                u64 len = superwindow(ctx, v.cur);
+                u64 len = std::min(superwindow(ctx, v.cur), item_max);
                 if (len > v.hi - v.cur) len = v.hi - v.cur;
                 const double dq = quad_density(v.cur + len) * (double)len;
                 if (n_items > 0 && quads_expected + dq > quads_budget) {
@@ -464,12 +470,15 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
             std::vector<ps4_quad> qs(nq);
             HIP_OK(hipMemcpy(qs.data(), ctx->d_quads, nq * sizeof(ps4_quad), hipMemcpyDeviceToHost));
             struct K { u64 S, px, py, pz; u32 cls; size_t i; };
-            std::vector<K> keys(nq);
+            std::vector<K> keys;
+            keys.reserve(nq);
             for (u64 i = 0; i < nq; ++i) {
                 const ps4_quad &x = qs[i];
+                if (x.rid == PS4_RID_NONE) continue;   // padding of a window's block
+                if (x.rid < rid_lo || x.rid > rid_hi) { ctx->last_error = "4-set queue entry of another batch"; return PS4_E_INTERNAL; }
                 const u64 S = (u64)x.r_ab * x.r_ab + (u64)x.r_cd * x.r_cd;
                 const u32 cls = x.r_ad < x.r_bc ? 1u : 0u;
-                keys[i] = {S, x.r_cd, x.r_bd, cls ? x.r_ad : x.r_bc, cls, (size_t)i};
+                keys.push_back({S, x.r_cd, x.r_bd, cls ? x.r_ad : x.r_bc, cls, (size_t)i});
             }
             std::sort(keys.begin(), keys.end(), [&](const K &x, const K &y) {
                 if (qs[x.i].rid != qs[y.i].rid) return qs[x.i].rid < qs[y.i].rid;

@@ -160,12 +160,15 @@ class Engine:
         return [int(out[i]) for i in range(min(n.value, cap))]
 
     def extend_triple_batch(self, triples, d_max, cap=4096):
+        """Triple queries in one launch: one ascending list of d per triple (a
+        query with more than cap answers is asked again alone, in full)."""
         n = len(triples)
         arr = (ctypes.c_uint64 * (3 * n))(*[int(x) for t in triples for x in t])
         out = (ctypes.c_uint64 * (n * cap))()
         cnt = (ctypes.c_size_t * n)()
         self._check(self.lib.ps4_extend_triple_batch(self.ctx, n, arr, int(d_max), out, cap, cnt))
-        return [[int(out[i * cap + k]) for k in range(min(cnt[i], cap))] for i in range(n)]
+        return [[int(out[i * cap + k]) for k in range(cnt[i])] if cnt[i] <= cap
+                else self.extend_triple(triples[i], d_max) for i in range(n)]
 
     def kernel_time(self, reset=False):
         """(ms in k_rq, ms in k_ext, batches) accumulated on the device stream."""
