@@ -180,8 +180,11 @@ struct RqArgs {
 };
 
 // Walk the representation triples with flat indices [c0, c1) of one window
-// (lexicographic x<y<z inside each S, S in bucket order) and emit the 4-sets
-// of both classes (exhaustive/s4.c:324-339).  With WRITE false only counts.
+// (lexicographic index triples x<y<z inside each S, S in bucket order) and emit
+// the 4-sets of both classes (exhaustive/s4.c:324-339).  The buckets are not
+// sorted: the three p values of each triple are put in ascending order here,
+// which is the order the classes are defined in (and makes every 4-set come
+// out a > b > c > d).  With WRITE false only counts.
 template <bool WRITE>
 __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u32 *s_tp, const u32 *s_pv,
                        u32 nq, u64 B0, ps4_quad *out, u64 obase, u64 ocap, u32 rid, u32 verify, u32 *err)
@@ -209,8 +212,11 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
     double Sd = (double)S;
     u32 Sodd = (u32)(S & 1);
     u32 n = 0;
+    u32 vx = s_pv[st + x], vy = s_pv[st + y];
     for (u32 c = c0; c < c1; ++c) {
-        const u32 px = s_pv[st + x], py = s_pv[st + y], pz = s_pv[st + z];
+        const u32 vz = s_pv[st + z];
+        const u32 lo2 = min(vx, vy), hi2 = max(vx, vy);
+        const u32 px = min(lo2, vz), pz = max(hi2, vz), py = max(lo2, min(hi2, vz));
         const double Px = (double)px * px, Py = (double)py * py, Pz = (double)pz * pz;
         const double sxy = Px + Py;
         const u32 par = (px + py + pz) & 1u;
@@ -275,8 +281,10 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
                     x = 0;
                 }
                 y = x + 1;
+                if (c + 1 < c1) vx = s_pv[st + x];
             }
             z = y + 1;
+            if (c + 1 < c1) vy = s_pv[st + y];
         }
     }
     return n;
@@ -401,16 +409,21 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             }
             c_sge3 += nq;
             c_trip += tr;
-            u32 tot_nq, tot_qr, tot_tr;
-            u32 b_nq = block_excl_scan<RQ_NT>(nq, s_w, &tot_nq);
-            u32 b_qr = block_excl_scan<RQ_NT>(qr, s_w, &tot_qr);
-            u32 b_tr = block_excl_scan<RQ_NT>(tr, s_w, &tot_tr);
+            // two scans: counted S and their representations packed in one word
+            // (each sum stays below 2^16: at most 8192 S and bin_cap <= 32768
+            // points per window), and the triples with bit 31 set by a bucket too
+            // large for the 32-bit triple index (without one, the sum of C(k,3)
+            // over <= 2560 representations stays below 2^31)
+            u32 tot_p, tot_t;
+            const u32 b_p = block_excl_scan<RQ_NT>(nq | (qr << 16), s_w, &tot_p);
+            u32 b_tr = block_excl_scan<RQ_NT>(big ? 0x80000000u : tr, s_w, &tot_t);
+            tot_p = uni(tot_p);
+            tot_t = uni(tot_t);
+            u32 b_nq = b_p & 0xFFFFu, b_qr = b_p >> 16;
+            const u32 tot_nq = tot_p & 0xFFFFu, tot_qr = tot_p >> 16, tot_tr = tot_t;
             // window lists beyond the LDS capacity, or a bucket too large for the
             // 32-bit triple index: the window is not processed and the batch fails
-            tot_nq = uni(tot_nq);
-            tot_qr = uni(tot_qr);
-            tot_tr = uni(tot_tr);
-            if (block_any<RQ_NT>(big, s_w) || tot_nq > RQ_QS_CAP || tot_qr > RQ_PV_CAP) {
+            if ((tot_t >> 31) || tot_nq > RQ_QS_CAP || tot_qr > RQ_PV_CAP) {
                 err |= ERR_LDS_OVF;
                 continue;
             }
@@ -446,17 +459,6 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                     const u32 old = atomicAdd(&s_cnt[s >> 1], 1u << sh);
                     const u32 slot = (old >> sh) & 0xFFFFu;
                     if (slot < tot_qr) s_pv[slot] = (u32)(pt >> 16);
-                }
-            }
-            __syncthreads();
-            // buckets in ascending p, which makes every 4-set come out a > b > c > d
-            for (u32 j = tid; j < tot_nq; j += RQ_NT) {
-                const u32 st = s_st[j], k = s_sk[j] >> 16;
-                for (u32 x = 1; x < k; ++x) {
-                    const u32 v = s_pv[st + x];
-                    u32 y = x;
-                    while (y > 0 && s_pv[st + y - 1] > v) { s_pv[st + y] = s_pv[st + y - 1]; --y; }
-                    s_pv[st + y] = v;
                 }
             }
             __syncthreads();

@@ -126,6 +126,8 @@ static inline unsigned long long __ballot(int p)
     b->wave_bar[w].wait();
     return m;
 }
+template <typename T> static inline T min(T a, T b) { return a < b ? a : b; }
+template <typename T> static inline T max(T a, T b) { return a < b ? b : a; }
 static inline unsigned __float_as_uint(float f) { unsigned u; memcpy(&u, &f, 4); return u; }
 static inline float __uint_as_float(unsigned u) { float f; memcpy(&f, &u, 4); return f; }
 static inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }

@@ -13,6 +13,7 @@ import random
 import re
 import subprocess
 import sys
+import time
 
 import pytest
 
@@ -355,3 +356,44 @@ def test_census_1e11_reference_log(eng, golden):
     assert sum(ln.startswith("5:") for ln in lines) == res["counters"]["5-sets"]
     assert sum(ln.startswith("M:") for ln in lines) == res["counters"]["multi-ext-4sets"]
     assert sum(res["hist5"].values()) == res["counters"]["5-sets"]
+
+
+def test_search_batch_uneven_subranges(eng):
+    """Seeded uneven subranges (the batched workload's generator, smaller): one
+    batched call equals one call per subrange, counters and lines."""
+    from ps4 import workloads
+    ranges = workloads.batched_ranges(48, seed=42, top=10**10, e0=3, e1=6)
+    say(f"batched {len(ranges)} subranges")
+    got = eng.search_batch([a for a, _ in ranges], [b for _, b in ranges])
+    for (a, b), (c, lines) in zip(ranges, got):
+        alone = eng.search(a, b)
+        assert alone[0] == c, (a, b)
+        assert sorted(alone[1]) == sorted(lines), (a, b)
+
+
+def test_serving_results_same_alone_and_in_the_mix(eng):
+    """Requests through the continuous-batching loop (arriving while others run)
+    give exactly the results of the same requests run alone."""
+    from ps4 import serve, workloads
+    stream = workloads.serving_stream(80, rate=2000.0, seed=7, s_top=10**10, len_e0=3, len_e1=6, c_max=10**7,
+                                      d_max=10**10)
+    say(f"serving {len(stream)} requests")
+    srv = serve.Server(eng, out_budget_bytes=1 << 20)
+    t0 = time.perf_counter()
+    reqs = []
+    for t_arr, kind, a in stream:
+        dt = t0 + t_arr - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        reqs.append(srv.submit(kind, a))
+    for r in reqs:
+        assert r.wait(120)
+    srv.close()
+    assert len(srv.steps) < len(reqs)           # some steps served several requests
+    for r in reqs:
+        assert r.error is None
+        if r.kind == "range":
+            alone = eng.search(*r.args)
+            assert alone[0] == r.result[0] and sorted(alone[1]) == sorted(r.result[1]), r.args
+        else:
+            assert eng.extend_triple(*r.args) == r.result, r.args
