@@ -544,8 +544,11 @@ struct ExArgs {
 
 // Per-4-set parameters of the candidate loop: one record of four 16-byte
 // words per 4-set of a pass (read together, before any branch):
-//   q0 = {offL | nL << 10 | offH << 20 | half << 30,  nH | lane << 10 | best << 16,
-//         um as float (>= (min |C - A| + 1) / 2),  isqrt(2A) when 2A is a square, else 0}
+//   q0 = {offL | nL << 10 | offH << 20,  nH | coff << 10 | lane << 23 | best << 29,
+//         ceil(2^23 / nL),  isqrt(2A) when 2A is a square, else 0}
+//         coff: the 4-set's first candidate in the pass; lane: its lane in the tile;
+//         ceil(2^23/nL) gives t / nL = (t << 9) * it >> 32 exactly for t < 8192
+//         and nL < 1024 (checked for every pair on the host)
 //   q1 = {mask mod 64 (lo, hi),  mask mod 63 (lo, hi)}
 //   q2 = {mask mod 65 bits 0..63 (lo, hi),  msm (lo, hi)}: msm bit 0 is bit 64 of
 //         the mod-65 mask, then the masks mod 11 (bit 1), 13 (12), 17 (25), 19 (42)
@@ -713,22 +716,15 @@ struct ExLists {
     double l0, h0, l1, h1;
 };
 
-__device__ __forceinline__ ExLists ex_lists(const ExRec &R, const double *s_ar, u32 t)
+// g: the candidate's index in the pass; t = g - coff is its index in its 4-set.
+__device__ __forceinline__ ExLists ex_lists(const ExRec &R, const double *s_ar, u32 g)
 {
     const u32 W0 = R.q0.x, W1 = R.q0.y;
     const u32 offL = W0 & 1023u, cnL = (W0 >> 10) & 1023u, offH = (W0 >> 20) & 1023u;
     const u32 cnH = W1 & 1023u;
-    // j = t / nL from a single-precision estimate within one of it (unsigned
-    // corrections: no signed arithmetic in the hot loop)
-    u32 j = (u32)((float)t * __builtin_amdgcn_rcpf((float)cnL));
-    u32 jn = j * cnL;
-    const bool over = jn > t;
-    j -= over ? 1u : 0u;
-    jn -= over ? cnL : 0u;
-    u32 i = t - jn;
-    const bool under = i >= cnL;
-    j += under ? 1u : 0u;
-    i -= under ? cnL : 0u;
+    const u32 t = g - ((W1 >> 10) & 8191u);
+    const u32 j = __umulhi(t << 9, R.q0.z);   // t / nL
+    const u32 i = t - j * cnL;
     return ExLists{s_ar[offL + i], s_ar[offH + j], s_ar[offL + cnL - 1 - i], s_ar[offH + cnH - 1 - j]};
 }
 
@@ -736,9 +732,8 @@ __device__ __forceinline__ ExCand ex_cand(const ExRec &R, const ExLists &Ls)
 {
     const uint4 q0 = R.q0, q1 = R.q1, q2 = R.q2, q3 = R.q3;
     const u32 W0 = q0.x, W1 = q0.y;
-    const double s2 = __builtin_fma(Ls.l1, Ls.h1, Ls.l0 * Ls.h0);
     ExCand c;
-    c.u = s2 * ((W0 >> 30) ? 0.5 : 1.0);
+    c.u = __builtin_fma(Ls.l1, Ls.h1, Ls.l0 * Ls.h0);   // H holds halves when u = (f+g)/2
     c.w1 = W1;
     const u32 r1 = rep_m1(c.u);
     const u32 r4095 = fold4095(r1);
@@ -750,15 +745,15 @@ __device__ __forceinline__ ExCand ex_cand(const ExRec &R, const ExLists &Ls)
     return c;
 }
 
-// The later filters of a counted candidate u of record word q0 and q2: the
-// size bound u <= (min |C - A| + 1) / 2 and the residue masks mod 11, 13, 17,
-// 19 (e + C1 and e + C2 both squares mod each).  True: it needs the exact test.
-__device__ __forceinline__ bool ex_late(double u, const uint4 &q0, const uint4 &q2)
+// The later filters of a counted candidate u of record word q2: the residue
+// masks mod 11, 13, 17, 19 (e + C1 and e + C2 both squares mod each).  True:
+// it needs the exact test.
+__device__ __forceinline__ bool ex_late(double u, const uint4 &q2)
 {
     const u32 r2 = mod_m2(u);
     const u32 m2b = bit64(q2.z, q2.w, 1u + mod11(r2)) & bit64(q2.z, q2.w, 12u + mod13(r2)) &
                     bit64(q2.z, q2.w, 25u + mod17(r2)) & bit64(q2.z, q2.w, 42u + mod19(r2));
-    return (m2b != 0u) & (u <= (double)__uint_as_float(q0.z));
+    return m2b != 0u;
 }
 
 __device__ __forceinline__ u64 wave_sum_u64(u64 v)
@@ -858,7 +853,6 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
             u32 nf = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0, best = 0;
             u64 pm64 = 0, pm63 = 0, pm65 = 0, pmsm = 0;
             u32 psqA = 0, pf1 = 0, pf2 = 0, pf3 = 0, pf4 = 0;
-            float pum = 0.f;
             if (valid) {
                 const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
                 const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
@@ -984,10 +978,6 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                     pf2 = (u32)P.F2;
                     pf3 = (u32)P.F3;
                     pf4 = (s2a * s2a == 2 * P.A) ? (u32)s2a : 0u;   // u >= 2: 0 never matches
-                    // u^2 + (C - A) can be square only if u <= (|C - A| + 1) / 2;
-                    // kept as a float rounded up (a weaker bound is still a bound)
-                    const u64 ad1 = P.C1 > P.A ? P.C1 - P.A : P.A - P.C1, ad2 = P.C2 > P.A ? P.C2 - P.A : P.A - P.C2;
-                    pum = (float)((double)(((ad1 < ad2 ? ad1 : ad2) + 1) >> 1) * (1.0 + 1.0 / 65536.0) + 1.0);
                 }
             }
 
@@ -1009,15 +999,18 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                 __syncthreads();
                 if (fits) {
                     uint4 *pw = s_par + slot * EX_PAR_WORDS;
-                    pw[0] = make_uint4(loff | (nL << 10) | ((loff + nL) << 20) | (half << 30), nH | (tid << 10) | (best << 16),
-                                       __float_as_uint(pum), pf4);
+                    pw[0] = make_uint4(loff | (nL << 10) | ((loff + nL) << 20), nH | (coff << 10) | (tid << 23) | (best << 29),
+                                       (0x800000u + nL - 1u) / nL, pf4);
                     pw[1] = make_uint4((u32)pm64, (u32)(pm64 >> 32), (u32)pm63, (u32)(pm63 >> 32));
                     pw[2] = make_uint4((u32)pm65, (u32)(pm65 >> 32), (u32)pmsm, (u32)(pmsm >> 32));
                     pw[3] = make_uint4(psqA, pf1, pf2, pf3);
                     atomicOr(&s_bits[coff >> 5], 1u << (coff & 31u));
                     // mixed-radix lists over the factor sequence: stage s multiplies
                     // the list so far by p_s^1 .. p_s^e (earlier stages are the less
-                    // significant digits, so index n and n_max - n are complements)
+                    // significant digits, so index n and n_max - n are complements).
+                    // When u = (f + g) / 2 the H list holds halves (exact doubles),
+                    // so that u = L[i] * H[j] + L[i'] * H[j'] in both cases.
+                    const double h0 = half ? 0.5 : 1.0;
                     const u32 nfa = nf + (e2p ? 1u : 0u);
                     double *lst = s_ar + loff;
                     u32 n = 1;
@@ -1026,7 +1019,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                         if (s == sL) {   // start the H list
                             lst = s_ar + loff + nL;
                             n = 1;
-                            lst[0] = 1.0;
+                            lst[0] = h0;
                         }
                         u32 p, e;
                         if (e2p && s == 0) { p = 2; e = e2p; }
@@ -1044,30 +1037,25 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                             for (; j < cur; ++j) lst[n++] = lst[j] * pk;
                         }
                     }
-                    if (sL == nfa) s_ar[loff + nL] = 1.0;   // empty H part: the list {1}
+                    if (sL == nfa) s_ar[loff + nL] = h0;   // empty H part: the list {1}
                 }
                 __syncthreads();
                 // lane l takes candidates l, l+64, l+128, ... of the pass, two per
                 // iteration (g and g + 64: two independent load/compute chains)
-                u32 kp = 0, sprev = 0;   // 4-set starts before base; start of the 4-set holding base-1
+                u32 kp = 0;   // 4-set starts before base
                 const u64 lane_mask = (2ull << tid) - 1ull;   // bits 0..tid
                 u32 qn1 = 0, qn2 = 0;                         // entries in the two queues
                 for (u32 base = 0; base < ctot; base += 128) {
                     const uint4 wv = s_bits4[base >> 7];
                     const u64 wa = (u64)wv.x | ((u64)wv.y << 32), wb = (u64)wv.z | ((u64)wv.w << 32);
                     const u32 ga = base + tid, gb = ga + 64u;
-                    const u64 ba = wa & lane_mask, bb = wb & lane_mask;   // starts in [base, ga], [base+64, gb]
-                    const u32 ca = (u32)__popcll(ba), cb = (u32)__popcll(bb);
-                    const u32 ka = kp + ca - 1u;
-                    const u32 sa = ca ? base + 63u - (u32)__clzll((long long)ba) : sprev;
+                    // the 4-set of candidate g: starts at or before g, minus one
+                    const u32 ka = kp + (u32)__popcll(wa & lane_mask) - 1u;
                     kp += (u32)__popcll(wa);
-                    if (wa) sprev = base + 63u - (u32)__clzll((long long)wa);
-                    const u32 kb = kp + cb - 1u;
-                    const u32 sb = cb ? base + 127u - (u32)__clzll((long long)bb) : sprev;
+                    const u32 kb = kp + (u32)__popcll(wb & lane_mask) - 1u;
                     kp += (u32)__popcll(wb);
-                    if (wb) sprev = base + 127u - (u32)__clzll((long long)wb);
-                    // past the end of the pass a lane evaluates candidate 0 of record 0
-                    // (in range) and discards it
+                    // past the end of the pass a lane evaluates record 0 at its first
+                    // candidate (in range) and discards it
                     const bool va = ga < ctot, vb = gb < ctot;
                     const ExRec RA = ex_rec(s_par, va ? ka : 0u), RB = ex_rec(s_par, vb ? kb : 0u);
                     PS4_KEEP("v"(RA.q1.x), "v"(RA.q1.y), "v"(RA.q1.z), "v"(RA.q1.w), "v"(RA.q2.x), "v"(RA.q2.y),
@@ -1075,7 +1063,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                              "v"(RA.q0.z), "v"(RA.q0.w), "v"(RB.q1.x), "v"(RB.q1.y), "v"(RB.q1.z), "v"(RB.q1.w),
                              "v"(RB.q2.x), "v"(RB.q2.y), "v"(RB.q2.z), "v"(RB.q2.w), "v"(RB.q3.x), "v"(RB.q3.y),
                              "v"(RB.q3.z), "v"(RB.q3.w), "v"(RB.q0.z), "v"(RB.q0.w));   // both records
-                    const ExLists LA = ex_lists(RA, s_ar, va ? ga - sa : 0u), LB = ex_lists(RB, s_ar, vb ? gb - sb : 0u);
+                    const ExLists LA = ex_lists(RA, s_ar, va ? ga : 0u), LB = ex_lists(RB, s_ar, vb ? gb : 0u);
                     PS4_KEEP("v"(LA.l0), "v"(LA.h0), "v"(LA.l1), "v"(LA.h1), "v"(LB.l0), "v"(LB.h0), "v"(LB.l1),
                              "v"(LB.h1));   // both candidates' list loads in flight together
                     const ExCand A = ex_cand(RA, LA);
@@ -1085,11 +1073,11 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                         my_esq += (oka ? 1u : 0u) + (okb ? 1u : 0u);
                     } else {   // a tile with several intervals (rare): count where each belongs
                         if (oka) {
-                            const u32 r = a.quads[tile * EX_NT + ((A.w1 >> 10) & 63u)].rid;
+                            const u32 r = a.quads[tile * EX_NT + ((A.w1 >> 23) & 63u)].rid;
                             atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
                         }
                         if (okb) {
-                            const u32 r = a.quads[tile * EX_NT + ((B.w1 >> 10) & 63u)].rid;
+                            const u32 r = a.quads[tile * EX_NT + ((B.w1 >> 23) & 63u)].rid;
                             atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
                         }
                     }
@@ -1111,7 +1099,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                                 const u64 e2 = exq2[q + tid];
                                 const u32 W1 = s_par[(u32)(e2 >> 52) * EX_PAR_WORDS].y;
                                 err |= ext_exact((double)(e2 & ((1ull << 52) - 1ull)), a.quads,
-                                                 tile * EX_NT + ((W1 >> 10) & 63u), (W1 >> 16) & 7u, a.ctl, a.hits,
+                                                 tile * EX_NT + ((W1 >> 23) & 63u), W1 >> 29, a.ctl, a.hits,
                                                  a.hit_cap);
                             }
                         }
@@ -1126,7 +1114,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                             if (q0 + tid < qn1) {
                                 ent = exq1[q0 + tid];
                                 const uint4 *pw = s_par + (u32)(ent >> 52) * EX_PAR_WORDS;
-                                go = ex_late((double)(ent & ((1ull << 52) - 1ull)), pw[0], pw[2]);
+                                go = ex_late((double)(ent & ((1ull << 52) - 1ull)), pw[2]);
                             }
                             const u64 bg = __ballot(go);
                             if (go) exq2[qn2 + (u32)__popcll(bg & lane_lt)] = ent;

@@ -157,6 +157,7 @@ static inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int ro
 static inline int __builtin_amdgcn_readlane(int v, int l) { return (int)(uint32_t)emu_wave_exchange((uint32_t)v, (unsigned)l); }
 static inline int __builtin_amdgcn_readfirstlane(int v) { return (int)emu_wave_exchange((uint32_t)v, 0); }
 static inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+static inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((unsigned long long)a * b) >> 32); }
 static inline unsigned __umul24(unsigned a, unsigned b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }
 
 template <typename T, typename V> static inline T atomicAdd(T *p, V v) { return __atomic_fetch_add(p, (T)v, __ATOMIC_SEQ_CST); }

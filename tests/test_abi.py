@@ -91,3 +91,14 @@ def test_codegen_has_no_signed_shift_of_unsigned_products(tmp_path):
     asm = out.read_text()
     assert "v_ashrrev_i32" not in asm
     assert "__umul24(" not in open(src).read()
+
+
+def test_division_magic_of_the_candidate_loop_is_exact():
+    """t / nL = ((t << 9) * ceil(2^23 / nL)) >> 32 for every t < 8192 and
+    1 <= nL < 1024 (the extension kernel's index of a candidate in its lists)."""
+    import numpy as np
+    t = np.arange(8192, dtype=np.uint64)
+    for d in range(1, 1024):
+        m = (0x800000 + d - 1) // d
+        assert m < (1 << 24)
+        assert np.array_equal(((t << np.uint64(9)) * np.uint64(m)) >> np.uint64(32), t // np.uint64(d)), d
