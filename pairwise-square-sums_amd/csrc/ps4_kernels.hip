@@ -48,12 +48,25 @@ __device__ __forceinline__ u64 d_isqrt64(u64 n)
 }
 
 // n is a perfect square, n < 2^106 (exhaustive/s4.c:101-108).
-__device__ __noinline__ bool d_is_square128(u128 n)
+__device__ __noinline__ bool d_is_square128_slow(u128 n)
 {
     const double x = (double)(u64)(n >> 64) * 18446744073709551616.0 + (double)(u64)n;
     u64 r = (u64)__builtin_sqrt(x);
     while ((u128)r * r > n) --r;
     while ((u128)(r + 1) * (r + 1) <= n) ++r;
+    return (u128)r * r == n;
+}
+
+// The same test without loops for n < 2^96: x = n rounded once (the high word
+// is exact, the fused multiply-add rounds once), so sqrt(x) is within
+// k * 2^-53 + a few ulp(k) < 0.25 of k = sqrt(n) when n = k^2 (k < 2^48), and
+// the nearest integer r is k; r^2 == n then decides exactly.
+__device__ __forceinline__ bool d_is_square128(u128 n)
+{
+    const u64 hi = (u64)(n >> 64), lo = (u64)n;
+    if (hi >> 32) return d_is_square128_slow(n);
+    const double x = __builtin_fma((double)hi, 18446744073709551616.0, (double)lo);
+    const u64 r = (u64)__builtin_rint(__builtin_sqrt(x));
     return (u128)r * r == n;
 }
 
@@ -646,7 +659,7 @@ __device__ __forceinline__ ExPair ex_pair(u32 best, u64 va, u64 vb, u64 vc, u64 
 // One surviving candidate: exact e = u^2 - A and the two square tests of
 // exhaustive/s4.c:195-200.  Rare (after all residue filters), so out of line;
 // A, C1, C2 are rederived from the queued 4-set.
-__device__ __noinline__ u32 ext_exact(double u, const ps4_quad *quads, u64 qidx, u32 best, ps4_ctl *ctl,
+__device__ __forceinline__ u32 ext_exact(double u, const ps4_quad *quads, u64 qidx, u32 best, ps4_ctl *ctl,
                                       ps4_hit *hits, u32 hit_cap)
 {
     const ps4_quad qd = quads[qidx];
