@@ -47,6 +47,16 @@ __device__ __forceinline__ u64 d_isqrt64(u64 n)
     return r;
 }
 
+// floor(sqrt(n)) for n < 2^52: n converts exactly, so the root of the
+// correctly rounded square root is within one of it; one step each way.
+__device__ __forceinline__ u64 d_isqrt52(u64 n)
+{
+    u64 r = (u64)__builtin_sqrt((double)n);
+    r -= (r * r > n) ? 1u : 0u;
+    r += ((r + 1) * (r + 1) <= n) ? 1u : 0u;
+    return r;
+}
+
 // n is a perfect square, n < 2^106 (exhaustive/s4.c:101-108).
 __device__ __noinline__ bool d_is_square128_slow(u128 n)
 {
@@ -612,8 +622,9 @@ __device__ __forceinline__ u32 mod19(u32 s) { return s - 19u * ((s * 55189u) >> 
 #define M1 262080.0
 #define M2 46189.0
 
-// x mod M2 for an integer-valued double 0 <= x < 2^52 (the representative of
-// rep_m1's kind, then two conditional subtractions as unsigned minima).
+// x mod M2 in [0, M2) for an integer-valued double |x| < 2^52 (the
+// representative of rep_m1's kind, then two conditional subtractions as
+// unsigned minima).
 __device__ __forceinline__ u32 mod_m2(double x)
 {
     const double q = __builtin_floor(x * (1.0 / M2));
@@ -624,7 +635,7 @@ __device__ __forceinline__ u32 mod_m2(double x)
 }
 
 // A representative in [0, 3*M1) of x mod M1 for an integer-valued double
-// 0 <= x < 2^52: the floor of the rounded quotient is within one of the true
+// |x| < 2^52: the floor of the rounded quotient is within one of the true
 // quotient, so x - q*M1 (exact by the fused multiply-add) lies in [-M1, 2*M1).
 // Residues mod 64, 63 and 65 do not depend on which representative.
 __device__ __forceinline__ u32 rep_m1(double x)
@@ -644,15 +655,23 @@ struct ExPair {
 __device__ __forceinline__ ExPair ex_pair(u32 best, u64 va, u64 vb, u64 vc, u64 vd, u64 rab, u64 rac, u64 rad,
                                           u64 rbc, u64 rbd, u64 rcd)
 {
+    // as selects (the lanes of a wave choose different pairs):
+    //   best 0: A=va C1=vc C2=vd F=(rab,rac,rad) ra=rac rb=rbc   (i,j,k) = (0,1,2)
+    //        1: A=vc C1=va C2=vb F=(rac,rbc,rcd) ra=rac rb=rad           (2,3,0)
+    //        2: A=va C1=vb C2=vd F=(rab,rac,rad) ra=rab rb=rbc           (0,2,1)
+    //        3: A=vb C1=va C2=vc F=(rab,rbc,rbd) ra=rab rb=rad           (1,3,0)
+    //        4: A=va C1=vb C2=vc F=(rab,rac,rad) ra=rab rb=rbd           (0,3,1)
+    //        5: A=vb C1=va C2=vd F=(rab,rbc,rbd) ra=rab rb=rac           (1,2,0)
+    const bool b1 = best == 1, odd = (best & 1u) != 0, b35 = best == 3 || best == 5;
     ExPair p;
-    switch (best) {
-    case 0: p = {va, vc, vd, rab, rac, rad, rac, rbc}; break;   // (i,j,k) = (0,1,2)
-    case 1: p = {vc, va, vb, rac, rbc, rcd, rac, rad}; break;   // (2,3,0)
-    case 2: p = {va, vb, vd, rab, rac, rad, rab, rbc}; break;   // (0,2,1)
-    case 3: p = {vb, va, vc, rab, rbc, rbd, rab, rad}; break;   // (1,3,0)
-    case 4: p = {va, vb, vc, rab, rac, rad, rab, rbd}; break;   // (0,3,1)
-    default: p = {vb, va, vd, rab, rbc, rbd, rab, rac}; break;  // (1,2,0)
-    }
+    p.A = b1 ? vc : (b35 ? vb : va);
+    p.C1 = best == 0 ? vc : (odd ? va : vb);
+    p.C2 = b1 ? vb : ((best == 3 || best == 4) ? vc : vd);
+    p.F1 = b1 ? rac : rab;
+    p.F2 = odd ? rbc : rac;
+    p.F3 = b1 ? rcd : (b35 ? rbd : rad);
+    p.ra = best < 2 ? rac : rab;
+    p.rb = (best == 0 || best == 2) ? rbc : ((best == 1 || best == 3) ? rad : (best == 4 ? rbd : rac));
     return p;
 }
 
@@ -840,6 +859,8 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
     u64 acc_cand = 0, acc_esq = 0, acc_md = 0;
     u32 err = 0;
 
+    u64 pf_tile = ~0ull;   // the tile whose entries qpf holds (loaded during the previous tile)
+    ps4_quad qpf;
     u32 chunk = 0;
     if (tid == 0) chunk = atomicAdd(&a.ctl->tile_next, 1u);
     chunk = uni(chunk);   // lane 0's value (every lane is active here)
@@ -852,7 +873,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
             if (tile >= ntiles) break;
             const u64 qidx = tile * EX_NT + tid;
             bool valid = qidx < nquads;
-            const ps4_quad qd = a.quads[valid ? qidx : tile * EX_NT];   // lane 0's entry exists
+            const ps4_quad qd = tile == pf_tile ? qpf : a.quads[valid ? qidx : tile * EX_NT];   // lane 0's entry exists
             valid = valid && qd.rid != PS4_RID_NONE;   // padding of a window's block
             const u32 rid0 = uni(qd.rid);
             const bool uniform = __ballot(valid && qd.rid != rid0) == 0ull && rid0 < a.n_rids;
@@ -962,8 +983,10 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                     // r^2 + (C1-A) and r^2 + (C2-A) are both squares mod m (the
                     // filters of exhaustive/s4.c:192-194)
                     const double d1 = (double)(i64)(P.C1 - P.A), d2 = (double)(i64)(P.C2 - P.A);
-                    const u32 r1a = (u32)dmod(d1, M1, 1.0 / M1), r1b = (u32)dmod(d2, M1, 1.0 / M1);
-                    const u32 r2a = (u32)dmod(d1, M2, 1.0 / M2), r2b = (u32)dmod(d2, M2, 1.0 / M2);
+                    // representatives mod 262080 (residues mod 64, 63, 65 follow) and
+                    // exact residues mod 46189; |C - A| < 2^51
+                    const u32 r1a = rep_m1(d1), r1b = rep_m1(d2);
+                    const u32 r2a = mod_m2(d1), r2b = mod_m2(d2);
                     const u32 fa = fold4095(r1a), fb = fold4095(r1b);
                     // residues index small tables: each index is checked against
                     // its table (a wrong residue is an error, never a load outside it)
@@ -985,12 +1008,22 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                     // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m,
                     // i.e. u is a root of a pair sum containing A, or u^2 = 2A
                     // (every root is below 2^26 for S < 2^50)
-                    const u64 sa = d_isqrt64(P.A), s2a = d_isqrt64(2 * P.A);
+                    const u64 sa = d_isqrt52(P.A), s2a = d_isqrt52(2 * P.A);
                     psqA = (u32)sa;
                     pf1 = (u32)P.F1;
                     pf2 = (u32)P.F2;
                     pf3 = (u32)P.F3;
                     pf4 = (s2a * s2a == 2 * P.A) ? (u32)s2a : 0u;   // u >= 2: 0 never matches
+                }
+            }
+
+            // the next tile's entries: loaded now, used after this tile's passes
+            {
+                const u64 nt = sub + 1 < EX_CHUNK ? tile + 1 : (u64)uni(next) * EX_CHUNK;
+                if (nt < ntiles) {
+                    const u64 ni = nt * EX_NT + tid;
+                    qpf = a.quads[ni < nquads ? ni : nt * EX_NT];
+                    pf_tile = nt;
                 }
             }
 
