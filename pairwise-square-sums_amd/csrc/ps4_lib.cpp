@@ -85,24 +85,56 @@ std::string dec(u128 v)
     return s;
 }
 
-// "tag [x...] S=..." with members largest first, and the verify note of
-// exhaustive/s4.c:209-220.
+// Decimal digits of v at p (no terminator); returns the end.
+char *put_dec(char *p, u128 v)
+{
+    char tmp[48];
+    int n = 0;
+    while (v >> 64) { tmp[n++] = (char)('0' + (int)(v % 10)); v /= 10; }
+    u64 w = (u64)v;
+    do { tmp[n++] = (char)('0' + (int)(w % 10)); w /= 10; } while (w);
+    while (n) *p++ = tmp[--n];
+    return p;
+}
+
+// Append "tag [x...] S=..." with members largest first, and the verify note of
+// exhaustive/s4.c:209-220, to out.  n <= 6 members.
+void append_set(std::string &out, const char *tag, const u128 *xin, int n, u64 S)
+{
+    u128 x[6];
+    for (int i = 0; i < n; ++i) {   // insertion sort, descending
+        u128 v = xin[i];
+        int j = i;
+        while (j > 0 && x[j - 1] < v) { x[j] = x[j - 1]; --j; }
+        x[j] = v;
+    }
+    bool ok = true;
+    for (int i = 0; i < n && ok; ++i)
+        for (int j = i + 1; j < n; ++j)
+            if (x[i] == x[j] || !is_square(x[i] + x[j])) { ok = false; break; }
+    char buf[512];
+    char *p = buf;
+    for (const char *t = tag; *t; ++t) *p++ = *t;
+    *p++ = ' ';
+    *p++ = '[';
+    for (int i = 0; i < n; ++i) {
+        if (i) *p++ = ',';
+        p = put_dec(p, x[i]);
+    }
+    *p++ = ']';
+    *p++ = ' ';
+    *p++ = 'S';
+    *p++ = '=';
+    p = put_dec(p, S);
+    if (!ok) for (const char *t = "  *** VERIFY FAILED ***"; *t; ++t) *p++ = *t;
+    *p++ = '\n';
+    out.append(buf, (size_t)(p - buf));
+}
+
 std::string format_set(const char *tag, std::vector<u128> x, u64 S)
 {
-    std::sort(x.begin(), x.end(), [](u128 p, u128 q) { return p > q; });
-    bool ok = true;
-    for (size_t i = 0; i < x.size() && ok; ++i)
-        for (size_t j = i + 1; j < x.size(); ++j)
-            if (x[i] == x[j] || !is_square(x[i] + x[j])) { ok = false; break; }
-    std::string s = tag;
-    s += " [";
-    for (size_t i = 0; i < x.size(); ++i) {
-        if (i) s += ",";
-        s += dec(x[i]);
-    }
-    s += "] S=" + std::to_string((unsigned long long)S);
-    if (!ok) s += "  *** VERIFY FAILED ***";
-    s += "\n";
+    std::string s;
+    append_set(s, tag, x.data(), (int)x.size(), S);
     return s;
 }
 
@@ -504,16 +536,25 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         const u32 nh = ctx->h_ctl->hit_count;
         if (nh) {
             HIP_OK(hipMemcpy(ctx->h_hits, ctx->d_hits, (size_t)nh * sizeof(ps4_hit), hipMemcpyDeviceToHost));
-            std::vector<u32> idx(nh);
-            for (u32 i = 0; i < nh; ++i) idx[i] = i;
-            auto key = [&](const ps4_hit &h) { return std::make_tuple(h.rid, h.r[0], h.r[1], h.r[2], h.r[3], h.r[4], h.r[5]); };
-            std::sort(idx.begin(), idx.end(), [&](u32 x, u32 y) {
-                const ps4_hit &p = ctx->h_hits[x], &q = ctx->h_hits[y];
-                const u64 Sp = (u64)p.r[0] * p.r[0] + (u64)p.r[5] * p.r[5], Sq = (u64)q.r[0] * q.r[0] + (u64)q.r[5] * q.r[5];
-                if (p.rid != q.rid) return p.rid < q.rid;
-                if (Sp != Sq) return Sp < Sq;
-                return key(p) < key(q);
+            // order: interval, S, then the 4-set's roots (one group per 4-set)
+            struct HK { u64 S, k1, k2, k3; u32 rid, i; };   // roots < 2^26: two per key word
+            std::vector<HK> hk(nh);
+            for (u32 i = 0; i < nh; ++i) {
+                const ps4_hit &h = ctx->h_hits[i];
+                const u64 S = (u64)h.r[0] * h.r[0] + (u64)h.r[5] * h.r[5];
+                hk[i] = {S, ((u64)h.r[0] << 26) | h.r[1], ((u64)h.r[2] << 26) | h.r[3], ((u64)h.r[4] << 26) | h.r[5],
+                         h.rid, i};
+            }
+            std::sort(hk.begin(), hk.end(), [](const HK &x, const HK &y) {
+                if (x.rid != y.rid) return x.rid < y.rid;
+                if (x.S != y.S) return x.S < y.S;
+                if (x.k1 != y.k1) return x.k1 < y.k1;
+                if (x.k2 != y.k2) return x.k2 < y.k2;
+                return x.k3 < y.k3;
             });
+            std::vector<u32> idx(nh);
+            for (u32 i = 0; i < nh; ++i) idx[i] = hk[i].i;
+            auto key = [&](const ps4_hit &h) { return std::make_tuple(h.rid, h.r[0], h.r[1], h.r[2], h.r[3], h.r[4], h.r[5]); };
             for (u32 i = 0; i < nh;) {
                 const ps4_hit &h0 = ctx->h_hits[idx[i]];
                 Group g;
@@ -555,7 +596,8 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
                     int lg = 0;
                     for (u128 x = g.ext[k]; x > 1; x >>= 1) ++lg;
                     ++o.hist5[lg > 99 ? 99 : lg];
-                    t += format_set("5:", {q.a, q.b, q.c, q.d, g.ext[k]}, q.S);
+                    const u128 v5[5] = {q.a, q.b, q.c, q.d, g.ext[k]};
+                    append_set(t, "5:", v5, 5, q.S);
                 }
                 for (size_t k = 0; k < ne; ++k)
                     for (size_t s = k + 1; s < ne; ++s)
