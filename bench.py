@@ -137,6 +137,28 @@ def s4_ref_run(lo, hi, threads):
     return counters, lines_sha(lines)
 
 
+def copy_probe(mib):
+    """One device-to-device hipMemcpy of mib MiB (the runtime's copy kernel),
+    between two buffers filled by hipMemset."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, n)
+    hip.hipMemset(b, 0, n)
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, n, 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed ({rc})")
+    log(f"copy probe: {mib} MiB device-to-device")
+
+
 def add_counters(acc, c):
     for k, v in c.items():
         acc[k] = acc.get(k, 0) + v
@@ -308,6 +330,9 @@ def main():
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--workload", choices=["plain", "batched", "band", "serving"], default="plain")
     ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device once (a plain copy of known size that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters in a profiled run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the census shards by S range "
@@ -320,6 +345,8 @@ def main():
     import ps4
     lo, hi = args.lo, args.hi
     eng = ps4.Engine(S_hi_max=hi)
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
     for w in range(args.warmup):
         log(f"warmup {w + 1}/{args.warmup}: census [{lo},{hi})")
         eng.search(lo, hi)
