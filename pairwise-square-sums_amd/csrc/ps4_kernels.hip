@@ -30,6 +30,7 @@
 typedef uint64_t u64;
 typedef uint32_t u32;
 typedef int64_t i64;
+typedef int32_t i32;
 typedef unsigned __int128 u128;
 
 // ----------------------------------------------------------------------------
@@ -79,6 +80,14 @@ __device__ __forceinline__ bool d_is_square128(u128 n)
     const u64 r = (u64)__builtin_rint(__builtin_sqrt(x));
     return (u128)r * r == n;
 }
+
+// Takes values in registers at this point: an empty asm statement with them as
+// inputs, so the compiler issues their loads before it and cannot sink them
+// into branches of their own.  (The host emulation of these sources defines it
+// as nothing.)
+#ifndef PS4_KEEP
+#define PS4_KEEP(...) asm volatile("" ::__VA_ARGS__)
+#endif
 
 __device__ __forceinline__ u32 lane_id() { return threadIdx.x & 63u; }
 
@@ -186,6 +195,7 @@ __device__ __forceinline__ u32 tri3(u32 k) { return (u32)(((u64)k * (k - 1) * (k
 #define RQ_QS_CAP 1024
 #define RQ_NB_MAX 1024
 #define RQ_K_MAX 2000u      // largest bucket whose triples a window indexes (C(2000,3) < 2^31)
+#define RQ_UNROLL 4         // window points a thread loads at once in the bucket passes
 
 struct RqArgs {
     const ps4_item *items;
@@ -402,10 +412,19 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             const u64 *pts = scr + (u64)b * a.bin_cap;
             for (u32 i = tid; i < RQ_W / 2; i += RQ_NT) s_cnt[i] = 0;
             __syncthreads();
-            // pass 1: count (exhaustive/s4.c:285-298)
-            for (u32 i = tid; i < npts; i += RQ_NT) {
-                const u32 s = (u32)(pts[i] & 0xFFFFu);
-                atomicAdd(&s_cnt[s >> 1], 1u << ((s & 1u) << 4));
+            // pass 1: count (exhaustive/s4.c:285-298); RQ_UNROLL points per thread
+            // loaded together (independent loads in flight), then counted
+            for (u32 i0 = tid; i0 < npts; i0 += RQ_UNROLL * RQ_NT) {
+                u32 sv[RQ_UNROLL];
+#pragma unroll
+                for (u32 u = 0; u < RQ_UNROLL; ++u) {
+                    const u32 i = i0 + u * RQ_NT;
+                    sv[u] = (u32)pts[i < npts ? i : i0] & 0xFFFFu;
+                }
+                PS4_KEEP("v"(sv[0]), "v"(sv[1]), "v"(sv[2]), "v"(sv[3]));   // all loads issued before the branches
+#pragma unroll
+                for (u32 u = 0; u < RQ_UNROLL; ++u)
+                    if (i0 + u * RQ_NT < npts) atomicAdd(&s_cnt[sv[u] >> 1], 1u << ((sv[u] & 1u) << 4));
             }
             __syncthreads();
             // pass 2: which S count, offsets (exhaustive/s4.c:300-305, 314-321)
@@ -475,13 +494,21 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             if (tid == 0) s_tp[tot_nq] = tot_tr;
             __syncthreads();
             // pass 3: fill the buckets of counted S only (exhaustive/s4.c:307-310)
-            for (u32 i = tid; i < npts; i += RQ_NT) {
-                const u64 pt = pts[i];
-                const u32 s = (u32)(pt & 0xFFFFu), sh = (s & 1u) << 4;
-                if (((s_cnt[s >> 1] >> sh) & 0xFFFFu) != 0xFFFFu) {
-                    const u32 old = atomicAdd(&s_cnt[s >> 1], 1u << sh);
-                    const u32 slot = (old >> sh) & 0xFFFFu;
-                    if (slot < tot_qr) s_pv[slot] = (u32)(pt >> 16);
+            for (u32 i0 = tid; i0 < npts; i0 += RQ_UNROLL * RQ_NT) {
+                u64 pv[RQ_UNROLL];
+#pragma unroll
+                for (u32 u = 0; u < RQ_UNROLL; ++u) {
+                    const u32 i = i0 + u * RQ_NT;
+                    pv[u] = pts[i < npts ? i : i0];
+                }
+#pragma unroll
+                for (u32 u = 0; u < RQ_UNROLL; ++u) {
+                    const u32 s = (u32)(pv[u] & 0xFFFFu), sh = (s & 1u) << 4;
+                    if (i0 + u * RQ_NT < npts && ((s_cnt[s >> 1] >> sh) & 0xFFFFu) != 0xFFFFu) {
+                        const u32 old = atomicAdd(&s_cnt[s >> 1], 1u << sh);
+                        const u32 slot = (old >> sh) & 0xFFFFu;
+                        if (slot < tot_qr) s_pv[slot] = (u32)(pv[u] >> 16);
+                    }
                 }
             }
             __syncthreads();
@@ -538,8 +565,9 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 // ----------------------------------------------------------------------------
 
 #define EX_NT 64          // one wave per workgroup: every barrier below is a wave's own
-#define EX_ARENA 1024     // doubles of divisor lists per pass; one 4-set with d(D) <= 16384
-                          // needs at most 918 (nL + nH over the best prefix split)
+#define EX_ARENA 992      // doubles of divisor lists per pass (the workgroup's LDS stays at
+                          // 16 KB); one 4-set with d(D) <= 16384 needs at most 748 (nL + nH
+                          // over the best prefix split, reached by the radices 2, 11, 33, 2, 11)
 #define EX_MAXF 12        // distinct odd primes of D < 2^51 (3*5*...*43 > 2^51)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
 #define EX_CHUNK 4        // tiles claimed per work-queue atomic
@@ -563,28 +591,25 @@ struct ExArgs {
     u32 hit_cap;
     u32 cap_divs;
     u32 n_rids;             // counter rows in ctr
+    double neg_m1;          // -262080 (a kernel argument, so that it sits in scalar registers)
 };
 
-// Per-4-set parameters of the candidate loop: one record of four 16-byte
+// Per-4-set parameters of the candidate loop: one record of three 16-byte
 // words per 4-set of a pass (read together, before any branch):
-//   q0 = {offL | nL << 10 | offH << 20,  nH | coff << 10 | lane << 23 | best << 29,
-//         ceil(2^23 / nL),  isqrt(2A) when 2A is a square, else 0}
-//         coff: the 4-set's first candidate in the pass; lane: its lane in the tile;
-//         ceil(2^23/nL) gives t / nL = (t << 9) * it >> 32 exactly for t < 8192
-//         and nL < 1024 (checked for every pair on the host)
+//   q0 = {aL | aL2 << 16,  aH | aH2 << 16,  magic,  (-8 nL mod 2^16) | coff << 16}
+//         aL, aL2: byte offsets in the arena of L[0] and L[nL-1]; aH, aH2: of
+//         H[0] and H[nH-1]; coff < 8192: the 4-set's first candidate in the
+//         pass; magic = ceil(2^23 / nL) gives t / nL = (t << 9) * magic >> 32
+//         exactly for t < 8192 and nL < 1024 (checked for every pair on the host)
 //   q1 = {mask mod 64 (lo, hi),  mask mod 63 (lo, hi)}
 //   q2 = {mask mod 65 bits 0..63 (lo, hi),  msm (lo, hi)}: msm bit 0 is bit 64 of
-//         the mod-65 mask, then the masks mod 11 (bit 1), 13 (12), 17 (25), 19 (42)
-//   q3 = {isqrt(A), F1, F2, F3}: the roots of the pair sums containing A
-#define EX_PAR_WORDS 4
+//         the mod-65 mask, then the masks mod 11 (bit 1), 13 (12), 17 (25), 19 (42);
+//         bits 61..63 hold the pair choice
+// Read only for queued candidates: s_mem[k] = {isqrt(A), F1, F2, F3} (the roots
+// of the pair sums containing A) and s_aux[k] = isqrt(2A) when 2A is a square,
+// else 0, | lane << 26 (the 4-set's lane in the tile).
+#define EX_PAR_WORDS 3
 
-// Takes values in registers at this point: an empty asm statement with them as
-// inputs, so the compiler issues their loads before it and cannot sink them
-// into branches of their own.  (The host emulation of these sources defines it
-// as nothing.)
-#ifndef PS4_KEEP
-#define PS4_KEEP(...) asm volatile("" ::__VA_ARGS__)
-#endif
 
 // bit r (< 64) of the 64-bit value hi:lo, as 0 or 1
 __device__ __forceinline__ u32 bit64(u32 lo, u32 hi, u32 r) { return (u32)((((u64)hi << 32) | lo) >> r) & 1u; }
@@ -682,6 +707,10 @@ __device__ __forceinline__ u32 ext_exact(double u, const ps4_quad *quads, u64 qi
                                       ps4_hit *hits, u32 hit_cap)
 {
     const ps4_quad qd = quads[qidx];
+    // every field in registers here: no load of this entry stays in flight past
+    // the drain (a pending load would make the candidate loop wait for all
+    // outstanding memory operations, its queue stores included, every iteration)
+    PS4_KEEP("v"(qd.r_ab), "v"(qd.r_ac), "v"(qd.r_ad), "v"(qd.r_bc), "v"(qd.r_bd), "v"(qd.r_cd), "v"(qd.rid));
     const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
     const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
     const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
@@ -725,22 +754,22 @@ __device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e,
 // One candidate: 4-set record k of the pass, candidate index t of that 4-set.
 // Straight-line: the record and the four list entries are loaded before the
 // tests (ex_rec, ex_lists), and every test is a select.  ok: the candidate
-// counts for ext_sqrt (the residue filters mod 64, 63, 65, e > 0, e not a
-// member).  The later filters (ex_late) run on the queued ok candidates.
+// passes the residue filters mod 64, 63, 65.  The queued ok candidates then
+// get the remaining tests of the ext_sqrt count (e > 0, e not a member) and
+// the later filters (ex_drain1).
 struct ExCand {
     double u;
-    u32 w1;
     bool ok;
 };
 
 struct ExRec {
-    uint4 q0, q1, q2, q3;
+    uint4 q0, q1, q2;
 };
 
 __device__ __forceinline__ ExRec ex_rec(const uint4 *s_par, u32 k)
 {
     const uint4 *pw = s_par + k * EX_PAR_WORDS;
-    return ExRec{pw[0], pw[1], pw[2], pw[3]};
+    return ExRec{pw[0], pw[1], pw[2]};
 }
 
 // The four list entries of candidate t: L[i], H[j], L[nL-1-i], H[nH-1-j].
@@ -748,33 +777,66 @@ struct ExLists {
     double l0, h0, l1, h1;
 };
 
-// g: the candidate's index in the pass; t = g - coff is its index in its 4-set.
-__device__ __forceinline__ ExLists ex_lists(const ExRec &R, const double *s_ar, u32 g)
+// (a mod 2^24) * (b mod 2^24) >> 32: the 24-bit multiply-high
+__device__ __forceinline__ u32 mulhi_u24(u32 a, u32 b)
 {
-    const u32 W0 = R.q0.x, W1 = R.q0.y;
-    const u32 offL = W0 & 1023u, cnL = (W0 >> 10) & 1023u, offH = (W0 >> 20) & 1023u;
-    const u32 cnH = W1 & 1023u;
-    const u32 t = g - ((W1 >> 10) & 8191u);
-    const u32 j = __umulhi(t << 9, R.q0.z);   // t / nL
-    const u32 i = t - j * cnL;
-    return ExLists{s_ar[offL + i], s_ar[offH + j], s_ar[offL + cnL - 1 - i], s_ar[offH + cnH - 1 - j]};
+    return (u32)(((u64)(a & 0xFFFFFFu) * (u64)(b & 0xFFFFFFu)) >> 32);
 }
 
-__device__ __forceinline__ ExCand ex_cand(const ExRec &R, const ExLists &Ls)
+// The double at byte offset off of the arena.
+__device__ __forceinline__ double ar_at(const double *s_ar, u32 off)
 {
-    const uint4 q0 = R.q0, q1 = R.q1, q2 = R.q2, q3 = R.q3;
-    const u32 W0 = q0.x, W1 = q0.y;
+    return *(const double *)((const char *)s_ar + off);
+}
+
+// g: the candidate's index in the pass; t = g - coff is its index in its 4-set,
+// i = t mod nL, j = t / nL.
+__device__ __forceinline__ ExLists ex_lists(const ExRec &R, const double *s_ar, u32 g)
+{
+    const uint4 q0 = R.q0;
+    const u32 t = g - (q0.w >> 16);
+    const u32 j = mulhi_u24(t << 9, q0.z);
+    const u32 i8 = (j * q0.w + (t << 3)) & 0xFFFFu;   // 8 (t - j nL) < 2^16 from the low halves
+    const u32 j8 = j << 3;
+    return ExLists{ar_at(s_ar, (q0.x & 0xFFFFu) + i8), ar_at(s_ar, (q0.y & 0xFFFFu) + j8),
+                   ar_at(s_ar, (q0.x >> 16) - i8), ar_at(s_ar, (q0.y >> 16) - j8)};
+}
+
+// neg_m1 = -M1, passed in by the launch so that it sits in scalar registers.
+__device__ __forceinline__ ExCand ex_cand(const ExRec &R, const ExLists &Ls, double neg_m1)
+{
+    const uint4 q1 = R.q1, q2 = R.q2;
     ExCand c;
     c.u = __builtin_fma(Ls.l1, Ls.h1, Ls.l0 * Ls.h0);   // H holds halves when u = (f+g)/2
-    c.w1 = W1;
-    const u32 r1 = rep_m1(c.u);
+    const double q = __builtin_floor(c.u * (1.0 / M1));
+    const u32 r1 = (u32)(__builtin_fma(q, neg_m1, c.u) + M1);   // rep_m1(c.u)
     const u32 r4095 = fold4095(r1);
     const u32 r63 = mod63(r4095), r65 = mod65(r4095);
     const u32 qrb = bit64(q1.x, q1.y, r1 & 63u) & bit64(q1.z, q1.w, r63) &
                     (r65 < 64u ? bit64(q2.x, q2.y, r65) : (q2.z & 1u));
-    const u32 ui = (u32)__builtin_fmin(c.u, 4294967295.0);   // roots are < 2^26
-    c.ok = (qrb != 0u) & (ui > q3.x) & (ui != q3.y) & (ui != q3.z) & (ui != q3.w) & (ui != q0.w);
+    c.ok = qrb != 0u;
     return c;
+}
+
+// Queue entries: an integer-valued u in [0, 2^52) and the record index k in
+// the top bits.  u + 2^52 has u as its mantissa (exact below 2^53).
+__device__ __forceinline__ u64 ex_qent(double u, u32 k)
+{
+    const u64 b = __builtin_bit_cast(u64, u + 4503599627370496.0);
+    return (b & ((1ull << 52) - 1ull)) | ((u64)k << 52);
+}
+__device__ __forceinline__ double ex_qu(u64 e)
+{
+    return __builtin_bit_cast(double, (e & ((1ull << 52) - 1ull)) | 0x4330000000000000ull) - 4503599627370496.0;
+}
+
+// The rest of the ext_sqrt tests for a queued candidate u of 4-set record k:
+// e = u^2 - A > 0 (u > isqrt(A)) and e not a member (u is not a root of a pair
+// sum containing A, and u^2 != 2A); every root is below 2^26 for S < 2^50.
+__device__ __forceinline__ bool ex_counted(double u, const uint4 &m, u32 aux)
+{
+    const u32 ui = (u32)__builtin_fmin(u, 4294967295.0);
+    return (ui > m.x) & (ui != m.y) & (ui != m.z) & (ui != m.w) & (ui != (aux & 0x3FFFFFFu));
 }
 
 // The later filters of a counted candidate u of record word q2: the residue
@@ -841,6 +903,8 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
     __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
     __shared__ u32 s_fac[EX_MAXF * EX_NT];     // odd prime factors of D: p << 5 | e, [slot][lane]
     __shared__ uint4 s_par[EX_NT * EX_PAR_WORDS];   // parameters of the 4-sets of the current pass
+    __shared__ uint4 s_mem[EX_NT];             // their e > 0 and member bounds
+    __shared__ u32 s_aux[EX_NT];               // isqrt(2A) or 0, | lane << 26
     __shared__ uint4 s_bits4[EX_CMAX / 128];   // bit g: a 4-set's candidates start at g
     u32 *s_bits = (u32 *)s_bits4;
 
@@ -883,7 +947,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                 acc_rid = uniform ? rid0 : 0xFFFFFFFFu;
             }
             u64 my_cand = 0, my_maxdiv = 0;
-            u32 my_esq = 0;
+            u32 my_esq = 0, my_unc = 0;   // ext_sqrt: residue-filter passes, minus the queued ones that fail the rest
             u32 nf = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0, best = 0;
             u64 pm64 = 0, pm63 = 0, pm65 = 0, pmsm = 0;
             u32 psqA = 0, pf1 = 0, pf2 = 0, pf3 = 0, pf4 = 0;
@@ -1045,11 +1109,13 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                 __syncthreads();
                 if (fits) {
                     uint4 *pw = s_par + slot * EX_PAR_WORDS;
-                    pw[0] = make_uint4(loff | (nL << 10) | ((loff + nL) << 20), nH | (coff << 10) | (tid << 23) | (best << 29),
-                                       (0x800000u + nL - 1u) / nL, pf4);
+                    const u32 aL = 8u * loff, aH = 8u * (loff + nL);
+                    pw[0] = make_uint4(aL | ((aL + 8u * (nL - 1u)) << 16), aH | ((aH + 8u * (nH - 1u)) << 16),
+                                       (0x800000u + nL - 1u) / nL, ((0u - 8u * nL) & 0xFFFFu) | (coff << 16));
                     pw[1] = make_uint4((u32)pm64, (u32)(pm64 >> 32), (u32)pm63, (u32)(pm63 >> 32));
-                    pw[2] = make_uint4((u32)pm65, (u32)(pm65 >> 32), (u32)pmsm, (u32)(pmsm >> 32));
-                    pw[3] = make_uint4(psqA, pf1, pf2, pf3);
+                    pw[2] = make_uint4((u32)pm65, (u32)(pm65 >> 32), (u32)pmsm, (u32)(pmsm >> 32) | (best << 29));
+                    s_mem[slot] = make_uint4(psqA, pf1, pf2, pf3);
+                    s_aux[slot] = pf4 | (tid << 26);
                     atomicOr(&s_bits[coff >> 5], 1u << (coff & 31u));
                     // mixed-radix lists over the factor sequence: stage s multiplies
                     // the list so far by p_s^1 .. p_s^e (earlier stages are the less
@@ -1087,43 +1153,55 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                 }
                 __syncthreads();
                 // lane l takes candidates l, l+64, l+128, ... of the pass, two per
-                // iteration (g and g + 64: two independent load/compute chains)
-                u32 kp = 0;   // 4-set starts before base
+                // iteration (g and g + 64: two independent load/compute chains).
+                // Pipelined: while this iteration's list entries load, the records
+                // of the next iteration and the bitmap words of the one after are
+                // read (LDS reads complete in order, so each wait covers only
+                // older reads).
+                u32 kp = 0;   // 4-set starts before the next fetched base
                 const u64 lane_mask = (2ull << tid) - 1ull;   // bits 0..tid
                 u32 qn1 = 0, qn2 = 0;                         // entries in the two queues
-                for (u32 base = 0; base < ctot; base += 128) {
-                    const uint4 wv = s_bits4[base >> 7];
+                // the records of candidates base + tid and base + 64 + tid from the
+                // bitmap words of base (the 4-set of candidate g starts at or before
+                // g, minus one); past the end of the pass a lane evaluates record 0
+                // at its first candidate (in range) and discards it
+                u32 ka, kb;
+                ExRec RA, RB;
+                auto fetch = [&](u32 base, const uint4 &wv) {
                     const u64 wa = (u64)wv.x | ((u64)wv.y << 32), wb = (u64)wv.z | ((u64)wv.w << 32);
-                    const u32 ga = base + tid, gb = ga + 64u;
-                    // the 4-set of candidate g: starts at or before g, minus one
-                    const u32 ka = kp + (u32)__popcll(wa & lane_mask) - 1u;
+                    ka = kp + (u32)__popcll(wa & lane_mask) - 1u;
                     kp += (u32)__popcll(wa);
-                    const u32 kb = kp + (u32)__popcll(wb & lane_mask) - 1u;
+                    kb = kp + (u32)__popcll(wb & lane_mask) - 1u;
                     kp += (u32)__popcll(wb);
-                    // past the end of the pass a lane evaluates record 0 at its first
-                    // candidate (in range) and discards it
+                    RA = ex_rec(s_par, base + tid < ctot ? ka : 0u);
+                    RB = ex_rec(s_par, base + 64u + tid < ctot ? kb : 0u);
+                };
+                // (read unconditionally: beyond the pass the bitmap holds stale bits,
+                // which only choose records of candidates that are discarded)
+                fetch(0u, s_bits4[0]);
+                uint4 wnext = s_bits4[1];
+                for (u32 base = 0; base < ctot; base += 128) {
+                    const u32 ga = base + tid, gb = ga + 64u;
                     const bool va = ga < ctot, vb = gb < ctot;
-                    const ExRec RA = ex_rec(s_par, va ? ka : 0u), RB = ex_rec(s_par, vb ? kb : 0u);
-                    PS4_KEEP("v"(RA.q1.x), "v"(RA.q1.y), "v"(RA.q1.z), "v"(RA.q1.w), "v"(RA.q2.x), "v"(RA.q2.y),
-                             "v"(RA.q2.z), "v"(RA.q2.w), "v"(RA.q3.x), "v"(RA.q3.y), "v"(RA.q3.z), "v"(RA.q3.w),
-                             "v"(RA.q0.z), "v"(RA.q0.w), "v"(RB.q1.x), "v"(RB.q1.y), "v"(RB.q1.z), "v"(RB.q1.w),
-                             "v"(RB.q2.x), "v"(RB.q2.y), "v"(RB.q2.z), "v"(RB.q2.w), "v"(RB.q3.x), "v"(RB.q3.y),
-                             "v"(RB.q3.z), "v"(RB.q3.w), "v"(RB.q0.z), "v"(RB.q0.w));   // both records
-                    const ExLists LA = ex_lists(RA, s_ar, va ? ga : 0u), LB = ex_lists(RB, s_ar, vb ? gb : 0u);
+                    const u32 ka0 = ka, kb0 = kb;
+                    const ExRec RA0 = RA, RB0 = RB;
+                    const ExLists LA = ex_lists(RA0, s_ar, va ? ga : 0u), LB = ex_lists(RB0, s_ar, vb ? gb : 0u);
+                    fetch(base + 128u, wnext);
+                    wnext = s_bits4[min((base + 256u) >> 7, EX_CMAX / 128u - 1u)];
                     PS4_KEEP("v"(LA.l0), "v"(LA.h0), "v"(LA.l1), "v"(LA.h1), "v"(LB.l0), "v"(LB.h0), "v"(LB.l1),
-                             "v"(LB.h1));   // both candidates' list loads in flight together
-                    const ExCand A = ex_cand(RA, LA);
-                    const ExCand B = ex_cand(RB, LB);
+                             "v"(LB.h1));   // both candidates' list entries (the next reads may still be in flight)
+                    const ExCand A = ex_cand(RA0, LA, a.neg_m1);
+                    const ExCand B = ex_cand(RB0, LB, a.neg_m1);
                     const bool oka = va & A.ok, okb = vb & B.ok;
                     if (uniform) {
                         my_esq += (oka ? 1u : 0u) + (okb ? 1u : 0u);
                     } else {   // a tile with several intervals (rare): count where each belongs
                         if (oka) {
-                            const u32 r = a.quads[tile * EX_NT + ((A.w1 >> 23) & 63u)].rid;
+                            const u32 r = a.quads[tile * EX_NT + (s_aux[ka0] >> 26)].rid;
                             atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
                         }
                         if (okb) {
-                            const u32 r = a.quads[tile * EX_NT + ((B.w1 >> 23) & 63u)].rid;
+                            const u32 r = a.quads[tile * EX_NT + (s_aux[kb0] >> 26)].rid;
                             atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
                         }
                     }
@@ -1132,9 +1210,9 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                     // run with all lanes busy
                     const u64 lane_lt = lane_mask >> 1;   // bits 0..tid-1
                     const u64 boa = __ballot(oka), bob = __ballot(okb);
-                    if (oka) exq1[qn1 + (u32)__popcll(boa & lane_lt)] = (u64)A.u | ((u64)ka << 52);
+                    if (oka) exq1[qn1 + (u32)__popcll(boa & lane_lt)] = ex_qent(A.u, ka0);
                     qn1 += (u32)__popcll(boa);
-                    if (okb) exq1[qn1 + (u32)__popcll(bob & lane_lt)] = (u64)B.u | ((u64)kb << 52);
+                    if (okb) exq1[qn1 + (u32)__popcll(bob & lane_lt)] = ex_qent(B.u, kb0);
                     qn1 += (u32)__popcll(bob);
                     const bool last = base + 128u >= ctot;
                     // exact tests of every entry of the second queue
@@ -1143,10 +1221,9 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                         for (u32 q = 0; q < qn2; q += EX_NT) {
                             if (q + tid < qn2) {
                                 const u64 e2 = exq2[q + tid];
-                                const u32 W1 = s_par[(u32)(e2 >> 52) * EX_PAR_WORDS].y;
-                                err |= ext_exact((double)(e2 & ((1ull << 52) - 1ull)), a.quads,
-                                                 tile * EX_NT + ((W1 >> 23) & 63u), W1 >> 29, a.ctl, a.hits,
-                                                 a.hit_cap);
+                                const u32 k = (u32)(e2 >> 52);
+                                err |= ext_exact(ex_qu(e2), a.quads, tile * EX_NT + (s_aux[k] >> 26),
+                                                 s_par[k * EX_PAR_WORDS + 2].w >> 29, a.ctl, a.hits, a.hit_cap);
                             }
                         }
                         __syncthreads();   // read before the next stores
@@ -1159,8 +1236,17 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                             u64 ent = 0;
                             if (q0 + tid < qn1) {
                                 ent = exq1[q0 + tid];
-                                const uint4 *pw = s_par + (u32)(ent >> 52) * EX_PAR_WORDS;
-                                go = ex_late((double)(ent & ((1ull << 52) - 1ull)), pw[2]);
+                                const u32 k = (u32)(ent >> 52);
+                                const double u = ex_qu(ent);
+                                const u32 aux = s_aux[k];
+                                if (ex_counted(u, s_mem[k], aux)) {
+                                    go = ex_late(u, s_par[k * EX_PAR_WORDS + 2]);
+                                } else if (uniform) {
+                                    ++my_unc;
+                                } else {   // take it back from its interval's count
+                                    const u32 r = a.quads[tile * EX_NT + (aux >> 26)].rid;
+                                    atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), ~0ull);
+                                }
                             }
                             const u64 bg = __ballot(go);
                             if (go) exq2[qn2 + (u32)__popcll(bg & lane_lt)] = ent;
@@ -1180,6 +1266,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
             if (uniform) {
                 acc_cand += my_cand;
                 acc_esq += my_esq;
+                acc_esq -= my_unc;   // lane sums may wrap; the wave's sum is exact
                 acc_md = my_maxdiv > acc_md ? my_maxdiv : acc_md;
             } else if (valid) {
                 unsigned long long *c = (unsigned long long *)(a.ctr + (u64)qd.rid * CT_N);
@@ -1220,6 +1307,7 @@ extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *q
     a.quads = quads; a.quad_cap = quad_cap; a.ctl = ctl; a.ftab = ftab; a.tab_n = tab_n; a.ndiv = ndiv; a.qr = qr;
     a.ctr = ctr;
     a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs; a.exq = exq;
+    a.neg_m1 = -M1;
     hipLaunchKernelGGL(k_ext, dim3(grid), dim3(EX_NT), 0, st, a);
     return hipGetLastError();
 }

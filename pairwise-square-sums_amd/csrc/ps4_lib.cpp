@@ -666,7 +666,7 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     if (cfg->cap_reps) ctx->cap_reps = cfg->cap_reps;
     if (cfg->cap_divs) ctx->cap_divs = cfg->cap_divs;
     // the extension kernel holds the divisor lists of any 4-set with d(D) <= 16384
-    // in its LDS arena (at most 918 entries); a larger cap would not fit
+    // in its LDS arena (at most 748 entries); a larger cap would not fit
     if (ctx->cap_divs > 16384) { ctx->last_error = "cap_divs above 16384 exceeds the divisor-list arena"; return PS4_E_ARG; }
     if (cfg->cap_ext) ctx->cap_ext = cfg->cap_ext;
     ctx->S_hi_max = cfg->S_hi_max;

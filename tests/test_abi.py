@@ -102,3 +102,48 @@ def test_division_magic_of_the_candidate_loop_is_exact():
         m = (0x800000 + d - 1) // d
         assert m < (1 << 24)
         assert np.array_equal(((t << np.uint64(9)) * np.uint64(m)) >> np.uint64(32), t // np.uint64(d)), d
+    assert (8191 << 9) < (1 << 24)   # both factors fit the 24-bit multiply
+
+
+def test_list_offsets_of_the_candidate_loop_are_exact():
+    """8 (t mod nL) is the low half of j * w + 8 t for j = t / nL and any word w
+    whose low half is -8 nL mod 2^16 (the record keeps the candidate offset in
+    the high half)."""
+    import numpy as np
+    t = np.arange(8192, dtype=np.uint64)
+    for d in range(1, 1024):
+        w = np.uint64(((-8 * d) & 0xFFFF) | (8191 << 16))
+        j = t // np.uint64(d)
+        got = (j * w + (t << np.uint64(3))) & np.uint64(0xFFFF)
+        assert np.array_equal(got, np.uint64(8) * (t % np.uint64(d))), d
+
+
+def test_divisor_list_arena_holds_every_4set():
+    """The largest nL + nH of one 4-set fits the extension kernel's arena: over
+    every factor sequence of D' (the power of 2 first, radix <= 49, then odd
+    primes in any order, radix <= 33) with d(D') <= 16384, the best prefix split
+    (first minimum of nL + nH, as the kernel chooses it) needs at most 748."""
+    src = open(os.path.join(REPO, "pairwise-square-sums_amd", "csrc", "ps4_kernels.hip")).read()
+    arena = int(re.search(r"#define EX_ARENA (\d+)", src).group(1))
+    worst = [0]
+
+    def split(seq, n):
+        best, nl = 1 + n, 1
+        for r in seq:
+            nl *= r
+            best = min(best, nl + n // nl)
+        return best
+
+    def rec(seq, prod):
+        if seq:
+            worst[0] = max(worst[0], split(seq, prod))
+        for r in range(2, (49 if not seq else 33) + 1):
+            if prod * r > 16384:
+                break
+            seq.append(r)
+            rec(seq, prod * r)
+            seq.pop()
+
+    rec([], 1)
+    assert worst[0] == 748
+    assert worst[0] <= arena
