@@ -217,10 +217,13 @@ struct RqArgs {
 // the 4-sets of both classes (exhaustive/s4.c:324-339).  The buckets are not
 // sorted: the three p values of each triple are put in ascending order here,
 // which is the order the classes are defined in (and makes every 4-set come
-// out a > b > c > d).  With WRITE false only counts.
+// out a > b > c > d).  With WRITE false only counts.  s_qv holds the q of each
+// representation (p^2 + q^2 = S), so q is read rather than recomputed; p
+// ascending is q descending.
 template <bool WRITE>
 __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u32 *s_tp, const u32 *s_pv,
-                       u32 nq, u64 B0, ps4_quad *out, u64 obase, u64 ocap, u32 rid, u32 verify, u32 *err)
+                       const u32 *s_qv, u32 nq, u64 B0, ps4_quad *out, u64 obase, u64 ocap, u32 rid, u32 verify,
+                       u32 *err)
 {
     if (c0 >= c1) return 0;
     u32 lo = 0, hi = nq;
@@ -246,8 +249,10 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
     u32 Sodd = (u32)(S & 1);
     u32 n = 0;
     u32 vx = s_pv[st + x], vy = s_pv[st + y];
+    u32 wx = WRITE ? s_qv[st + x] : 0u, wy = WRITE ? s_qv[st + y] : 0u;
     for (u32 c = c0; c < c1; ++c) {
         const u32 vz = s_pv[st + z];
+        const u32 wz = WRITE ? s_qv[st + z] : 0u;
         const u32 lo2 = min(vx, vy), hi2 = max(vx, vy);
         const u32 px = min(lo2, vz), pz = max(hi2, vz), py = max(lo2, min(hi2, vz));
         const double Px = (double)px * px, Py = (double)py * py, Pz = (double)pz * pz;
@@ -259,9 +264,8 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
         const bool cb = (par == Sodd) && (Pz + Pz != Sd) && (sxy + Pz > Sd);
         if (WRITE) {
             if (ca || cb) {
-                const u32 qx = (u32)__builtin_rint(__builtin_sqrt(Sd - Px));
-                const u32 qy = (u32)__builtin_rint(__builtin_sqrt(Sd - Py));
-                const u32 qz = (u32)__builtin_rint(__builtin_sqrt(Sd - Pz));
+                const u32 wlo = min(wx, wy), whi = max(wx, wy);
+                const u32 qx = max(whi, wz), qz = min(wlo, wz), qy = max(wlo, min(whi, wz));
                 if (ca) {
                     if (obase + n < ocap) {
                         uint4 *o = (uint4 *)(out + obase + n);
@@ -314,10 +318,16 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
                     x = 0;
                 }
                 y = x + 1;
-                if (c + 1 < c1) vx = s_pv[st + x];
+                if (c + 1 < c1) {
+                    vx = s_pv[st + x];
+                    if (WRITE) wx = s_qv[st + x];
+                }
             }
             z = y + 1;
-            if (c + 1 < c1) vy = s_pv[st + y];
+            if (c + 1 < c1) {
+                vy = s_pv[st + y];
+                if (WRITE) wy = s_qv[st + y];
+            }
         }
     }
     return n;
@@ -327,6 +337,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 {
     __shared__ u32 s_cnt[RQ_W / 2];        // u16 bucket counts, two per word; later offsets
     __shared__ u32 s_pv[RQ_PV_CAP];        // p of every representation of a counted S
+    __shared__ u32 s_qv[RQ_PV_CAP];        // and its q
     __shared__ u32 s_sk[RQ_QS_CAP];        // counted S: window offset | k << 16
     __shared__ u32 s_st[RQ_QS_CAP];        // first slot in s_pv
     __shared__ u32 s_tp[RQ_QS_CAP + 1];    // triple prefix
@@ -385,7 +396,10 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                         const u32 bin = (u32)(o >> RQ_LOGW);
                         if (bin >= nb) break;     // o < len implies bin < nb
                         const u32 slot = atomicAdd(&s_bin[bin], 1u);
-                        if (slot < a.bin_cap) scr[(u64)bin * a.bin_cap + slot] = (p << 16) | (o & (RQ_W - 1));
+                        // a point: offset in its window | p << 13 | q << 38 (p < 2^25 and
+                        // q < 2^26 for S < 2^50)
+                        if (slot < a.bin_cap)
+                            scr[(u64)bin * a.bin_cap + slot] = (qq << 38) | (p << RQ_LOGW) | (o & (RQ_W - 1));
                         o += 2 * qq + 1;
                         ++qq;
                     }
@@ -419,7 +433,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 #pragma unroll
                 for (u32 u = 0; u < RQ_UNROLL; ++u) {
                     const u32 i = i0 + u * RQ_NT;
-                    sv[u] = (u32)pts[i < npts ? i : i0] & 0xFFFFu;
+                    sv[u] = (u32)pts[i < npts ? i : i0] & (RQ_W - 1u);
                 }
                 PS4_KEEP("v"(sv[0]), "v"(sv[1]), "v"(sv[2]), "v"(sv[3]));   // all loads issued before the branches
 #pragma unroll
@@ -503,11 +517,14 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                 }
 #pragma unroll
                 for (u32 u = 0; u < RQ_UNROLL; ++u) {
-                    const u32 s = (u32)(pv[u] & 0xFFFFu), sh = (s & 1u) << 4;
+                    const u32 s = (u32)pv[u] & (RQ_W - 1u), sh = (s & 1u) << 4;
                     if (i0 + u * RQ_NT < npts && ((s_cnt[s >> 1] >> sh) & 0xFFFFu) != 0xFFFFu) {
                         const u32 old = atomicAdd(&s_cnt[s >> 1], 1u << sh);
                         const u32 slot = (old >> sh) & 0xFFFFu;
-                        if (slot < tot_qr) s_pv[slot] = (u32)(pv[u] >> 16);
+                        if (slot < tot_qr) {
+                            s_pv[slot] = (u32)(pv[u] >> RQ_LOGW) & 0x1FFFFFFu;
+                            s_qv[slot] = (u32)(pv[u] >> 38);
+                        }
                     }
                 }
             }
@@ -515,7 +532,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             // pass 4: triples -> 4-sets, count then write
             const u32 T = tot_tr;   // uniform
             const u32 c0 = (u32)(((u64)tid * T) / RQ_NT), c1 = (u32)(((u64)(tid + 1) * T) / RQ_NT);
-            const u32 n4 = rq_walk<false>(c0, c1, s_sk, s_st, s_tp, s_pv, tot_nq, B0, nullptr, 0, 0, 0, 0, &err);
+            const u32 n4 = rq_walk<false>(c0, c1, s_sk, s_st, s_tp, s_pv, s_qv, tot_nq, B0, nullptr, 0, 0, 0, 0, &err);
             u32 tot4;
             const u32 b4 = block_excl_scan<RQ_NT>(n4, s_w, &tot4);
             tot4 = uni(tot4);
@@ -531,7 +548,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                 err |= ERR_QUAD_OVF;
             } else {
                 if (n4)
-                    rq_walk<true>(c0, c1, s_sk, s_st, s_tp, s_pv, tot_nq, B0, a.quads, gb + b4, a.quad_cap, item.rid,
+                    rq_walk<true>(c0, c1, s_sk, s_st, s_tp, s_pv, s_qv, tot_nq, B0, a.quads, gb + b4, a.quad_cap, item.rid,
                                   a.verify, &err);
                 if (tid < tot4p - tot4) {
                     uint4 *o = (uint4 *)(a.quads + gb + tot4 + tid);
