@@ -131,6 +131,30 @@ __device__ __forceinline__ u32 block_excl_scan(u32 v, u32 *s_w, u32 *total)
     return pre + incl - v;
 }
 
+// Two exclusive block scans sharing their two barriers; *t1, *t2 get the sums.
+template <int NT>
+__device__ __forceinline__ void block_excl_scan2(u32 v1, u32 v2, u32 *s_w, u32 *e1, u32 *e2, u32 *t1, u32 *t2)
+{
+    const u32 i1 = wave_incl_scan(v1), i2 = wave_incl_scan(v2);
+    const u32 wid = threadIdx.x >> 6;
+    if (lane_id() == 63) { s_w[wid] = i1; s_w[NT / 64 + wid] = i2; }
+    __syncthreads();
+    u32 p1 = 0, p2 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+        const u32 a = s_w[w], b = s_w[NT / 64 + w];
+        p1 += ((u32)w < wid) ? a : 0u;
+        p2 += ((u32)w < wid) ? b : 0u;
+        s1 += a;
+        s2 += b;
+    }
+    __syncthreads();
+    *e1 = p1 + i1 - v1;
+    *e2 = p2 + i2 - v2;
+    *t1 = s1;
+    *t2 = s2;
+}
+
 // Block-wide OR of a predicate, uniform result (barriers only, no library call).
 template <int NT>
 __device__ __forceinline__ bool block_any(bool p, u32 *s_w)
@@ -182,7 +206,9 @@ __device__ __forceinline__ u64 block_max_u64(u64 v, u64 *s_w)
     return t;
 }
 
-__device__ __forceinline__ u32 tri3(u32 k) { return (u32)(((u64)k * (k - 1) * (k - 2)) / 6u); }
+// C(k,3) for 2 <= k <= 2000: C(k,2) * (k - 2) stays below 2^32 and is a
+// multiple of 3 (x / 3 = mulhi(x, 0xAAAAAAAB) >> 1 for every 32-bit x)
+__device__ __forceinline__ u32 tri3(u32 k) { return __umulhi((k * (k - 1) >> 1) * (k - 2), 0xAAAAAAABu) >> 1; }
 
 // ----------------------------------------------------------------------------
 // k_rq: representations and 4-sets
@@ -342,7 +368,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
     __shared__ u32 s_st[RQ_QS_CAP];        // first slot in s_pv
     __shared__ u32 s_tp[RQ_QS_CAP + 1];    // triple prefix
     __shared__ u32 s_bin[RQ_NB_MAX];       // points per window of the superwindow
-    __shared__ u32 s_w[RQ_NT / 64];
+    __shared__ u32 s_w[2 * RQ_NT / 64];
     __shared__ u64 s_w64[RQ_NT / 64];
     __shared__ u32 s_item;
     __shared__ u64 s_gbase;
@@ -444,17 +470,17 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             // pass 2: which S count, offsets (exhaustive/s4.c:300-305, 314-321)
             constexpr u32 SPT = RQ_W / RQ_NT;
             const u32 s0 = tid * SPT;
-            u32 nq = 0, qr = 0, tr = 0;
+            const u32 b0 = (u32)B0;   // S mod 4 = (b0 + s) mod 4
+            u32 nq = 0, qr = 0, tr = 0, w_pairs = 0, w_maxk = 0;
             bool big = false;
             for (u32 w = 0; w < SPT / 2; ++w) {
                 const u32 word = s_cnt[(s0 >> 1) + w];
 #pragma unroll
                 for (u32 h = 0; h < 2; ++h) {
                     const u32 k = (word >> (16 * h)) & 0xFFFFu;
-                    const u64 S = B0 + s0 + 2 * w + h;
-                    c_pairs += k;
-                    c_maxk = k > c_maxk ? k : c_maxk;
-                    if (k >= 3 && (S & 3) != 2) {
+                    w_pairs += k;
+                    w_maxk = k > w_maxk ? k : w_maxk;
+                    if (k >= 3 && ((b0 + s0 + 2 * w + h) & 3u) != 2u) {
                         ++nq;
                         qr += k;
                         if (k > a.cap_reps) err |= ERR_CAP_REPS;
@@ -463,6 +489,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                     }
                 }
             }
+            c_pairs += w_pairs;
+            c_maxk = w_maxk > c_maxk ? w_maxk : c_maxk;
             c_sge3 += nq;
             c_trip += tr;
             // two scans: counted S and their representations packed in one word
@@ -470,9 +498,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             // points per window), and the triples with bit 31 set by a bucket too
             // large for the 32-bit triple index (without one, the sum of C(k,3)
             // over <= 2560 representations stays below 2^31)
-            u32 tot_p, tot_t;
-            const u32 b_p = block_excl_scan<RQ_NT>(nq | (qr << 16), s_w, &tot_p);
-            u32 b_tr = block_excl_scan<RQ_NT>(big ? 0x80000000u : tr, s_w, &tot_t);
+            u32 tot_p, tot_t, b_p, b_tr;
+            block_excl_scan2<RQ_NT>(nq | (qr << 16), big ? 0x80000000u : tr, s_w, &b_p, &b_tr, &tot_p, &tot_t);
             tot_p = uni(tot_p);
             tot_t = uni(tot_t);
             u32 b_nq = b_p & 0xFFFFu, b_qr = b_p >> 16;
@@ -490,9 +517,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                 for (u32 h = 0; h < 2; ++h) {
                     const u32 k = (word >> (16 * h)) & 0xFFFFu;
                     const u32 s = s0 + 2 * w + h;
-                    const u64 S = B0 + s;
                     u32 o = 0xFFFFu;
-                    if (k >= 3 && (S & 3) != 2) {
+                    if (k >= 3 && ((b0 + s) & 3u) != 2u) {
                         s_sk[b_nq] = s | (k << 16);
                         s_st[b_nq] = b_qr;
                         s_tp[b_nq] = b_tr;

@@ -147,3 +147,14 @@ def test_divisor_list_arena_holds_every_4set():
     rec([], 1)
     assert worst[0] == 748
     assert worst[0] <= arena
+
+
+def test_triple_count_of_a_bucket_is_exact_in_32_bits():
+    """C(k,3) = mulhi(C(k,2) * (k - 2), 0xAAAAAAAB) >> 1 with every step in 32
+    bits for 3 <= k <= 2000 (the representation kernel's triple count of a
+    bucket; larger buckets are refused before it is used)."""
+    import math
+    for k in range(3, 2001):
+        x = (k * (k - 1) >> 1) * (k - 2)
+        assert x < 1 << 32
+        assert ((x * 0xAAAAAAAB) >> 32) >> 1 == math.comb(k, 3), k
