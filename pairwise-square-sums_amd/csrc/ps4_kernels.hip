@@ -260,7 +260,6 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
     u32 j = lo;
     u32 tau = c0 - s_tp[j];
     u32 sk = s_sk[j], k = sk >> 16, st = s_st[j];
-    u64 S = B0 + (sk & 0xFFFFu);
     u32 x = 0;
     for (;;) {
         const u32 m = k - 1 - x, c = m * (m - 1) / 2;
@@ -271,14 +270,14 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
     u32 y = x + 1;
     while (tau >= k - 1 - y) { tau -= k - 1 - y; ++y; }
     u32 z = y + 1 + tau;
-    double Sd = (double)S;
-    u32 Sodd = (u32)(S & 1);
+    const double B0d = (double)B0;
+    const u32 b0 = (u32)B0;
+    double Sd = B0d + (double)(sk & 0xFFFFu);
+    u32 Sodd = (b0 + sk) & 1u;
     u32 n = 0;
-    u32 vx = s_pv[st + x], vy = s_pv[st + y];
-    u32 wx = WRITE ? s_qv[st + x] : 0u, wy = WRITE ? s_qv[st + y] : 0u;
+    u32 vx = s_pv[st + x], vy = s_pv[st + y], vz = s_pv[st + z];
+    u32 wx = WRITE ? s_qv[st + x] : 0u, wy = WRITE ? s_qv[st + y] : 0u, wz = WRITE ? s_qv[st + z] : 0u;
     for (u32 c = c0; c < c1; ++c) {
-        const u32 vz = s_pv[st + z];
-        const u32 wz = WRITE ? s_qv[st + z] : 0u;
         const u32 lo2 = min(vx, vy), hi2 = max(vx, vy);
         const u32 px = min(lo2, vz), pz = max(hi2, vz), py = max(lo2, min(hi2, vz));
         const double Px = (double)px * px, Py = (double)py * py, Pz = (double)pz * pz;
@@ -321,7 +320,7 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
                         const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
                         const bool ok = va > vb && vb > vc && vc > vd && vd > 0 && vb + vc == rbc * rbc &&
                                         vb + vd == rbd * rbd && vc + vd == rcd * rcd && va + vd == rad * rad &&
-                                        va + vb + vc + vd == S;
+                                        va + vb + vc + vd == B0 + (sk & 0xFFFFu);
                         if (!ok) *err |= ERR_VERIFY;
                     }
                 }
@@ -329,31 +328,30 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
         } else {
             n += (u32)ca + (u32)cb;
         }
-        if (++z == k) {
-            if (++y == k - 1) {
-                if (++x == k - 2) {
-                    if (c + 1 < c1) {
-                        ++j;
-                        sk = s_sk[j];
-                        k = sk >> 16;
-                        st = s_st[j];
-                        S = B0 + (sk & 0xFFFFu);
-                        Sd = (double)S;
-                        Sodd = (u32)(S & 1);
-                    }
-                    x = 0;
-                }
-                y = x + 1;
-                if (c + 1 < c1) {
-                    vx = s_pv[st + x];
-                    if (WRITE) wx = s_qv[st + x];
-                }
-            }
-            z = y + 1;
-            if (c + 1 < c1) {
-                vy = s_pv[st + y];
-                if (WRITE) wy = s_qv[st + y];
-            }
+        // the next triple in lexicographic order: z advances, a z at k moves y
+        // on, a y at k - 1 moves x on, and an x at k - 2 moves to the next
+        // counted S (an exec-masked block, rare per lane).  The three p (and q)
+        // of the next triple are read together, so one LDS round trip per
+        // triple; every index stays inside the current or next S's bucket.
+        const bool tz = z + 1u == k, ty = tz && y + 2u == k, tn = ty && x + 3u == k;
+        if (tn && c + 1 < c1) {
+            ++j;
+            sk = s_sk[j];
+            k = sk >> 16;
+            st = s_st[j];
+            Sd = B0d + (double)(sk & 0xFFFFu);
+            Sodd = (b0 + sk) & 1u;
+        }
+        x = tn ? 0u : x + (ty ? 1u : 0u);
+        y = ty ? x + 1u : y + (tz ? 1u : 0u);
+        z = tz ? y + 1u : z + 1u;
+        vx = s_pv[st + x];
+        vy = s_pv[st + y];
+        vz = s_pv[st + z];
+        if (WRITE) {
+            wx = s_qv[st + x];
+            wy = s_qv[st + y];
+            wz = s_qv[st + z];
         }
     }
     return n;
