@@ -368,6 +368,14 @@ void order_extensions(Group &g)
     for (size_t i = 0; i < keyed.size(); ++i) g.ext[i] = keyed[i].second;
 }
 
+// What a finished batch leaves for the host: its interval rows, the 4-set
+// queue (only when 4: lines are wanted) and the extensions found.
+struct Pending {
+    u32 rid_lo = 0, rid_hi = 0;
+    std::vector<ps4_quad> qs;
+    std::vector<ps4_hit> hits;
+};
+
 int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink sink, void *const *users,
             void *single_user, ps4_counters *out)
 {
@@ -385,6 +393,135 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
     const bool noext = (ctx->flags & PS4_NOEXT) != 0;
     double budget_scale = 1.0;
     size_t next_iv = 0;
+    // the lines of a finished batch are made on the host while the next batch
+    // runs on the device (batches emit in order)
+    Pending pending;
+    bool have_pending = false;
+    auto emit = [&](const Pending &pb) -> int {
+        const u32 rid_lo = pb.rid_lo, rid_hi = pb.rid_hi;
+        std::vector<std::string> text(rid_hi - rid_lo + 1);
+        // ---- "4:" lines (exhaustive/s4.c:237-241), in the reference's single-thread order
+        if (!pb.qs.empty()) {
+            const u64 nq = pb.qs.size();
+            const std::vector<ps4_quad> &qs = pb.qs;
+            struct K { u64 S, px, py, pz; u32 cls; size_t i; };
+            std::vector<K> keys;
+            keys.reserve(nq);
+            for (u64 i = 0; i < nq; ++i) {
+                const ps4_quad &x = qs[i];
+                if (x.rid == PS4_RID_NONE) continue;   // padding of a window's block
+                if (x.rid < rid_lo || x.rid > rid_hi) { ctx->last_error = "4-set queue entry of another batch"; return PS4_E_INTERNAL; }
+                const u64 S = (u64)x.r_ab * x.r_ab + (u64)x.r_cd * x.r_cd;
+                const u32 cls = x.r_ad < x.r_bc ? 1u : 0u;
+                keys.push_back({S, x.r_cd, x.r_bd, cls ? x.r_ad : x.r_bc, cls, (size_t)i});
+            }
+            std::sort(keys.begin(), keys.end(), [&](const K &x, const K &y) {
+                if (qs[x.i].rid != qs[y.i].rid) return qs[x.i].rid < qs[y.i].rid;
+                if (x.S != y.S) return x.S < y.S;
+                if (x.px != y.px) return x.px < y.px;
+                if (x.py != y.py) return x.py < y.py;
+                if (x.pz != y.pz) return x.pz < y.pz;
+                return x.cls < y.cls;
+            });
+            char line[160];
+            for (const K &k : keys) {
+                const ps4_quad &x = qs[k.i];
+                if (dump_left[x.rid] <= 0) continue;
+                --dump_left[x.rid];
+                const u32 r6[6] = {x.r_ab, x.r_ac, x.r_ad, x.r_bc, x.r_bd, x.r_cd};
+                const Quad6 q = members(r6);
+                snprintf(line, sizeof line, "4: [%lld,%lld,%lld,%lld] S=%llu\n", (long long)q.a, (long long)q.b,
+                         (long long)q.c, (long long)q.d, (unsigned long long)q.S);
+                text[x.rid - rid_lo] += line;
+            }
+        }
+        // ---- extensions -> M:, 5: and 6-set lines (exhaustive/s4.c:245-277)
+        const u32 nh = (u32)pb.hits.size();
+        if (nh) {
+            const ps4_hit *hits = pb.hits.data();
+            // order: interval, S, then the 4-set's roots (one group per 4-set)
+            struct HK { u64 S, k1, k2, k3; u32 rid, i; };   // roots < 2^26: two per key word
+            std::vector<HK> hk(nh);
+            for (u32 i = 0; i < nh; ++i) {
+                const ps4_hit &h = hits[i];
+                const u64 S = (u64)h.r[0] * h.r[0] + (u64)h.r[5] * h.r[5];
+                hk[i] = {S, ((u64)h.r[0] << 26) | h.r[1], ((u64)h.r[2] << 26) | h.r[3], ((u64)h.r[4] << 26) | h.r[5],
+                         h.rid, i};
+            }
+            std::sort(hk.begin(), hk.end(), [](const HK &x, const HK &y) {
+                if (x.rid != y.rid) return x.rid < y.rid;
+                if (x.S != y.S) return x.S < y.S;
+                if (x.k1 != y.k1) return x.k1 < y.k1;
+                if (x.k2 != y.k2) return x.k2 < y.k2;
+                return x.k3 < y.k3;
+            });
+            std::vector<u32> idx(nh);
+            for (u32 i = 0; i < nh; ++i) idx[i] = hk[i].i;
+            auto key = [&](const ps4_hit &h) { return std::make_tuple(h.rid, h.r[0], h.r[1], h.r[2], h.r[3], h.r[4], h.r[5]); };
+            for (u32 i = 0; i < nh;) {
+                const ps4_hit &h0 = hits[idx[i]];
+                Group g;
+                g.rid = h0.rid;
+                g.q = members(h0.r);
+                g.best = h0.pad;
+                u32 j = i;
+                while (j < nh && key(hits[idx[j]]) == key(h0)) {
+                    const ps4_hit &h = hits[idx[j]];
+                    g.ext.push_back(((u128)h.e_hi << 64) | h.e_lo);
+                    ++j;
+                }
+                i = j;
+                ps4_counters &o = out[g.rid];
+                const u64 ne = g.ext.size();
+                if (ne > ctx->cap_ext) { ctx->last_error = "extension capacity exceeded"; return PS4_E_CAPACITY; }
+                o.max_extensions = std::max(o.max_extensions, ne);
+                order_extensions(g);
+                std::string &t = text[g.rid - rid_lo];
+                const Quad6 &q = g.q;
+                if (ne >= 2) {
+                    ++o.multi;
+                    std::string m = "M: [" + std::to_string((unsigned long long)q.a) + "," + std::to_string((unsigned long long)q.b) +
+                                    "," + std::to_string((unsigned long long)q.c) + "," + std::to_string((unsigned long long)q.d) +
+                                    "] ext=[";
+                    for (size_t k = 0; k < ne; ++k) { if (k) m += ","; m += dec(g.ext[k]); }
+                    m += "] S=" + std::to_string((unsigned long long)q.S) + "\n";
+                    t += m;
+                    if ((ctx->flags & PS4_TEST6) && !ctx->test6_fired) {
+                        ctx->test6_fired = true;
+                        const std::string l = format_set("TEST-6SET (NOT A SOLUTION, --test-sixset drill):",
+                                                         {q.a, q.b, q.c, q.d, g.ext[0], g.ext[1]}, q.S);
+                        if (ctx->alert) ctx->alert(ctx->alert_user, l.c_str()); else t += l;
+                    }
+                }
+                for (size_t k = 0; k < ne; ++k) {
+                    if (g.ext[k] <= q.a) continue;
+                    ++o.sets5;
+                    int lg = 0;
+                    for (u128 x = g.ext[k]; x > 1; x >>= 1) ++lg;
+                    ++o.hist5[lg > 99 ? 99 : lg];
+                    const u128 v5[5] = {q.a, q.b, q.c, q.d, g.ext[k]};
+                    append_set(t, "5:", v5, 5, q.S);
+                }
+                for (size_t k = 0; k < ne; ++k)
+                    for (size_t s = k + 1; s < ne; ++s)
+                        if (is_square(g.ext[k] + g.ext[s])) {
+                            ++o.sets6;
+                            const std::string l = format_set("!!!!!!!! FOUND 6-SET:", {q.a, q.b, q.c, q.d, g.ext[k], g.ext[s]}, q.S);
+                            if (ctx->alert) ctx->alert(ctx->alert_user, l.c_str()); else t += l;
+                        }
+            }
+        }
+        if (sink)
+            for (u32 r = rid_lo; r <= rid_hi; ++r) {
+                const std::string &t = text[r - rid_lo];
+                if (t.empty()) continue;
+                void *u = users ? users[r] : single_user;
+                if (sink(u, t.data(), t.size())) { ctx->last_error = "sink returned nonzero"; return PS4_E_SINK; }
+            }
+        return 0;
+    };
+    // an error after a launch: let the device finish before returning
+    auto fail = [&](int rc) { (void)hipStreamSynchronize(ctx->st); return rc; };
     for (;;) {
         while (next_iv < n && iv[next_iv].cur >= iv[next_iv].hi) ++next_iv;
         if (next_iv >= n) break;
@@ -450,6 +587,10 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
             if (int rc = dbg("k_ext")) return rc;
         HIP_OK(hipMemcpyAsync(ctx->h_ctl, ctx->d_ctl, sizeof(ps4_ctl), hipMemcpyDeviceToHost, ctx->st));
         HIP_OK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, (size_t)n_ctr * sizeof(u64), hipMemcpyDeviceToHost, ctx->st));
+        if (have_pending) {
+            have_pending = false;
+            if (int rc = emit(pending)) return fail(rc);
+        }
         HIP_OK(hipStreamSynchronize(ctx->st));
         float t_rq = 0.f, t_ext = 0.f;
         HIP_OK(hipEventElapsedTime(&t_rq, ctx->ev[0], ctx->ev[1]));
@@ -493,130 +634,27 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
             o.max_reps = std::max(o.max_reps, c[CT_MAX_REPS]);
             o.max_divisors = std::max(o.max_divisors, c[CT_MAX_DIVS]);
         }
-        std::vector<std::string> text(rid_hi - rid_lo + 1);
-        // ---- "4:" lines (exhaustive/s4.c:237-241), in the reference's single-thread order
+        // ---- this batch's 4-set dump and extensions, to be turned into lines
+        // while the next batch runs on the device
+        Pending cur;
+        cur.rid_lo = rid_lo;
+        cur.rid_hi = rid_hi;
         bool want_dump = false;
         for (u32 r = rid_lo; r <= rid_hi; ++r) want_dump |= dump_left[r] > 0;
         if (want_dump && ctx->h_ctl->quad_count) {
-            const u64 nq = ctx->h_ctl->quad_count;
-            std::vector<ps4_quad> qs(nq);
-            HIP_OK(hipMemcpy(qs.data(), ctx->d_quads, nq * sizeof(ps4_quad), hipMemcpyDeviceToHost));
-            struct K { u64 S, px, py, pz; u32 cls; size_t i; };
-            std::vector<K> keys;
-            keys.reserve(nq);
-            for (u64 i = 0; i < nq; ++i) {
-                const ps4_quad &x = qs[i];
-                if (x.rid == PS4_RID_NONE) continue;   // padding of a window's block
-                if (x.rid < rid_lo || x.rid > rid_hi) { ctx->last_error = "4-set queue entry of another batch"; return PS4_E_INTERNAL; }
-                const u64 S = (u64)x.r_ab * x.r_ab + (u64)x.r_cd * x.r_cd;
-                const u32 cls = x.r_ad < x.r_bc ? 1u : 0u;
-                keys.push_back({S, x.r_cd, x.r_bd, cls ? x.r_ad : x.r_bc, cls, (size_t)i});
-            }
-            std::sort(keys.begin(), keys.end(), [&](const K &x, const K &y) {
-                if (qs[x.i].rid != qs[y.i].rid) return qs[x.i].rid < qs[y.i].rid;
-                if (x.S != y.S) return x.S < y.S;
-                if (x.px != y.px) return x.px < y.px;
-                if (x.py != y.py) return x.py < y.py;
-                if (x.pz != y.pz) return x.pz < y.pz;
-                return x.cls < y.cls;
-            });
-            char line[160];
-            for (const K &k : keys) {
-                const ps4_quad &x = qs[k.i];
-                if (dump_left[x.rid] <= 0) continue;
-                --dump_left[x.rid];
-                const u32 r6[6] = {x.r_ab, x.r_ac, x.r_ad, x.r_bc, x.r_bd, x.r_cd};
-                const Quad6 q = members(r6);
-                snprintf(line, sizeof line, "4: [%lld,%lld,%lld,%lld] S=%llu\n", (long long)q.a, (long long)q.b,
-                         (long long)q.c, (long long)q.d, (unsigned long long)q.S);
-                text[x.rid - rid_lo] += line;
-            }
+            cur.qs.resize(ctx->h_ctl->quad_count);
+            HIP_OK(hipMemcpy(cur.qs.data(), ctx->d_quads, cur.qs.size() * sizeof(ps4_quad), hipMemcpyDeviceToHost));
         }
-        // ---- extensions -> M:, 5: and 6-set lines (exhaustive/s4.c:245-277)
-        const u32 nh = ctx->h_ctl->hit_count;
-        if (nh) {
+        if (const u32 nh = ctx->h_ctl->hit_count) {
             HIP_OK(hipMemcpy(ctx->h_hits, ctx->d_hits, (size_t)nh * sizeof(ps4_hit), hipMemcpyDeviceToHost));
-            // order: interval, S, then the 4-set's roots (one group per 4-set)
-            struct HK { u64 S, k1, k2, k3; u32 rid, i; };   // roots < 2^26: two per key word
-            std::vector<HK> hk(nh);
-            for (u32 i = 0; i < nh; ++i) {
-                const ps4_hit &h = ctx->h_hits[i];
-                const u64 S = (u64)h.r[0] * h.r[0] + (u64)h.r[5] * h.r[5];
-                hk[i] = {S, ((u64)h.r[0] << 26) | h.r[1], ((u64)h.r[2] << 26) | h.r[3], ((u64)h.r[4] << 26) | h.r[5],
-                         h.rid, i};
-            }
-            std::sort(hk.begin(), hk.end(), [](const HK &x, const HK &y) {
-                if (x.rid != y.rid) return x.rid < y.rid;
-                if (x.S != y.S) return x.S < y.S;
-                if (x.k1 != y.k1) return x.k1 < y.k1;
-                if (x.k2 != y.k2) return x.k2 < y.k2;
-                return x.k3 < y.k3;
-            });
-            std::vector<u32> idx(nh);
-            for (u32 i = 0; i < nh; ++i) idx[i] = hk[i].i;
-            auto key = [&](const ps4_hit &h) { return std::make_tuple(h.rid, h.r[0], h.r[1], h.r[2], h.r[3], h.r[4], h.r[5]); };
-            for (u32 i = 0; i < nh;) {
-                const ps4_hit &h0 = ctx->h_hits[idx[i]];
-                Group g;
-                g.rid = h0.rid;
-                g.q = members(h0.r);
-                g.best = h0.pad;
-                u32 j = i;
-                while (j < nh && key(ctx->h_hits[idx[j]]) == key(h0)) {
-                    const ps4_hit &h = ctx->h_hits[idx[j]];
-                    g.ext.push_back(((u128)h.e_hi << 64) | h.e_lo);
-                    ++j;
-                }
-                i = j;
-                ps4_counters &o = out[g.rid];
-                const u64 ne = g.ext.size();
-                if (ne > ctx->cap_ext) { ctx->last_error = "extension capacity exceeded"; return PS4_E_CAPACITY; }
-                o.max_extensions = std::max(o.max_extensions, ne);
-                order_extensions(g);
-                std::string &t = text[g.rid - rid_lo];
-                const Quad6 &q = g.q;
-                if (ne >= 2) {
-                    ++o.multi;
-                    std::string m = "M: [" + std::to_string((unsigned long long)q.a) + "," + std::to_string((unsigned long long)q.b) +
-                                    "," + std::to_string((unsigned long long)q.c) + "," + std::to_string((unsigned long long)q.d) +
-                                    "] ext=[";
-                    for (size_t k = 0; k < ne; ++k) { if (k) m += ","; m += dec(g.ext[k]); }
-                    m += "] S=" + std::to_string((unsigned long long)q.S) + "\n";
-                    t += m;
-                    if ((ctx->flags & PS4_TEST6) && !ctx->test6_fired) {
-                        ctx->test6_fired = true;
-                        const std::string l = format_set("TEST-6SET (NOT A SOLUTION, --test-sixset drill):",
-                                                         {q.a, q.b, q.c, q.d, g.ext[0], g.ext[1]}, q.S);
-                        if (ctx->alert) ctx->alert(ctx->alert_user, l.c_str()); else t += l;
-                    }
-                }
-                for (size_t k = 0; k < ne; ++k) {
-                    if (g.ext[k] <= q.a) continue;
-                    ++o.sets5;
-                    int lg = 0;
-                    for (u128 x = g.ext[k]; x > 1; x >>= 1) ++lg;
-                    ++o.hist5[lg > 99 ? 99 : lg];
-                    const u128 v5[5] = {q.a, q.b, q.c, q.d, g.ext[k]};
-                    append_set(t, "5:", v5, 5, q.S);
-                }
-                for (size_t k = 0; k < ne; ++k)
-                    for (size_t s = k + 1; s < ne; ++s)
-                        if (is_square(g.ext[k] + g.ext[s])) {
-                            ++o.sets6;
-                            const std::string l = format_set("!!!!!!!! FOUND 6-SET:", {q.a, q.b, q.c, q.d, g.ext[k], g.ext[s]}, q.S);
-                            if (ctx->alert) ctx->alert(ctx->alert_user, l.c_str()); else t += l;
-                        }
-            }
+            cur.hits.assign(ctx->h_hits, ctx->h_hits + nh);
         }
-        if (sink)
-            for (u32 r = rid_lo; r <= rid_hi; ++r) {
-                const std::string &t = text[r - rid_lo];
-                if (t.empty()) continue;
-                void *u = users ? users[r] : single_user;
-                if (sink(u, t.data(), t.size())) { ctx->last_error = "sink returned nonzero"; return PS4_E_SINK; }
-            }
+        pending = std::move(cur);
+        have_pending = true;
         budget_scale = std::min(1.0, budget_scale * 2.0);
     }
+    if (have_pending)
+        if (int rc = emit(pending)) return rc;
     return 0;
 }
 
