@@ -53,6 +53,10 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "pairwise-square-sums_amd"))
 
 PEAK_LANE_OPS = 256 * 64 * 2.4e9      # 256 CUs x 64 lanes x 2.4 GHz, 32-bit integer lane-ops/s
+# HBM bytes per k_ext launch, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
+# [1,1e10) census (4 launches), each counter scaled by what it reads for a 2 GiB device copy
+# (profiles/census_1e10_counters.md); reported for that workload only
+K_EXT_TRAFFIC = {(1, 10**10): 16.48e9}
 GOLDEN = os.path.join(REPO, "tests", "golden", "golden.json")
 
 
@@ -392,7 +396,8 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_ext", "achieved": ext_achieved / 1e12, "peak": PEAK_LANE_OPS / 1e12,
                      "unit": "Tlane-op/s", "frac": ext_achieved / PEAK_LANE_OPS,
                      "avg_launch_ms": ext_ms, "ops_per_launch": ext_ops / max(1, launches // args.steps),
-                     "traffic": None},
+                     "traffic": K_EXT_TRAFFIC.get((lo, hi)), "traffic_unit": "B/launch",
+                     "algorithmic_bytes_per_launch": 32 * c["4-sets"] / max(1, launches // args.steps)},
         "roofline_job": {"achieved": (rq_ops + ext_ops) * args.steps / wall / 1e12, "peak": PEAK_LANE_OPS / 1e12,
                          "unit": "Tlane-op/s", "frac": (rq_ops + ext_ops) * args.steps / wall / PEAK_LANE_OPS,
                          "k_rq_ms_per_step": ms_rq / args.steps, "k_ext_ms_per_step": ms_ext / args.steps},

@@ -612,8 +612,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 #define EX_MAXF 12        // distinct odd primes of D < 2^51 (3*5*...*43 > 2^51)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
 #define EX_CHUNK 4        // tiles claimed per work-queue atomic
-#define EX_Q1 192         // queue of candidates counted for ext_sqrt, per workgroup (drained
-                          // at >= 64; an iteration adds at most 128)
+#define EX_Q1 192         // queue of candidates past the residue filters, per workgroup
+                          // (drained at >= 64; an iteration adds at most 128)
 #define EX_Q2 128         // queue of candidates for the exact test (drained at >= 64; a
                           // drain round of the first queue adds at most 64)
 #define EX_QCAP (EX_Q1 + EX_Q2)
