@@ -606,17 +606,17 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 // ----------------------------------------------------------------------------
 
 #define EX_NT 64          // one wave per workgroup: every barrier below is a wave's own
-#define EX_ARENA 992      // doubles of divisor lists per pass (the workgroup's LDS stays at
+#define EX_ARENA 800      // doubles of divisor lists per pass (the workgroup's LDS stays at
                           // 16 KB); one 4-set with d(D) <= 16384 needs at most 748 (nL + nH
                           // over the best prefix split, reached by the radices 2, 11, 33, 2, 11)
 #define EX_MAXF 12        // distinct odd primes of D < 2^51 (3*5*...*43 > 2^51)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
 #define EX_CHUNK 4        // tiles claimed per work-queue atomic
-#define EX_Q1 192         // queue of candidates past the residue filters, per workgroup
-                          // (drained at >= 64; an iteration adds at most 128)
-#define EX_Q2 128         // queue of candidates for the exact test (drained at >= 64; a
-                          // drain round of the first queue adds at most 64)
-#define EX_QCAP (EX_Q1 + EX_Q2)
+#define EX_Q1 192         // LDS stack of candidates past the residue filters (rounds of 64
+                          // run while it holds >= 64, before an iteration adds at most 128)
+#define EX_Q2 128         // queue of candidates for the exact test, per workgroup in global
+                          // memory, kept across passes and tiles (drained at >= 64 and at
+                          // the end; a round of the first queue adds at most 64)
 
 struct ExArgs {
     const ps4_quad *quads;
@@ -628,7 +628,7 @@ struct ExArgs {
     const ps4_qrtab *qr;
     u64 *ctr;
     ps4_hit *hits;
-    u64 *exq;               // EX_QCAP entries per workgroup (both queues): u | record << 52
+    uint4 *exq;             // EX_Q2 entries per workgroup: {u (52 bits) | pair choice << 52, 4-set index}
     u32 hit_cap;
     u32 cap_divs;
     u32 n_rids;             // counter rows in ctr
@@ -939,7 +939,7 @@ __device__ __forceinline__ void ex_flush(u64 *ctr, u32 n_rids, u32 rid, u64 cand
 // unordered pair f != g exactly once, and f + g needs no ordering.
 // Counters are summed per wave over consecutive tiles of one interval and
 // added to the device counters when the interval changes and at the end.
-__global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
+__global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIMD: <= 168 VGPRs
 {
     __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
     __shared__ u32 s_fac[EX_MAXF * EX_NT];     // odd prime factors of D: p << 5 | e, [slot][lane]
@@ -947,6 +947,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
     __shared__ uint4 s_mem[EX_NT];             // their e > 0 and member bounds
     __shared__ u32 s_aux[EX_NT];               // isqrt(2A) or 0, | lane << 26
     __shared__ uint4 s_bits4[EX_CMAX / 128];   // bit g: a 4-set's candidates start at g
+    __shared__ u64 s_q1[EX_Q1];                // candidates past the residue filters: u | record << 52
     u32 *s_bits = (u32 *)s_bits4;
 
     const u32 tid = threadIdx.x;
@@ -956,13 +957,27 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
     const u64 ntiles = (nquads + EX_NT - 1) / EX_NT;
     const u64 nchunks = (ntiles + EX_CHUNK - 1) / EX_CHUNK;
     const ps4_qrtab *qr = a.qr;
-    u64 *exq1 = a.exq + (u64)blockIdx.x * EX_QCAP;   // counted candidates
-    u64 *exq2 = exq1 + EX_Q1;                        // candidates for the exact test
+    uint4 *exq2 = a.exq + (u64)blockIdx.x * EX_Q2;   // candidates for the exact test
+    u32 qn2 = 0;                                     // entries in it
 
     // this wave's counters of interval acc_rid (lane partial sums)
     u32 acc_rid = 0xFFFFFFFFu;
     u64 acc_cand = 0, acc_esq = 0, acc_md = 0;
     u32 err = 0;
+
+    // exact tests of every entry of the second queue
+    auto drain2 = [&]() {
+        __syncthreads();   // the queue's stores are visible to every lane
+        for (u32 q = 0; q < qn2; q += EX_NT) {
+            if (q + tid < qn2) {
+                const uint4 e = exq2[q + tid];
+                const double u = ex_qu((u64)e.x | ((u64)(e.y & 0xFFFFFu) << 32));
+                err |= ext_exact(u, a.quads, (u64)e.z | ((u64)e.w << 32), e.y >> 20, a.ctl, a.hits, a.hit_cap);
+            }
+        }
+        __syncthreads();   // read before the next stores
+        qn2 = 0;
+    };
 
     u64 pf_tile = ~0ull;   // the tile whose entries qpf holds (loaded during the previous tile)
     ps4_quad qpf;
@@ -1201,7 +1216,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                 // older reads).
                 u32 kp = 0;   // 4-set starts before the next fetched base
                 const u64 lane_mask = (2ull << tid) - 1ull;   // bits 0..tid
-                u32 qn1 = 0, qn2 = 0;                         // entries in the two queues
+                u32 qn1 = 0;                                  // entries in the first queue
                 // the records of candidates base + tid and base + 64 + tid from the
                 // bitmap words of base (the 4-set of candidate g starts at or before
                 // g, minus one); past the end of the pass a lane evaluates record 0
@@ -1246,58 +1261,55 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
                             atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
                         }
                     }
-                    // counted candidates wait in a queue for the later filters, and
-                    // the survivors in a second one for the exact test, so that both
-                    // run with all lanes busy
+                    // candidates past the residue filters wait on an LDS stack (their
+                    // order does not matter) for the rest of the ext_sqrt tests and the
+                    // later filters, which run 64 at a time with every lane busy; the
+                    // survivors go to the exact-test queue.  The workgroup is one wave
+                    // and its LDS operations complete in order, so a wave barrier (no
+                    // instruction: it keeps the compiler from moving memory operations
+                    // across it) orders the stack's writes before a round's reads.
                     const u64 lane_lt = lane_mask >> 1;   // bits 0..tid-1
-                    const u64 boa = __ballot(oka), bob = __ballot(okb);
-                    if (oka) exq1[qn1 + (u32)__popcll(boa & lane_lt)] = ex_qent(A.u, ka0);
-                    qn1 += (u32)__popcll(boa);
-                    if (okb) exq1[qn1 + (u32)__popcll(bob & lane_lt)] = ex_qent(B.u, kb0);
-                    qn1 += (u32)__popcll(bob);
-                    const bool last = base + 128u >= ctot;
-                    // exact tests of every entry of the second queue
-                    auto drain2 = [&]() {
-                        __syncthreads();   // the queue's stores are visible to every lane
-                        for (u32 q = 0; q < qn2; q += EX_NT) {
-                            if (q + tid < qn2) {
-                                const u64 e2 = exq2[q + tid];
-                                const u32 k = (u32)(e2 >> 52);
-                                err |= ext_exact(ex_qu(e2), a.quads, tile * EX_NT + (s_aux[k] >> 26),
-                                                 s_par[k * EX_PAR_WORDS + 2].w >> 29, a.ctl, a.hits, a.hit_cap);
+                    auto round1 = [&]() {   // the top min(64, qn1) entries
+                        __builtin_amdgcn_wave_barrier();
+                        const u32 m = qn1 < 64u ? qn1 : 64u, b1 = qn1 - m;
+                        bool go = false;
+                        u64 ent = 0;
+                        u32 aux = 0;
+                        if (tid < m) {
+                            ent = s_q1[b1 + tid];
+                            const u32 k = (u32)(ent >> 52);
+                            const double u = ex_qu(ent);
+                            aux = s_aux[k];
+                            if (ex_counted(u, s_mem[k], aux)) {
+                                const uint4 q2 = s_par[k * EX_PAR_WORDS + 2];
+                                go = ex_late(u, q2);
+                                aux = (aux >> 26) | ((q2.w >> 29) << 6);   // lane | pair choice << 6
+                            } else if (uniform) {
+                                ++my_unc;
+                            } else {   // take it back from its interval's count
+                                const u32 r = a.quads[tile * EX_NT + (aux >> 26)].rid;
+                                atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), ~0ull);
                             }
                         }
-                        __syncthreads();   // read before the next stores
-                        qn2 = 0;
+                        const u64 bg = __ballot(go);
+                        if (go) {
+                            const u64 qi = tile * EX_NT + (aux & 63u);
+                            exq2[qn2 + (u32)__popcll(bg & lane_lt)] =
+                                make_uint4((u32)ent, ((u32)(ent >> 32) & 0xFFFFFu) | ((aux >> 6) << 20), (u32)qi,
+                                           (u32)(qi >> 32));
+                        }
+                        qn2 += (u32)__popcll(bg);
+                        qn1 = b1;
+                        if (qn2 >= 64u) drain2();
                     };
-                    if (qn1 >= 64u || (last && (qn1 | qn2) != 0u)) {
-                        __syncthreads();   // the queue's stores are visible to every lane
-                        for (u32 q0 = 0; q0 < qn1; q0 += EX_NT) {
-                            bool go = false;
-                            u64 ent = 0;
-                            if (q0 + tid < qn1) {
-                                ent = exq1[q0 + tid];
-                                const u32 k = (u32)(ent >> 52);
-                                const double u = ex_qu(ent);
-                                const u32 aux = s_aux[k];
-                                if (ex_counted(u, s_mem[k], aux)) {
-                                    go = ex_late(u, s_par[k * EX_PAR_WORDS + 2]);
-                                } else if (uniform) {
-                                    ++my_unc;
-                                } else {   // take it back from its interval's count
-                                    const u32 r = a.quads[tile * EX_NT + (aux >> 26)].rid;
-                                    atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), ~0ull);
-                                }
-                            }
-                            const u64 bg = __ballot(go);
-                            if (go) exq2[qn2 + (u32)__popcll(bg & lane_lt)] = ent;
-                            qn2 += (u32)__popcll(bg);
-                            if (qn2 >= 64u) drain2();
-                        }
-                        if (last && qn2 > 0u) drain2();
-                        __syncthreads();   // read before the next stores
-                        qn1 = 0;
-                    }
+                    while (qn1 >= 64u) round1();   // then at most 63 + 128 after this iteration
+                    const u64 boa = __ballot(oka), bob = __ballot(okb);
+                    if (oka) s_q1[qn1 + (u32)__popcll(boa & lane_lt)] = ex_qent(A.u, ka0);
+                    qn1 += (u32)__popcll(boa);
+                    if (okb) s_q1[qn1 + (u32)__popcll(bob & lane_lt)] = ex_qent(B.u, kb0);
+                    qn1 += (u32)__popcll(bob);
+                    if (base + 128u >= ctot)   // the pass's records are about to be replaced
+                        while (qn1 > 0u) round1();
                 }
                 __syncthreads();
                 if (fits) pending = false;
@@ -1318,6 +1330,7 @@ __global__ __launch_bounds__(EX_NT) void k_ext(ExArgs a)
         }
         chunk = uni(next);
     }
+    if (qn2 > 0u) drain2();
     ex_flush(a.ctr, a.n_rids, acc_rid, acc_cand, acc_esq, acc_md);
     if (err) atomicOr(&a.ctl->err, err);
 }
@@ -1347,7 +1360,7 @@ extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *q
     a.n_rids = n_rids;
     a.quads = quads; a.quad_cap = quad_cap; a.ctl = ctl; a.ftab = ftab; a.tab_n = tab_n; a.ndiv = ndiv; a.qr = qr;
     a.ctr = ctr;
-    a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs; a.exq = exq;
+    a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs; a.exq = (uint4 *)exq;
     a.neg_m1 = -M1;
     hipLaunchKernelGGL(k_ext, dim3(grid), dim3(EX_NT), 0, st, a);
     return hipGetLastError();
@@ -1374,7 +1387,7 @@ extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u
 
 extern "C" int ps4_rq_threads(void) { return RQ_NT; }
 extern "C" int ps4_ext_threads(void) { return EX_NT; }
-extern "C" int ps4_ext_queue_cap(void) { return EX_QCAP; }
+extern "C" int ps4_ext_queue_cap(void) { return 2 * EX_Q2; }   // u64 words per workgroup
 extern "C" int ps4_rq_window(void) { return (int)RQ_W; }
 extern "C" int ps4_rq_nb_max(void) { return RQ_NB_MAX; }
 

@@ -71,6 +71,9 @@ extern thread_local dim3 t_tid, t_bid, t_bdim, t_gdim;
 #define gridDim (emu::t_gdim)
 
 static inline void __syncthreads() { emu::g_block->bar.wait(); }
+// a wave barrier; only kernels whose workgroup is one wave use it, so it is
+// the workgroup barrier here
+static inline void __builtin_amdgcn_wave_barrier() { emu::g_block->bar.wait(); }
 
 static inline int emu_block_reduce(int v, bool is_or)
 {
