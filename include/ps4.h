@@ -112,6 +112,17 @@ int ps4_set_dump4(ps4_ctx *ctx, int64_t limit);
 int ps4_extend_triple_batch(ps4_ctx *ctx, size_t n, const uint64_t *abc, uint64_t d_max,
                             uint64_t *d_out, size_t cap, size_t *n_out);
 
+/* Element-range search: every 4-set {a < b < c < d} with all six pair sums
+ * square and d <= n_max, written through the sink as "[a,b,c,d]\n" lines in
+ * ascending order of (a, b, c, d); *n_sets gets their number and *n_triples
+ * the number of square triples {a < b < c} (a+b, a+c, b+c square) with
+ * c <= n_max.  The reference ranges over S only (exhaustive/s4.c:416-426);
+ * this is the same census (process_chunk, :280-342) over S < 4 n_max - 5,
+ * filtered by largest member, without extensions.  Needs S_hi_max >=
+ * 4 n_max - 6 and n_max <= 10^10. */
+int ps4_search_elements(ps4_ctx *ctx, uint64_t n_max, ps4_sink sink, void *user, uint64_t *n_sets,
+                        uint64_t *n_triples);
+
 /* Device time of the census kernels (ms, HIP events on the context's stream)
  * accumulated since the last reset, and the number of batches launched.  No
  * reference counterpart (the reference reports per-thread seconds,

@@ -457,3 +457,80 @@ int64_t ora_extend_triple(const uint64_t abc[3], uint64_t d_max, uint64_t *d_out
     free(found);
     return nf;
 }
+
+/* ------------------------------------------------------------------ */
+/* element-range mode (no s4.c entry point: the plain search by largest  */
+/* element of the task's first configuration), by brute force           */
+/* ------------------------------------------------------------------ */
+
+/* Square triples {a < b < c}: a+b = x^2, a+c = y^2, b+c = z^2 with x < y < z,
+ * x+y+z even, a = (x^2+y^2-z^2)/2 >= 1, c = (y^2+z^2-x^2)/2 <= n_max; every z
+ * tried one by one. */
+uint64_t ora_square_triples(uint64_t n_max)
+{
+    u64 n = 0;
+    for (u64 x = 1; x * x <= 2 * n_max; x++)
+        for (u64 y = x + 1; y * y <= 2 * n_max + x * x; y++)
+            for (u64 z = y + 1; z * z < x * x + y * y; z++) {
+                if ((x + y + z) & 1) continue;
+                const u64 c2 = y * y + z * z - x * x;
+                if (c2 > 2 * n_max) break;
+                n++;
+            }
+    return n;
+}
+
+typedef struct { u64 v[4]; } ora_set4;
+
+static int cmp_set4(const void *p, const void *q)
+{
+    const ora_set4 *a = (const ora_set4 *)p, *b = (const ora_set4 *)q;
+    for (int i = 0; i < 4; i++)
+        if (a->v[i] != b->v[i]) return a->v[i] < b->v[i] ? -1 : 1;
+    return 0;
+}
+
+static int is_square_u64(u64 n)
+{
+    const u64 r = ora_isqrt64(n);
+    return r * r == n;
+}
+
+/* Every 4-set {a < b < c < d} with six square pair sums and d <= n_max: each
+ * square triple {a, b, c} as above, extended by every d in (c, n_max] with
+ * d + c = w^2 and d + a, d + b square. */
+int64_t ora_elements(uint64_t n_max, ora_line_fn fn, void *user)
+{
+    size_t cap = 1024, n = 0;
+    ora_set4 *sets = (ora_set4 *)malloc(cap * sizeof(ora_set4));
+    if (!sets) return -1;
+    for (u64 x = 1; x * x <= 2 * n_max; x++)
+        for (u64 y = x + 1; y * y <= 2 * n_max + x * x; y++)
+            for (u64 z = y + 1; z * z < x * x + y * y; z++) {
+                if ((x + y + z) & 1) continue;
+                if (y * y + z * z - x * x > 2 * n_max) break;
+                const u64 A = (x * x + y * y - z * z) / 2, B = x * x - A, C = y * y - A;   /* A < B < C */
+                for (u64 w = ora_isqrt64(2 * C) + 1; w * w - C <= n_max; w++) {
+                    const u64 d = w * w - C;
+                    if (d <= C) continue;
+                    if (!is_square_u64(d + A) || !is_square_u64(d + B)) continue;
+                    if (n == cap) {
+                        cap *= 2;
+                        ora_set4 *t = (ora_set4 *)realloc(sets, cap * sizeof(ora_set4));
+                        if (!t) { free(sets); return -1; }
+                        sets = t;
+                    }
+                    sets[n].v[0] = A; sets[n].v[1] = B; sets[n].v[2] = C; sets[n].v[3] = d;
+                    n++;
+                }
+            }
+    qsort(sets, n, sizeof(ora_set4), cmp_set4);
+    char line[128];
+    for (size_t i = 0; i < n && fn; i++) {
+        snprintf(line, sizeof line, "[%llu,%llu,%llu,%llu]\n", (unsigned long long)sets[i].v[0],
+                 (unsigned long long)sets[i].v[1], (unsigned long long)sets[i].v[2], (unsigned long long)sets[i].v[3]);
+        fn(user, line);
+    }
+    free(sets);
+    return (int64_t)n;
+}

@@ -55,6 +55,14 @@ uint32_t ora_num_divisors(uint64_t n);
  * fact_add(m2) (exhaustive/s4.c:140-156).  Returns the count, or -1 if > cap. */
 int ora_divisors_of_product(uint64_t m1, uint64_t m2, uint64_t *out, int cap);
 
+/* Element-range mode (no s4.c entry point; the plain search by largest
+ * element): the number of square triples {a<b<c} (a+b, a+c, b+c square) with
+ * c <= n_max, and every 4-set {a<b<c<d} with six square pair sums and
+ * d <= n_max as "[a,b,c,d]\n" lines in ascending order (returns their count,
+ * -1 when out of memory).  Brute force over the roots of a+b, a+c, b+c. */
+uint64_t ora_square_triples(uint64_t n_max);
+int64_t ora_elements(uint64_t n_max, ora_line_fn fn, void *user);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,10 @@ def load():
         lib.ora_num_divisors.argtypes = [u64]
         lib.ora_num_divisors.restype = ctypes.c_uint32
         lib.ora_divisors_of_product.argtypes = [u64, u64, ctypes.POINTER(u64), ctypes.c_int]
+        lib.ora_square_triples.argtypes = [u64]
+        lib.ora_square_triples.restype = u64
+        lib.ora_elements.argtypes = [u64, LINE, ctypes.c_void_p]
+        lib.ora_elements.restype = ctypes.c_int64
         _lib = lib
     return _lib
 
@@ -99,3 +103,22 @@ def divisors_of_product(m1, m2, cap=1 << 16):
     if n < 0:
         raise ValueError("too many divisors")
     return [int(out[i]) for i in range(n)]
+
+
+def square_triples(n_max):
+    """Number of square triples {a<b<c} with c <= n_max (brute force)."""
+    return int(load().ora_square_triples(int(n_max)))
+
+
+def elements(n_max):
+    """Every 4-set with largest element <= n_max, as sorted "[a,b,c,d]" lines (brute force)."""
+    lib = load()
+    lines = []
+
+    def cb(_u, line):
+        lines.append(line.decode().rstrip("\n"))
+    f = LINE(cb)
+    n = lib.ora_elements(int(n_max), f, None)
+    if n < 0:
+        raise RuntimeError("oracle element search failed")
+    return lines

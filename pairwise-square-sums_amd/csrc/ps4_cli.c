@@ -5,6 +5,13 @@
  *
  *   ps4 S_lo S_hi [threads] [log2 chunk] [--out FILE] [--ckpt FILE] [--resume]
  *                 [--noext] [--verify] [--dump4 N] [--test-sixset] [--device N]
+ *   ps4 --elements N [--out FILE] [--device N]
+ *
+ * --elements N (no s4 counterpart: the element-range form of the search) prints
+ * every 4-set {a<b<c<d} with six square pair sums and d <= N as "[a,b,c,d]"
+ * lines in ascending order, and on stderr one line
+ * "ELEMENTS: n_max=N sets=... square_triples=..." (square triples {a<b<c}
+ * with c <= N).
  *
  * Same result lines on stdout (or appended to --out), same stderr reports
  * (header, progress, per-thread lines, TOTAL, 5-set histogram, CAPACITY), same
@@ -154,11 +161,37 @@ static int read_ckpt(ull S_lo, ull S_hi, int T, ull W, region_t *rg)
     return done ? 2 : 1;
 }
 
+static int run_elements(ull n_max, int device, const char *out)
+{
+    if (out) { out_fp = fopen(out, "a"); if (!out_fp) { perror(out); return 1; } }
+    else out_fp = stdout;
+    ps4_config cfg;
+    memset(&cfg, 0, sizeof cfg);
+    cfg.device = device;
+    cfg.S_hi_max = n_max >= 4 ? 4 * n_max - 5 : 2;
+    ps4_ctx *ctx = NULL;
+    int rc = ps4_open(&cfg, &ctx);
+    if (rc) { fprintf(stderr, "FATAL: %s: %s\n", ps4_strerror(rc), ctx ? ps4_last_error(ctx) : ""); return 2; }
+    uint64_t ns = 0, nt = 0;
+    rc = ps4_search_elements(ctx, n_max, sink_out, NULL, &ns, &nt);
+    if (rc) {
+        fprintf(stderr, "FATAL: %s: %s\n", ps4_strerror(rc), ps4_last_error(ctx));
+        ps4_close(ctx);
+        return rc == PS4_E_ARG ? 1 : 2;
+    }
+    fflush(out_fp);
+    fprintf(stderr, "ELEMENTS: n_max=%llu sets=%llu square_triples=%llu\n", n_max, (ull)ns, (ull)nt);
+    ps4_close(ctx);
+    if (out) fclose(out_fp);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     ull S_lo = 1, S_hi = 10000000ull, W = 1ull << 20;
     int T = 0, pos = 0, device = 0, opt_noext = 0, opt_verify = 0, opt_resume = 0, opt_test6 = 0;
     long long opt_dump4 = 0;
+    ull opt_elements = 0;
     const char *opt_out = NULL;
     for (int i = 1; i < argc; i++) {
         if (!strcmp(argv[i], "--noext")) opt_noext = 1;
@@ -169,6 +202,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "--out") && i + 1 < argc) opt_out = argv[++i];
         else if (!strcmp(argv[i], "--ckpt") && i + 1 < argc) opt_ckpt = argv[++i];
         else if (!strcmp(argv[i], "--device") && i + 1 < argc) device = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "--elements") && i + 1 < argc) opt_elements = strtoull(argv[++i], 0, 10);
         else {
             pos++;
             if (pos == 1) S_lo = strtoull(argv[i], 0, 10);
@@ -177,6 +211,7 @@ int main(int argc, char **argv)
             else if (pos == 4) W = 1ull << atoi(argv[i]);
         }
     }
+    if (opt_elements) return run_elements(opt_elements, device, opt_out);
     if (S_lo < 1) S_lo = 1;
     if (S_hi < S_lo) { fprintf(stderr, "S_hi < S_lo\n"); return 1; }
     if (T <= 0) {

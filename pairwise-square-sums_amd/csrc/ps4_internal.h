@@ -25,6 +25,12 @@ typedef struct {
 /* rid of a null entry: each window's block of the queue is padded to a whole
  * number of 64-entry tiles with entries carrying this rid (and zero roots). */
 #define PS4_RID_NONE 0xFFFFFFFFu
+// element-range mode: qualifying 4-sets one batch can return (the batch's
+// 4-set budget is held below it)
+#define PS4_ELEM_CAP (1u << 25)
+// largest n_max of element-range mode (the square-triple count takes about
+// sqrt(2 n_max) threads of up to sqrt(2 n_max) steps)
+#define PS4_ELEM_NMAX 10000000000ull
 
 /* An extension e of a 4-set (e + each member is square). */
 typedef struct {

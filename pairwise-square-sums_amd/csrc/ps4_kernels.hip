@@ -23,6 +23,7 @@
 // All integer work is exact: uint64, doubles only where every value is an
 // integer below 2^53, and unsigned __int128 for e = u^2 - A.
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #include "ps4_internal.h"
@@ -1508,6 +1509,73 @@ extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u
 
 // Fill n 32-bit words with a value (used instead of memset so launch
 // sequences stay free of memset nodes).
+// ----------------------------------------------------------------------------
+// element-range mode: the 4-sets of a census whose largest member is <= n_max,
+// and the square triples {a < b < c} with c <= n_max
+// ----------------------------------------------------------------------------
+
+// Copy the queued 4-sets with largest member a = (r_ab^2 + r_ac^2 - r_bc^2) / 2
+// <= n_max to out (grid-stride; slots by atomic, *count may exceed cap, which
+// the host reports).
+__global__ void k_elem(const ps4_quad *quads, const ps4_ctl *ctl, u64 quad_cap, u64 n_max, ps4_quad *out,
+                       u32 cap, u32 *count)
+{
+    u64 nq = ctl->quad_count;
+    if (nq > quad_cap) nq = quad_cap;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (u64)gridDim.x * blockDim.x) {
+        const ps4_quad q = quads[i];
+        if (q.rid == PS4_RID_NONE) continue;   // padding of a window's block
+        const u64 a = ((u64)q.r_ab * q.r_ab + (u64)q.r_ac * q.r_ac - (u64)q.r_bc * q.r_bc) >> 1;
+        if (a > n_max) continue;
+        const u32 slot = atomicAdd(count, 1u);
+        if (slot < cap) out[slot] = q;
+    }
+}
+
+// Square triples {a < b < c}: a + b = x^2, a + c = y^2, b + c = z^2 with
+// x < y < z, x + y + z even, a = (x^2 + y^2 - z^2) / 2 >= 1 and
+// c = (y^2 + z^2 - x^2) / 2 <= n_max.  For one x and each y the admissible z
+// form the range y < z <= min(isqrt(x^2 + y^2 - 2), isqrt(2 n_max + x^2 - y^2))
+// of one parity, counted in closed form; y stops where the range is empty for
+// good (both bounds fall with y).  One thread per x (grid-stride), a sum per
+// wave added to *total.
+__global__ void k_sqtri(u64 n_max, u64 x_max, unsigned long long *total)
+{
+    u64 acc = 0;
+    for (u64 x = 1 + (u64)blockIdx.x * blockDim.x + threadIdx.x; x <= x_max; x += (u64)gridDim.x * blockDim.x) {
+        const u64 xx = x * x;
+        for (u64 y = x + 1;; ++y) {
+            const u64 yy = y * y;
+            if (2 * y + 3 > xx) break;                 // a >= 1 needs (y + 1)^2 <= x^2 + y^2 - 2
+            if (yy + (y + 1) * (y + 1) > 2 * n_max + xx) break;   // c <= n_max needs z = y + 1 to fit
+            const u64 z1 = d_isqrt64(xx + yy - 2), z2 = d_isqrt64(2 * n_max + xx - yy);
+            const u64 zmax = z1 < z2 ? z1 : z2;
+            const u64 zl = y + 1 + ((y + 1 + x + y) & 1u);   // z = x + y (mod 2)
+            if (zmax >= zl) acc += (zmax - zl) / 2 + 1;
+        }
+    }
+    acc = wave_sum_u64(acc);
+    if (lane_id() == 0 && acc) atomicAdd(total, (unsigned long long)acc);
+}
+
+extern "C" hipError_t ps4_launch_elem(hipStream_t st, const ps4_quad *quads, const ps4_ctl *ctl, u64 quad_cap,
+                                      u64 n_max, ps4_quad *out, u32 cap, u32 *count, u32 grid)
+{
+    hipLaunchKernelGGL(k_elem, dim3(grid), dim3(256), 0, st, quads, ctl, quad_cap, n_max, out, cap, count);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ps4_launch_sqtri(hipStream_t st, u64 n_max, unsigned long long *total)
+{
+    // x < y and c <= n_max give x^2 + 6x + 5 <= 2 n_max: x <= isqrt(2 n_max)
+    u64 x_max = (u64)sqrt((double)(2 * n_max));
+    while (x_max * x_max > 2 * n_max) --x_max;
+    while ((x_max + 1) * (x_max + 1) <= 2 * n_max) ++x_max;
+    const u32 grid = (u32)((x_max + 255) / 256);
+    hipLaunchKernelGGL(k_sqtri, dim3(grid), dim3(256), 0, st, n_max, x_max, total);
+    return hipGetLastError();
+}
+
 __global__ void k_fill_u32(u32 *p, u32 n, u32 v)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;

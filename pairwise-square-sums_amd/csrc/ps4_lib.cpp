@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -35,6 +36,9 @@ extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u
 extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u64 d_max, const u64 *ftab,
                                         u64 tab_n, u64 *d_out, u32 cap, u32 *count, u32 *err);
 extern "C" hipError_t ps4_launch_fill_u32(hipStream_t st, u32 *p, u32 n, u32 v);
+extern "C" hipError_t ps4_launch_elem(hipStream_t st, const ps4_quad *quads, const ps4_ctl *ctl, u64 quad_cap,
+                                      u64 n_max, ps4_quad *out, u32 cap, u32 *count, u32 grid);
+extern "C" hipError_t ps4_launch_sqtri(hipStream_t st, u64 n_max, unsigned long long *total);
 extern "C" int ps4_rq_threads(void);
 extern "C" int ps4_ext_threads(void);
 extern "C" int ps4_ext_queue_cap(void);
@@ -217,6 +221,13 @@ struct ps4_ctx {
     u32 *d_tcnt = nullptr, *d_terr = nullptr;
     u32 tq_cap = 0, tq_out = 0;
     bool debug_sync = false;
+    // element-range mode: largest member allowed (0: off), the device buffer of
+    // the 4-sets that qualify in a batch, and those collected so far
+    u64 elem_max = 0;
+    ps4_quad *d_eout = nullptr;
+    u32 *d_ecnt = nullptr;
+    unsigned long long *d_etri = nullptr;
+    std::vector<ps4_quad> elem_sets;
 };
 
 #define HIP_OK(call)                                                                     \
@@ -528,7 +539,9 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         // ---- assemble one batch of superwindows
         std::vector<Interval> save(iv.begin() + next_iv, iv.end());
         double quads_expected = 0.0;
-        const double quads_budget = 0.8 * (double)ctx->quad_cap * budget_scale;
+        // element mode: a batch's 4-sets must also fit the buffer of qualifying ones
+        const double quads_budget =
+            0.8 * (double)(ctx->elem_max ? std::min<u64>(ctx->quad_cap, PS4_ELEM_CAP) : ctx->quad_cap) * budget_scale;
         u32 n_items = 0;
         u32 rid_lo = iv[next_iv].rid, rid_hi = rid_lo;
         // little work left (small queries): shorter superwindows, so that the
@@ -578,6 +591,11 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
                              (ctx->flags & PS4_VERIFY) ? 1u : 0u));
         HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
         if (int rc = dbg("k_rq")) return rc;
+        if (ctx->elem_max) {
+            HIP_OK(ps4_launch_fill_u32(ctx->st, ctx->d_ecnt, 1, 0));
+            HIP_OK(ps4_launch_elem(ctx->st, ctx->d_quads, ctx->d_ctl, ctx->quad_cap, ctx->elem_max, ctx->d_eout,
+                                   PS4_ELEM_CAP, ctx->d_ecnt, ctx->n_cu * 8));
+        }
         if (!noext)
             HIP_OK(ps4_launch_ext(ctx->st, ctx->ext_grid, ctx->d_quads, ctx->quad_cap, ctx->d_ctl, ctx->d_ftab,
                                   ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, rid_hi + 1, ctx->d_hits,
@@ -634,6 +652,17 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
             o.max_reps = std::max(o.max_reps, c[CT_MAX_REPS]);
             o.max_divisors = std::max(o.max_divisors, c[CT_MAX_DIVS]);
         }
+        // ---- element mode: the batch's 4-sets with largest member <= elem_max
+        if (ctx->elem_max) {
+            u32 ne = 0;
+            HIP_OK(hipMemcpy(&ne, ctx->d_ecnt, sizeof ne, hipMemcpyDeviceToHost));
+            if (ne > PS4_ELEM_CAP) { ctx->last_error = "element-mode buffer overflow"; return fail(PS4_E_INTERNAL); }
+            const size_t old = ctx->elem_sets.size();
+            ctx->elem_sets.resize(old + ne);
+            if (ne)
+                HIP_OK(hipMemcpy(ctx->elem_sets.data() + old, ctx->d_eout, (size_t)ne * sizeof(ps4_quad),
+                                 hipMemcpyDeviceToHost));
+        }
         // ---- this batch's 4-set dump and extensions, to be turned into lines
         // while the next batch runs on the device
         Pending cur;
@@ -687,7 +716,7 @@ extern "C" void ps4_close(ps4_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     void *dev[] = {ctx->d_ftab, ctx->d_ndiv, ctx->d_qr, ctx->d_ctl, ctx->d_ctr, ctx->d_items, ctx->d_scratch,
-                   ctx->d_quads, ctx->d_hits, ctx->d_exq, ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
+                   ctx->d_quads, ctx->d_hits, ctx->d_exq, ctx->d_eout, ctx->d_ecnt, ctx->d_etri, ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
     for (void *p : dev) if (p) (void)hipFree(p);
     void *host[] = {ctx->h_ctl, ctx->h_ctr, ctx->h_items, ctx->h_hits};
     for (void *p : host) if (p) (void)hipHostFree(p);
@@ -871,4 +900,76 @@ extern "C" int ps4_extend_triple(ps4_ctx *ctx, const uint64_t abc[3], uint64_t d
                                  size_t *n_out)
 {
     return ps4_extend_triple_batch(ctx, 1, abc, d_max, d_out, cap, n_out);
+}
+
+
+extern "C" int ps4_search_elements(ps4_ctx *ctx, uint64_t n_max, ps4_sink sink, void *user, uint64_t *n_sets,
+                                   uint64_t *n_triples)
+{
+    if (!ctx) return PS4_E_ARG;
+    if (!n_sets || !n_triples) { ctx->last_error = "null output"; return PS4_E_ARG; }
+    *n_sets = 0;
+    *n_triples = 0;
+    if (n_max < 1) return 0;
+    // a + b + c + d <= (n - 3) + (n - 2) + (n - 1) + n: the census of S < 4n - 5 holds them all
+    // (no 4-set below n = 4)
+    const u64 lo = 1, hi = n_max >= 4 ? 4 * n_max - 5 : 1;
+    if (n_max > PS4_ELEM_NMAX || hi > ctx->S_hi_max) {
+        ctx->last_error = "n_max beyond 10^10 or beyond the tables (S_hi_max >= 4 n_max - 6 is needed)";
+        return PS4_E_ARG;
+    }
+    HIP_OK(hipSetDevice(ctx->device));
+    if (!ctx->d_eout) {
+        HIP_OK(hipMalloc(&ctx->d_eout, (size_t)PS4_ELEM_CAP * sizeof(ps4_quad)));
+        HIP_OK(hipMalloc(&ctx->d_ecnt, sizeof(u32)));
+        HIP_OK(hipMalloc(&ctx->d_etri, sizeof(unsigned long long)));
+    }
+    // the square triples
+    HIP_OK(ps4_launch_fill_u32(ctx->st, (u32 *)ctx->d_etri, 2, 0));
+    HIP_OK(ps4_launch_sqtri(ctx->st, n_max, ctx->d_etri));
+    unsigned long long ntri = 0;
+    HIP_OK(hipMemcpyAsync(&ntri, ctx->d_etri, sizeof ntri, hipMemcpyDeviceToHost, ctx->st));
+    HIP_OK(hipStreamSynchronize(ctx->st));
+    // the 4-sets: a census without extensions and without lines, filtered on the device
+    const uint32_t flags = ctx->flags;
+    const int64_t dump4 = ctx->dump4_limit;
+    ctx->flags = PS4_NOEXT;
+    ctx->dump4_limit = 0;
+    ctx->elem_max = n_max;
+    ctx->elem_sets.clear();
+    ps4_counters c;
+    const int rc = hi > lo ? run_job(ctx, 1, &lo, &hi, nullptr, nullptr, nullptr, &c) : 0;
+    ctx->flags = flags;
+    ctx->dump4_limit = dump4;
+    ctx->elem_max = 0;
+    if (rc) { ctx->elem_sets.clear(); return rc; }
+    // members ascending, the list in ascending order of (a, b, c, d)
+    std::vector<std::array<u64, 4>> sets;
+    sets.reserve(ctx->elem_sets.size());
+    for (const ps4_quad &x : ctx->elem_sets) {
+        const u32 r6[6] = {x.r_ab, x.r_ac, x.r_ad, x.r_bc, x.r_bd, x.r_cd};
+        const Quad6 q = members(r6);
+        sets.push_back({(u64)q.d, (u64)q.c, (u64)q.b, (u64)q.a});
+    }
+    std::vector<ps4_quad>().swap(ctx->elem_sets);
+    std::sort(sets.begin(), sets.end());
+    *n_sets = sets.size();
+    *n_triples = ntri;
+    if (sink) {
+        std::string t;
+        for (size_t i = 0; i < sets.size(); ++i) {
+            t += '[';
+            for (int k = 0; k < 4; ++k) {
+                char num[48];
+                if (k) t += ',';
+                t.append(num, (size_t)(put_dec(num, sets[i][k]) - num));
+            }
+            t += "]\n";
+            if (t.size() >= (1u << 20) || i + 1 == sets.size()) {
+                if (sink(user, t.data(), t.size())) { ctx->last_error = "sink returned nonzero"; return PS4_E_SINK; }
+                t.clear();
+            }
+        }
+    }
+    return 0;
 }

@@ -78,6 +78,7 @@ def load():
     lib.ps4_extend_triple.argtypes = [P, u64p, u64, u64p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     lib.ps4_extend_triple_batch.argtypes = [P, ctypes.c_size_t, u64p, u64, u64p, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)]
+    lib.ps4_search_elements.argtypes = [P, u64, SINK, P, u64p, u64p]
     lib.ps4_set_alert.argtypes = [P, ALERT, P]
     lib.ps4_set_dump4.argtypes = [P, ctypes.c_int64]
     lib.ps4_close.argtypes = [P]
@@ -151,6 +152,22 @@ class Engine:
         cs = (Counters * n)()
         self._check(self.lib.ps4_search_batch(self.ctx, n, lo, hi, cb, users, cs))
         return [(cs[i].as_dict(), b"".join(texts[i]).decode().splitlines()) for i in range(n)]
+
+    def search_elements(self, n_max):
+        """Element-range search: (sorted "[a,b,c,d]" lines of every 4-set with
+        largest element <= n_max, number of square triples with c <= n_max)."""
+        chunks = []
+
+        def sink(_user, text, n):
+            chunks.append(ctypes.string_at(text, n))
+            return 0
+        cb = SINK(sink)
+        ns, nt = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.lib.ps4_search_elements(self.ctx, int(n_max), cb, None, ctypes.byref(ns), ctypes.byref(nt)))
+        lines = b"".join(chunks).decode().splitlines()
+        if len(lines) != ns.value:
+            raise RuntimeError("element search: line count differs from the set count")
+        return lines, int(nt.value)
 
     def extend_triple(self, abc, d_max, cap=1 << 16):
         arr = (ctypes.c_uint64 * 3)(*[int(x) for x in abc])

@@ -5,6 +5,7 @@
 //   ps4_emu [--noext] [--dump4 N] [--smax S] [--batch] lo hi [lo hi ...]
 //   (--batch: all ranges in one ps4_search_batch call, so tiles mix intervals)
 //   ps4_emu --triple a b c dmax
+//   ps4_emu --elements n_max
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -42,26 +43,40 @@ int main(int argc, char **argv)
     long long dump4 = 0;
     unsigned long long smax = 0;
     std::vector<unsigned long long> nums;
-    bool triple = false, batch = false;
+    bool triple = false, batch = false, elements = false;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--noext")) flags |= PS4_NOEXT;
         else if (!strcmp(argv[i], "--dump4")) dump4 = atoll(argv[++i]);
         else if (!strcmp(argv[i], "--smax")) smax = strtoull(argv[++i], 0, 10);
         else if (!strcmp(argv[i], "--triple")) triple = true;
         else if (!strcmp(argv[i], "--batch")) batch = true;
+        else if (!strcmp(argv[i], "--elements")) elements = true;
         else nums.push_back(strtoull(argv[i], 0, 10));
     }
     ps4_config cfg;
     memset(&cfg, 0, sizeof cfg);
     unsigned long long top = 0;
     for (auto x : nums) top = std::max(top, x);
-    cfg.S_hi_max = smax ? smax : std::max(top + 1, 100ull);
+    cfg.S_hi_max = smax ? smax : std::max((elements ? 4 : 1) * top + 1, 100ull);
     cfg.flags = flags;
     cfg.dump4_limit = dump4;
     ps4_ctx *ctx = nullptr;
     int rc = ps4_open(&cfg, &ctx);
     if (rc) { printf("open failed %d\n", rc); return 2; }
     int bad = 0;
+    if (elements) {
+        std::string got, want;
+        uint64_t ns = 0, nt = 0;
+        rc = ps4_search_elements(ctx, nums[0], sink, &got, &ns, &nt);
+        const int64_t m = ora_elements(nums[0], ora_line, &want);
+        const uint64_t mt = ora_square_triples(nums[0]);
+        const bool ok = rc == 0 && got == want && (int64_t)ns == m && nt == mt;
+        printf("elements %llu %s sets=%llu want=%lld triples=%llu want=%llu rc=%d\n", (unsigned long long)nums[0],
+               ok ? "OK" : "MISMATCH", (unsigned long long)ns, (long long)m, (unsigned long long)nt,
+               (unsigned long long)mt, rc);
+        ps4_close(ctx);
+        return ok ? 0 : 1;
+    }
     if (triple) {
         uint64_t abc[3] = {nums[0], nums[1], nums[2]};
         uint64_t out[4096], want[4096];

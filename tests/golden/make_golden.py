@@ -14,6 +14,11 @@ and bruteforce-scratchpad/dotnet/1mln.txt.
 
 Usage:  python3 tests/golden/make_golden.py [--big]
         --big adds [1,1e9) and [1,1e10) (about 4 minutes on 8 cores).
+        python3 tests/golden/make_golden.py --elements N
+        runs the reference once more on [1, 4N-5) with every 4-set printed and
+        records the 4-sets with largest element <= N (the element-range form
+        of the search), their count and the SHA-256 of their "[a,b,c,d]"
+        lines, members and list ascending;
         python3 tests/golden/make_golden.py --add LO HI
         runs one more range (8 threads) and adds it to the existing fixture;
         [1, 6250000000) is there this way (about 2.5 minutes on 8 cores): it is
@@ -99,7 +104,32 @@ def add_run(lo, hi):
     print(lo, hi, rec["counters"]["4-sets"], rec["n_lines"], "added to", OUT)
 
 
+def element_record(n_max):
+    """The 4-sets with largest element <= n_max, from the reference's dump of
+    every 4-set with S < 4 n_max - 5 (a + b + c + d <= 4 n_max - 6)."""
+    _, lines = run_s4(1, 4 * n_max - 5, 1, ["--noext", "--dump4", "9223372036854775807"])
+    sets = []
+    for ln in lines:
+        vals = sorted(int(x) for x in re.match(rb"4: \[([\d,]+)\]", ln).group(1).split(b","))
+        if vals[3] <= n_max:
+            sets.append(vals)
+    sets.sort()
+    text = "".join("[%d,%d,%d,%d]\n" % tuple(v) for v in sets)
+    return {"n_max": n_max, "n_sets": len(sets), "sha256": hashlib.sha256(text.encode()).hexdigest()}
+
+
+def add_elements(n_max):
+    data = json.load(open(OUT))
+    rec = element_record(n_max)
+    data["elements"] = [r for r in data.get("elements", []) if r["n_max"] != n_max] + [rec]
+    with open(OUT, "w") as f:
+        json.dump(data, f, indent=1)
+    print("elements", rec, "added to", OUT)
+
+
 def main():
+    if "--elements" in sys.argv:
+        return add_elements(int(sys.argv[sys.argv.index("--elements") + 1]))
     if "--add" in sys.argv:
         i = sys.argv.index("--add")
         return add_run(int(sys.argv[i + 1]), int(sys.argv[i + 2]))

@@ -8,6 +8,8 @@ drop-in CLI built on it, and compares with
 Bit-exact throughout: counters, 5-set histogram, capacity maxima, and the
 byte-sorted result lines.  Tests run from small to large ranges.
 """
+import hashlib
+import math
 import os
 import random
 import re
@@ -397,3 +399,70 @@ def test_serving_results_same_alone_and_in_the_mix(eng):
             assert alone[0] == r.result[0] and sorted(alone[1]) == sorted(r.result[1]), r.args
         else:
             assert eng.extend_triple(*r.args) == r.result, r.args
+
+
+def _closed_form_square_triples(n_max):
+    """Square triples {a<b<c}, c <= n_max, counted per root pair (x, y) in closed form."""
+    n = 0
+    x = 1
+    while x * x <= 2 * n_max:
+        y = x + 1
+        while 2 * y + 3 <= x * x and y * y + (y + 1) ** 2 <= 2 * n_max + x * x:
+            zmax = min(math.isqrt(x * x + y * y - 2), math.isqrt(2 * n_max + x * x - y * y))
+            zl = y + 1 + ((y + 1 + x + y) & 1)
+            if zmax >= zl:
+                n += (zmax - zl) // 2 + 1
+            y += 1
+        x += 1
+    return n
+
+
+def test_elements_small_vs_oracle():
+    """Element-range search for small N (edge cases included) against the
+    brute-force oracle: the 4-set lines and the square-triple count."""
+    e = ps4.Engine(S_hi_max=4 * 10**5)
+    try:
+        for n_max in (1, 3, 4, 7, 100, 1000, 25000, 10**5):
+            say(f"elements n_max={n_max}")
+            lines, ntri = e.search_elements(n_max)
+            assert lines == ora.elements(n_max), n_max
+            assert ntri == ora.square_triples(n_max), n_max
+    finally:
+        e.close()
+
+
+def test_elements_reference_fixtures(golden):
+    """N = 25000, 1e6, 1e7: the 4-sets with largest element <= N from the
+    reference's --dump4 output (count and SHA-256 of the sorted lines); the
+    square-triple count against the closed form (N <= 1e6)."""
+    e = ps4.Engine(S_hi_max=4 * 10**7)
+    try:
+        for rec in golden["elements"]:
+            say(f"elements n_max={rec['n_max']}")
+            lines, ntri = e.search_elements(rec["n_max"])
+            assert len(lines) == rec["n_sets"]
+            assert hashlib.sha256("".join(x + "\n" for x in lines).encode()).hexdigest() == rec["sha256"]
+            if rec["n_max"] <= 10**6:
+                assert ntri == _closed_form_square_triples(rec["n_max"])
+    finally:
+        e.close()
+
+
+def test_elements_limits():
+    """n_max beyond the tables is an argument error, not a truncated answer."""
+    e = ps4.Engine(S_hi_max=10**6)
+    try:
+        with pytest.raises(RuntimeError):
+            e.search_elements(10**6)
+    finally:
+        e.close()
+
+
+def test_cli_elements(tmp_path):
+    say("cli --elements 25000")
+    exe = os.path.join(ps4.PKG_DIR, "lib", "ps4")
+    r = subprocess.run([exe, "--elements", "25000"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.decode().splitlines()
+    assert lines == ora.elements(25000)
+    assert f"ELEMENTS: n_max=25000 sets={len(lines)} square_triples={ora.square_triples(25000)}" in r.stderr.decode()

@@ -253,3 +253,65 @@ def test_planted_mistake_is_caught(golden):
     bad["counters"] = dict(res["counters"], ext_sqrt=res["counters"]["ext_sqrt"] + 1)
     with pytest.raises(AssertionError):
         _check_run(rec, bad, lines)
+
+
+# ---- element-range mode (the search by largest element) ----
+
+def _closed_form_square_triples(n_max):
+    """Square triples {a<b<c}, c <= n_max, counted per root pair (x, y) with the
+    admissible z (y < z, x + y + z even, a >= 1, c <= n_max) in closed form."""
+    n = 0
+    x = 1
+    while x * x <= 2 * n_max:
+        y = x + 1
+        while 2 * y + 3 <= x * x and y * y + (y + 1) ** 2 <= 2 * n_max + x * x:
+            zmax = min(math.isqrt(x * x + y * y - 2), math.isqrt(2 * n_max + x * x - y * y))
+            zl = y + 1 + ((y + 1 + x + y) & 1)
+            if zmax >= zl:
+                n += (zmax - zl) // 2 + 1
+            y += 1
+        x += 1
+    return n
+
+
+def test_square_triples_brute_force():
+    """Every {a<b<c} with c <= 300 tried directly against the oracle's count."""
+    sq = {i * i for i in range(60)}
+    for n_max in (1, 10, 60, 150, 300):
+        want = sum(1 for c in range(1, n_max + 1) for b in range(1, c) if b + c in sq
+                   for a in range(1, b) if a + b in sq and a + c in sq)
+        assert ora.square_triples(n_max) == want, n_max
+
+
+def test_square_triples_closed_form():
+    for n_max in (1000, 25000, 10**5):
+        assert ora.square_triples(n_max) == _closed_form_square_triples(n_max), n_max
+
+
+def test_elements_match_reference_dump(golden):
+    """The oracle's element-range search equals the 4-sets with largest element
+    <= N that the reference prints with --dump4 (tests/golden/make_golden.py
+    --elements), and the [1,1e5) dump filtered by hand."""
+    for rec in golden["elements"]:
+        if rec["n_max"] > 10**5:
+            continue   # larger ones are checked on the GPU
+        lines = ora.elements(rec["n_max"])
+        assert len(lines) == rec["n_sets"]
+        assert hashlib.sha256("".join(x + "\n" for x in lines).encode()).hexdigest() == rec["sha256"]
+    dump = next(r for r in golden["dump4"] if (r["lo"], r["hi"]) == (1, 10**5))
+    n_max = 25000   # every 4-set with d <= 25000 has S <= 99994 < 1e5
+    sets = sorted(sorted(int(v) for v in ln.split("[")[1].split("]")[0].split(",")) for ln in dump["lines"])
+    want = ["[%d,%d,%d,%d]" % tuple(v) for v in sets if v[3] <= n_max]
+    assert ora.elements(n_max) == want
+
+
+def test_elements_are_square_sets():
+    """Every set the oracle returns has six square pair sums and d <= N, and
+    contains square triples counted by the triple count."""
+    for ln in ora.elements(10**5):
+        v = [int(x) for x in ln[1:-1].split(",")]
+        assert v == sorted(v) and len(set(v)) == 4 and v[3] <= 10**5
+        for i in range(4):
+            for j in range(i + 1, 4):
+                r = math.isqrt(v[i] + v[j])
+                assert r * r == v[i] + v[j]
