@@ -8,7 +8,7 @@ is the input; nothing is cached between steps.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
                     [--lo 1] [--hi 10000000000] [--no-cpu] [--no-latency]
-    python bench.py --workload batched|band|serving
+    python bench.py --workload batched|band|serving|elements
 
 The default workload is the plain census above.  The others print their own
 JSON line:
@@ -19,6 +19,9 @@ JSON line:
            band counters against the reference's own logs
            (exhaustive/logs/r12_9.err + boundary_9.err), and a seeded sample of
            subranges bit-exact against the reference s4 run on this host.
+  elements every 4-set with largest element <= 1e7 and the square-triple
+           count (the search by largest element), against the reference's
+           --dump4 output; the reference s4 on one thread beside it.
   serving  an open-loop Poisson stream (200 req/s, seed 7; 70% range queries
            lo in [1,1e12], length log-uniform in [1e3,1e8]; 30% triple queries
            c <= 1e9, d <= 1e12) through the continuous-batching request loop;
@@ -323,6 +326,47 @@ def bench_serving(args):
     }
 
 
+def bench_elements(args):
+    """Element-range search (every 4-set with largest element <= N, and the
+    square-triple count) at N = 1e7, against the reference fixture; the CPU leg
+    is the reference s4 printing every 4-set of S < 4N - 5 on one thread
+    (filtered by largest element afterwards)."""
+    import ps4
+    n_max = 10**7
+    eng = ps4.Engine(S_hi_max=4 * n_max)
+    eng.search_elements(10**5)   # warm the device
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        lines, ntri = eng.search_elements(n_max)
+        times.append(time.perf_counter() - t0)
+    eng.close()
+    t = min(times)
+    out = {"metric": "element-range search: 4-sets with largest element <= 1e7 and square triples",
+           "value": t * 1e3, "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "u64",
+           "data": "synthetic", "config": {"workload": "elements, N = 1e7", "n_max": n_max},
+           "sets": len(lines), "square_triples": ntri, "times_s": times}
+    if os.path.exists(GOLDEN):
+        gold = json.load(open(GOLDEN))
+        rec = next((r for r in gold.get("elements", []) if r["n_max"] == n_max), None)
+        if rec:
+            sha = hashlib.sha256("".join(x + "\n" for x in lines).encode()).hexdigest()
+            out["bit_exact_vs_s4"] = len(lines) == rec["n_sets"] and sha == rec["sha256"]
+    s4 = os.path.join(REPO, "oracle", "_ref", "s4")
+    if not args.no_cpu and os.path.exists(s4):
+        with tempfile.TemporaryDirectory() as tmp:
+            t0 = time.perf_counter()
+            r = subprocess.run([s4, "1", str(4 * n_max - 5), "1", "20", "--noext", "--dump4", str(2**62), "--out",
+                                os.path.join(tmp, "o")], cwd=tmp, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL, timeout=600)
+            dt = time.perf_counter() - t0
+        if r.returncode == 0:
+            out["cpu_baseline"] = {"value": dt * 1e3, "unit": "ms", "cores": 1, "kind": "reference",
+                                   "sample": "the whole workload: s4 --noext --dump4 on [1, 4e7 - 5), one thread "
+                                             "(square-triple count and the filter not included)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,7 +376,7 @@ def main():
     ap.add_argument("--hi", type=int, default=10**10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "batched", "band", "serving"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "batched", "band", "serving", "elements"], default="plain")
     ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once (a plain copy of known size that "
@@ -343,7 +387,8 @@ def main():
               f"across GPUs (disjoint S intervals, one engine per device), but this bench measures one device.", file=sys.stderr)
         sys.exit(2)
     if args.workload != "plain":
-        fn = {"batched": bench_batched, "band": bench_band, "serving": bench_serving}[args.workload]
+        fn = {"batched": bench_batched, "band": bench_band, "serving": bench_serving,
+              "elements": bench_elements}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     import ps4
