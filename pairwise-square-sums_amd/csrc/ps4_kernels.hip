@@ -1193,18 +1193,25 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                         u32 p, e;
                         if (e2p && s == 0) { p = 2; e = e2p; }
                         else { const u32 w = s_fac[(s - (e2p ? 1u : 0u)) * EX_NT + tid]; p = w >> 5; e = w & 31u; }
+                        // entry (ee, j) = lst[j] * p^ee at ee * cur + j: each entry of the
+                        // list so far is read once and its e multiples are written, so
+                        // the only read-after-write dependence is between stages
                         const u32 cur = n;
-                        double pk = 1.0;
-                        for (u32 ee = 0; ee < e; ++ee) {
-                            pk *= (double)p;
-                            u32 j = 0;
-                            for (; j + 4 <= cur; j += 4) {   // four loads in flight
-                                const double x0 = lst[j], x1 = lst[j + 1], x2 = lst[j + 2], x3 = lst[j + 3];
-                                lst[n] = x0 * pk; lst[n + 1] = x1 * pk; lst[n + 2] = x2 * pk; lst[n + 3] = x3 * pk;
-                                n += 4;
+                        const double dp = (double)p;
+                        u32 j = 0;
+                        for (; j + 4 <= cur; j += 4) {   // four loads in flight
+                            double y0 = lst[j], y1 = lst[j + 1], y2 = lst[j + 2], y3 = lst[j + 3];
+                            for (u32 ee = 1; ee <= e; ++ee) {
+                                y0 *= dp; y1 *= dp; y2 *= dp; y3 *= dp;
+                                double *o = lst + ee * cur + j;
+                                o[0] = y0; o[1] = y1; o[2] = y2; o[3] = y3;
                             }
-                            for (; j < cur; ++j) lst[n++] = lst[j] * pk;
                         }
+                        for (; j < cur; ++j) {
+                            double y = lst[j];
+                            for (u32 ee = 1; ee <= e; ++ee) { y *= dp; lst[ee * cur + j] = y; }
+                        }
+                        n = cur * (e + 1);
                     }
                     if (sL == nfa) s_ar[loff + nL] = h0;   // empty H part: the list {1}
                 }
