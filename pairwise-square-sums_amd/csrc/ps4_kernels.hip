@@ -666,21 +666,40 @@ __device__ __forceinline__ double dmod(double x, double m, double inv_m)
     return r;
 }
 
-// Residues of a value mod 4095 = 63*65 given a representative below 2^20 of
-// its class mod 262080 = 64*4095.
-__device__ __forceinline__ u32 fold4095(u32 r)
+// a * b + c as one full-rate 24-bit multiply-add (a, b taken mod 2^24, as
+// unsigned or as signed numbers): left to itself the compiler picks a 32-bit or
+// a 64-bit multiply for these
+#ifndef PS4_MAD_U24
+#define PS4_MAD_U24(a, b, c)                                                              \
+    ({                                                                                    \
+        u32 r_;                                                                           \
+        asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r_) : "v"((u32)(a)), "v"((u32)(b)), "v"((u32)(c))); \
+        r_;                                                                               \
+    })
+#define PS4_MAD_I24(a, b, c)                                                              \
+    ({                                                                                    \
+        u32 r_;                                                                           \
+        asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r_) : "v"((u32)(a)), "s"((int)(b)), "v"((u32)(c))); \
+        r_;                                                                               \
+    })
+#endif
+
+// (a mod 2^24) * (b mod 2^24) >> 32: the 24-bit multiply-high
+__device__ __forceinline__ u32 mulhi_u24(u32 a, u32 b)
 {
-    u32 s = (r >> 12) + (r & 4095u);
-    return s >= 4095u ? s - 4095u : s;
+    return (u32)(((u64)(a & 0xFFFFFFu) * (u64)(b & 0xFFFFFFu)) >> 32);
 }
-// s mod 63, s mod 65 for s < 4095 and s mod 11..19 for s < 46189 by a 32-bit
-// multiply and a logical shift (the products stay below 2^32; the constants are
-// checked exhaustively on the host).  Plain multiplies, not __umul24: with
-// __umul24 the compiler emitted an arithmetic shift of the product, which is
-// wrong once the product reaches 2^31 (the residue then came out near 90000 and
-// indexed far past the residue tables).
-__device__ __forceinline__ u32 mod63(u32 s) { return s - 63u * ((s * 16645u) >> 20); }
-__device__ __forceinline__ u32 mod65(u32 s) { return s - 65u * ((s * 16132u) >> 20); }
+
+// s mod 63 and s mod 65 for s < 2^20 (a representative of a class mod
+// 262080 = 64*63*65): floor(s / m) = (8 s * ceil(2^29 / m)) >> 32 by the 24-bit
+// multiply-high, exact because the rounding error s * 2^-29 stays below 1/m.
+// s mod 11..19 for s < 46189 by a 32-bit multiply and a logical shift (the
+// products stay below 2^32).  All constants are checked exhaustively on the
+// host.  Plain multiplies, not __umul24: with __umul24 the compiler emitted an
+// arithmetic shift of the product, which is wrong once the product reaches 2^31
+// (the residue then came out near 90000 and indexed far past the residue tables).
+__device__ __forceinline__ u32 mod63(u32 s) { return PS4_MAD_I24(mulhi_u24(s << 3, 8521761u), -63, s); }
+__device__ __forceinline__ u32 mod65(u32 s) { return PS4_MAD_I24(mulhi_u24(s << 3, 8259553u), -65, s); }
 __device__ __forceinline__ u32 mod11(u32 s) { return s - 11u * ((s * 47663u) >> 19); }
 __device__ __forceinline__ u32 mod13(u32 s) { return s - 13u * ((s * 40330u) >> 19); }
 __device__ __forceinline__ u32 mod17(u32 s) { return s - 17u * ((s * 30841u) >> 19); }
@@ -698,7 +717,7 @@ __device__ __forceinline__ u32 mod_m2(double x)
     u32 r = (u32)(__builtin_fma(-q, M2, x) + M2);   // [0, 3*M2)
     r = r < r - 46189u ? r : r - 46189u;
     r = r < r - 46189u ? r : r - 46189u;
-    return r;
+    return r & 0xFFFFu;   // (no-op: tells the compiler r < 2^16, so 24-bit multiplies suffice)
 }
 
 // A representative in [0, 3*M1) of x mod M1 for an integer-valued double
@@ -795,7 +814,7 @@ __device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e,
 
 // One candidate: 4-set record k of the pass, candidate index t of that 4-set.
 // Straight-line: the record and the four list entries are loaded before the
-// tests (ex_rec, ex_lists), and every test is a select.  ok: the candidate
+// tests (ex_lists), and every test is a select.  ok: the candidate
 // passes the residue filters mod 64, 63, 65.  The queued ok candidates then
 // get the remaining tests of the ext_sqrt count (e > 0, e not a member) and
 // the later filters (ex_drain1).
@@ -804,26 +823,10 @@ struct ExCand {
     bool ok;
 };
 
-struct ExRec {
-    uint4 q0, q1, q2;
-};
-
-__device__ __forceinline__ ExRec ex_rec(const uint4 *s_par, u32 k)
-{
-    const uint4 *pw = s_par + k * EX_PAR_WORDS;
-    return ExRec{pw[0], pw[1], pw[2]};
-}
-
 // The four list entries of candidate t: L[i], H[j], L[nL-1-i], H[nH-1-j].
 struct ExLists {
     double l0, h0, l1, h1;
 };
-
-// (a mod 2^24) * (b mod 2^24) >> 32: the 24-bit multiply-high
-__device__ __forceinline__ u32 mulhi_u24(u32 a, u32 b)
-{
-    return (u32)(((u64)(a & 0xFFFFFFu) * (u64)(b & 0xFFFFFFu)) >> 32);
-}
 
 // The double at byte offset off of the arena.
 __device__ __forceinline__ double ar_at(const double *s_ar, u32 off)
@@ -832,28 +835,26 @@ __device__ __forceinline__ double ar_at(const double *s_ar, u32 off)
 }
 
 // g: the candidate's index in the pass; t = g - coff is its index in its 4-set,
-// i = t mod nL, j = t / nL.
-__device__ __forceinline__ ExLists ex_lists(const ExRec &R, const double *s_ar, u32 g)
+// i = t mod nL, j = t / nL; q0: the first word of the 4-set's record.
+__device__ __forceinline__ ExLists ex_lists(const uint4 &q0, const double *s_ar, u32 g)
 {
-    const uint4 q0 = R.q0;
     const u32 t = g - (q0.w >> 16);
     const u32 j = mulhi_u24(t << 9, q0.z);
-    const u32 i8 = (j * q0.w + (t << 3)) & 0xFFFFu;   // 8 (t - j nL) < 2^16 from the low halves
+    const u32 i8 = PS4_MAD_U24(j, q0.w, t << 3) & 0xFFFFu;   // 8 (t - j nL) < 2^16 from the low halves
     const u32 j8 = j << 3;
     return ExLists{ar_at(s_ar, (q0.x & 0xFFFFu) + i8), ar_at(s_ar, (q0.y & 0xFFFFu) + j8),
                    ar_at(s_ar, (q0.x >> 16) - i8), ar_at(s_ar, (q0.y >> 16) - j8)};
 }
 
-// neg_m1 = -M1, passed in by the launch so that it sits in scalar registers.
-__device__ __forceinline__ ExCand ex_cand(const ExRec &R, const ExLists &Ls, double neg_m1)
+// q1, q2: the mask words of the 4-set's record.  neg_m1 = -M1, passed in by
+// the launch so that it sits in scalar registers.
+__device__ __forceinline__ ExCand ex_cand(const uint4 &q1, const uint4 &q2, const ExLists &Ls, double neg_m1)
 {
-    const uint4 q1 = R.q1, q2 = R.q2;
     ExCand c;
     c.u = __builtin_fma(Ls.l1, Ls.h1, Ls.l0 * Ls.h0);   // H holds halves when u = (f+g)/2
     const double q = __builtin_floor(c.u * (1.0 / M1));
     const u32 r1 = (u32)(__builtin_fma(q, neg_m1, c.u) + M1);   // rep_m1(c.u)
-    const u32 r4095 = fold4095(r1);
-    const u32 r63 = mod63(r4095), r65 = mod65(r4095);
+    const u32 r63 = mod63(r1), r65 = mod65(r1);
     const u32 qrb = bit64(q1.x, q1.y, r1 & 63u) & bit64(q1.z, q1.w, r63) &
                     (r65 < 64u ? bit64(q2.x, q2.y, r65) : (q2.z & 1u));
     c.ok = qrb != 0u;
@@ -904,6 +905,12 @@ __device__ __forceinline__ u64 wave_max_u64(u64 v)
     for (int m = 32; m >= 1; m >>= 1) { const u64 o = shfl_xor_u64(v, m); v = o > v ? o : v; }
     return v;
 }
+// set bits of a wave-wide mask (a ballot) in the lanes below this one
+__device__ __forceinline__ u32 lanes_below(u64 m)
+{
+    return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+}
+
 // exclusive prefix sum over the wave; *total gets the sum (uniform)
 __device__ __forceinline__ u32 wave_excl_scan(u32 v, u32 *total)
 {
@@ -1108,10 +1115,9 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // exact residues mod 46189; |C - A| < 2^51
                     const u32 r1a = rep_m1(d1), r1b = rep_m1(d2);
                     const u32 r2a = mod_m2(d1), r2b = mod_m2(d2);
-                    const u32 fa = fold4095(r1a), fb = fold4095(r1b);
                     // residues index small tables: each index is checked against
                     // its table (a wrong residue is an error, never a load outside it)
-                    u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(fa), mod63(fb), mod65(fa), mod65(fb), mod11(r2a),
+                    u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(r1a), mod63(r1b), mod65(r1a), mod65(r1b), mod11(r2a),
                                   mod11(r2b), mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
                     const u32 lim[7] = {64, 63, 65, 11, 13, 17, 19};
 #pragma unroll
@@ -1218,10 +1224,11 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 __syncthreads();
                 // lane l takes candidates l, l+64, l+128, ... of the pass, two per
                 // iteration (g and g + 64: two independent load/compute chains).
-                // Pipelined: while this iteration's list entries load, the records
-                // of the next iteration and the bitmap words of the one after are
-                // read (LDS reads complete in order, so each wait covers only
-                // older reads).
+                // Pipelined: while this iteration's list entries load, the first
+                // record words of the next iteration and the bitmap words of the
+                // one after are read, and the next iteration's mask words once this
+                // iteration's tests have used theirs (LDS reads complete in order,
+                // so each wait covers only older reads).
                 u32 kp = 0;   // 4-set starts before the next fetched base
                 const u64 lane_mask = (2ull << tid) - 1ull;   // bits 0..tid
                 u32 qn1 = 0;                                  // entries in the first queue
@@ -1229,33 +1236,43 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 // bitmap words of base (the 4-set of candidate g starts at or before
                 // g, minus one); past the end of the pass a lane evaluates record 0
                 // at its first candidate (in range) and discards it
-                u32 ka, kb;
-                ExRec RA, RB;
+                u32 ka, kb, ia, ib;       // ia, ib: the records read (ka, kb or 0)
+                uint4 A0, A1, A2, B0, B1, B2;   // their words
                 auto fetch = [&](u32 base, const uint4 &wv) {
                     const u64 wa = (u64)wv.x | ((u64)wv.y << 32), wb = (u64)wv.z | ((u64)wv.w << 32);
                     ka = kp + (u32)__popcll(wa & lane_mask) - 1u;
                     kp += (u32)__popcll(wa);
                     kb = kp + (u32)__popcll(wb & lane_mask) - 1u;
                     kp += (u32)__popcll(wb);
-                    RA = ex_rec(s_par, base + tid < ctot ? ka : 0u);
-                    RB = ex_rec(s_par, base + 64u + tid < ctot ? kb : 0u);
+                    ia = (base + tid < ctot ? ka : 0u) & 63u;   // (a record index: the mask only
+                    ib = (base + 64u + tid < ctot ? kb : 0u) & 63u;   //  tells the compiler its range)
+                    A0 = s_par[ia * EX_PAR_WORDS];
+                    B0 = s_par[ib * EX_PAR_WORDS];
+                };
+                auto fetch_masks = [&]() {
+                    A1 = s_par[ia * EX_PAR_WORDS + 1];
+                    A2 = s_par[ia * EX_PAR_WORDS + 2];
+                    B1 = s_par[ib * EX_PAR_WORDS + 1];
+                    B2 = s_par[ib * EX_PAR_WORDS + 2];
                 };
                 // (read unconditionally: beyond the pass the bitmap holds stale bits,
                 // which only choose records of candidates that are discarded)
                 fetch(0u, s_bits4[0]);
+                fetch_masks();
                 uint4 wnext = s_bits4[1];
                 for (u32 base = 0; base < ctot; base += 128) {
                     const u32 ga = base + tid, gb = ga + 64u;
                     const bool va = ga < ctot, vb = gb < ctot;
                     const u32 ka0 = ka, kb0 = kb;
-                    const ExRec RA0 = RA, RB0 = RB;
-                    const ExLists LA = ex_lists(RA0, s_ar, va ? ga : 0u), LB = ex_lists(RB0, s_ar, vb ? gb : 0u);
+                    const ExLists LA = ex_lists(A0, s_ar, va ? ga : 0u), LB = ex_lists(B0, s_ar, vb ? gb : 0u);
+                    __builtin_amdgcn_sched_barrier(0);   // both candidates' list reads issued before any use
                     fetch(base + 128u, wnext);
                     wnext = s_bits4[min((base + 256u) >> 7, EX_CMAX / 128u - 1u)];
                     PS4_KEEP("v"(LA.l0), "v"(LA.h0), "v"(LA.l1), "v"(LA.h1), "v"(LB.l0), "v"(LB.h0), "v"(LB.l1),
                              "v"(LB.h1));   // both candidates' list entries (the next reads may still be in flight)
-                    const ExCand A = ex_cand(RA0, LA, a.neg_m1);
-                    const ExCand B = ex_cand(RB0, LB, a.neg_m1);
+                    const ExCand A = ex_cand(A1, A2, LA, a.neg_m1);
+                    const ExCand B = ex_cand(B1, B2, LB, a.neg_m1);
+                    fetch_masks();
                     const bool oka = va & A.ok, okb = vb & B.ok;
                     if (uniform) {
                         my_esq += (oka ? 1u : 0u) + (okb ? 1u : 0u);
@@ -1276,7 +1293,6 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // and its LDS operations complete in order, so a wave barrier (no
                     // instruction: it keeps the compiler from moving memory operations
                     // across it) orders the stack's writes before a round's reads.
-                    const u64 lane_lt = lane_mask >> 1;   // bits 0..tid-1
                     auto round1 = [&]() {   // the top min(64, qn1) entries
                         __builtin_amdgcn_wave_barrier();
                         const u32 m = qn1 < 64u ? qn1 : 64u, b1 = qn1 - m;
@@ -1302,7 +1318,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                         const u64 bg = __ballot(go);
                         if (go) {
                             const u64 qi = tile * EX_NT + (aux & 63u);
-                            exq2[qn2 + (u32)__popcll(bg & lane_lt)] =
+                            exq2[qn2 + lanes_below(bg)] =
                                 make_uint4((u32)ent, ((u32)(ent >> 32) & 0xFFFFFu) | ((aux >> 6) << 20), (u32)qi,
                                            (u32)(qi >> 32));
                         }
@@ -1312,9 +1328,9 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     };
                     while (qn1 >= 64u) round1();   // then at most 63 + 128 after this iteration
                     const u64 boa = __ballot(oka), bob = __ballot(okb);
-                    if (oka) s_q1[qn1 + (u32)__popcll(boa & lane_lt)] = ex_qent(A.u, ka0);
+                    if (oka) s_q1[qn1 + lanes_below(boa)] = ex_qent(A.u, ka0);
                     qn1 += (u32)__popcll(boa);
-                    if (okb) s_q1[qn1 + (u32)__popcll(bob & lane_lt)] = ex_qent(B.u, kb0);
+                    if (okb) s_q1[qn1 + lanes_below(bob)] = ex_qent(B.u, kb0);
                     qn1 += (u32)__popcll(bob);
                     if (base + 128u >= ctot)   // the pass's records are about to be replaced
                         while (qn1 > 0u) round1();

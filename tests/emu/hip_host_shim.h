@@ -29,6 +29,10 @@
 #define __noinline__ __attribute__((noinline))
 #define __launch_bounds__(...)
 #define PS4_KEEP(...) ((void)0)
+#define PS4_MAD_U24(a, b, c) (((uint32_t)(a) & 0xFFFFFFu) * ((uint32_t)(b) & 0xFFFFFFu) + (uint32_t)(c))
+#define PS4_MAD_I24(a, b, c)                                                                           \
+    ((uint32_t)((int64_t)((int32_t)((uint32_t)(a) << 8) >> 8) * (int64_t)((int32_t)((uint32_t)(b) << 8) >> 8) + \
+                (int64_t)(uint32_t)(c)))
 // LDS: every __shared__ array lives in one section that is filled with garbage
 // at each workgroup launch (real LDS starts with arbitrary contents).
 #define __shared__ static __attribute__((section("emu_lds")))
@@ -159,6 +163,18 @@ static inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int ro
 }
 static inline int __builtin_amdgcn_readlane(int v, int l) { return (int)(uint32_t)emu_wave_exchange((uint32_t)v, (unsigned)l); }
 static inline int __builtin_amdgcn_readfirstlane(int v) { return (int)emu_wave_exchange((uint32_t)v, 0); }
+static inline void __builtin_amdgcn_sched_barrier(int) {}   // an instruction-scheduling fence only
+// set bits of a 64-lane mask below this lane: low word, then high word
+static inline unsigned __builtin_amdgcn_mbcnt_lo(unsigned m, unsigned c)
+{
+    const unsigned lane = threadIdx.x & 63;
+    return c + (unsigned)__builtin_popcount(lane >= 32 ? m : (m & ((1u << lane) - 1u)));
+}
+static inline unsigned __builtin_amdgcn_mbcnt_hi(unsigned m, unsigned c)
+{
+    const unsigned lane = threadIdx.x & 63;
+    return c + (lane < 32 ? 0u : (unsigned)__builtin_popcount(m & ((1u << (lane - 32)) - 1u)));
+}
 static inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 static inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((unsigned long long)a * b) >> 32); }
 static inline unsigned __umul24(unsigned a, unsigned b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }

@@ -49,8 +49,7 @@ def test_struct_layouts_match_header():
     assert ps4.Config.S_hi_max.offset == 8 and ps4.Config.dump4_limit.offset == 24
 
 
-@pytest.mark.parametrize("name,m,shift,lim", [("mod63", 63, 20, 4095), ("mod65", 65, 20, 4095),
-                                              ("mod11", 11, 19, 46189), ("mod13", 13, 19, 46189),
+@pytest.mark.parametrize("name,m,shift,lim", [("mod11", 11, 19, 46189), ("mod13", 13, 19, 46189),
                                               ("mod17", 17, 19, 46189), ("mod19", 19, 20, 46189)])
 def test_kernel_residue_constants_are_exact(name, m, shift, lim):
     """The multiply-shift divisions in the extension kernel equal true division
@@ -65,11 +64,22 @@ def test_kernel_residue_constants_are_exact(name, m, shift, lim):
         assert (s * magic) >> sh == s // m
 
 
-def test_fold4095_is_exact():
-    for r in range(0, 262080):
-        s = (r >> 12) + (r & 4095)
-        s = s - 4095 if s >= 4095 else s
-        assert s == r % 4095
+@pytest.mark.parametrize("name,m", [("mod63", 63), ("mod65", 65)])
+def test_kernel_residues_mod_63_65_are_exact(name, m):
+    """s mod 63 and s mod 65 by the 24-bit multiply-high, for every s < 2^20 (the
+    representatives of a class mod 262080 that the extension kernel forms are
+    below 3 * 262080 < 2^20)."""
+    import numpy as np
+    src = open(KERNELS).read()
+    mt = re.search(name + r"\(u32 s\) \{ return PS4_MAD_I24\(mulhi_u24\(s << 3, (\d+)u\), -(\d+), s\); \}", src)
+    assert mt, name
+    magic, mm = int(mt.group(1)), int(mt.group(2))
+    assert mm == m and magic < 2**24
+    s = np.arange(1 << 20, dtype=np.uint64)
+    assert int(s[-1] << np.uint64(3)) < 2**24                         # the first factor fits 24 bits
+    q = ((s << np.uint64(3)) * np.uint64(magic)) >> np.uint64(32)
+    assert np.array_equal(s - np.uint64(m) * q, s % np.uint64(m))
+    assert 3 * 262080 < (1 << 20)
 
 
 def test_codegen_has_no_signed_shift_of_unsigned_products(tmp_path):
