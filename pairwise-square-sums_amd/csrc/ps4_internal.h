@@ -22,8 +22,9 @@ typedef struct {
     uint32_t rid, pad;
 } ps4_quad;
 
-/* rid of a null entry: each window's block of the queue is padded to a whole
- * number of 64-entry tiles with entries carrying this rid (and zero roots). */
+/* rid of a null entry: the queue is filled in whole 64-entry tiles, each
+ * with the 4-sets of one superwindow; the slots a superwindow leaves unused
+ * carry this rid (and zero roots). */
 #define PS4_RID_NONE 0xFFFFFFFFu
 // element-range mode: qualifying 4-sets one batch can return (the batch's
 // 4-set budget is held below it)

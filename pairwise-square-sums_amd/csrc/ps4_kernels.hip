@@ -90,12 +90,27 @@ __device__ __forceinline__ bool d_is_square128(u128 n)
 #define PS4_KEEP(...) asm volatile("" ::__VA_ARGS__)
 #endif
 
+// Two 16-bit halves of a word at once: the halfwise maximum of a and b, and
+// |a.lo - b.lo| + |a.hi - b.hi| + c.
+#ifndef PS4_MAX_U16X2
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+#define PS4_MAX_U16X2(a, b)                                                                                     \
+    __builtin_bit_cast(u32, __builtin_elementwise_max(__builtin_bit_cast(u16x2, (u32)(a)), __builtin_bit_cast(u16x2, (u32)(b))))
+#define PS4_SAD_U16(a, b, c) __builtin_amdgcn_sad_u16((u32)(a), (u32)(b), (u32)(c))
+#endif
+
 __device__ __forceinline__ u32 lane_id() { return threadIdx.x & 63u; }
 
 // A value every thread of the workgroup holds (read from LDS after a barrier),
 // made provably uniform so that branches and loop exits around barriers stay
 // scalar and the compiler cannot split them across lanes.
 __device__ __forceinline__ u32 uni(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
+
+// set bits of a wave-wide mask (a ballot) in the lanes below this one
+__device__ __forceinline__ u32 lanes_below(u64 m)
+{
+    return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+}
 
 // Inclusive prefix sum over the 64 lanes of a wave (all lanes active): four
 // row_shr steps inside each 16-lane row, then the row totals by row_bcast:15
@@ -239,102 +254,175 @@ struct RqArgs {
     u32 n_rids;         // counter rows in ctr
 };
 
-// Walk the representation triples with flat indices [c0, c1) of one window
-// (lexicographic index triples x<y<z inside each S, S in bucket order) and emit
-// the 4-sets of both classes (exhaustive/s4.c:324-339).  The buckets are not
-// sorted: the three p values of each triple are put in ascending order here,
-// which is the order the classes are defined in (and makes every 4-set come
-// out a > b > c > d).  With WRITE false only counts.  s_qv holds the q of each
-// representation (p^2 + q^2 = S), so q is read rather than recomputed; p
-// ascending is q descending.
-template <bool WRITE>
-__device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u32 *s_tp, const u32 *s_pv,
-                       const u32 *s_qv, u32 nq, u64 B0, ps4_quad *out, u64 obase, u64 ocap, u32 rid, u32 verify,
-                       u32 *err)
+// A wave's place in the 4-set queue.  The queue is filled a tile (64 entries,
+// the extension kernel's unit) at a time: a wave appends its 4-sets at
+// consecutive slots of its current tile and takes its next tile from a run of
+// tiles it claimed with one atomic, 1, 2, 4, then 8 tiles per claim (the
+// queue's counter is one address, so claims stay rare).  Every tile holds the
+// 4-sets of one interval: when a wave moves to an item of another interval,
+// and at its end, the unused slots of its tiles are written as null entries.
+#define RQ_RUN_MAX 8u
+struct RqTiles {
+    u64 cur;     // first slot of the tile being filled
+    u64 run;     // first slot of the next unused tile of the claimed run
+    u32 fill;    // entries of the current tile written (64: none left, or no tile yet)
+    u32 left;    // tiles left in the run
+    u32 next;    // tiles of the next claim
+    u32 rid;     // interval of the tiles (PS4_RID_NONE: none)
+};
+
+// a uniform 64-bit value: lane 0's
+__device__ __forceinline__ u64 uni64(u64 v) { return ((u64)uni((u32)(v >> 32)) << 32) | uni((u32)v); }
+
+__device__ __forceinline__ void rq_tiles_reset(RqTiles &tl, u32 rid)
 {
-    if (c0 >= c1) return 0;
-    u32 lo = 0, hi = nq;
-    while (hi - lo > 1) {
-        const u32 mid = (lo + hi) >> 1;
-        if (s_tp[mid] <= c0) lo = mid; else hi = mid;
+    tl.cur = tl.run = 0;
+    tl.fill = 64u;
+    tl.left = 0;
+    tl.next = 1;
+    tl.rid = rid;
+}
+
+// Append one 4-set {w0, w1} for every lane with `on` (every lane active).
+__device__ __forceinline__ void rq_put(RqTiles &tl, bool on, const uint4 &w0, const uint4 &w1, ps4_quad *out, u64 ocap,
+                                      ps4_ctl *ctl, u32 *err)
+{
+    const u64 m = __ballot(on);
+    if (m == 0ull) return;
+    const u32 cnt = (u32)__popcll(m);
+    const u32 idx = tl.fill + lanes_below(m);
+    const bool cross = tl.fill + cnt >= 64u;   // uniform: the current tile fills up
+    u64 nx = 0;
+    if (cross) {
+        if (tl.left == 0) {   // claim the next run of tiles
+            u64 v = 0;
+            if (lane_id() == 0)
+                v = atomicAdd((unsigned long long *)&ctl->quad_count, (unsigned long long)(64u * tl.next));
+            tl.run = uni64(v);
+            tl.left = tl.next;
+            tl.next = tl.next < RQ_RUN_MAX ? 2u * tl.next : RQ_RUN_MAX;
+        }
+        nx = tl.run;
+        tl.run += 64u;
+        --tl.left;
     }
-    u32 j = lo;
-    u32 tau = c0 - s_tp[j];
+    if (on) {
+        const u64 pos = (idx < 64u ? tl.cur : nx) + (idx & 63u);
+        if (pos < ocap) {
+            uint4 *o = (uint4 *)(out + pos);
+            o[0] = w0;
+            o[1] = w1;
+        } else {
+            *err |= ERR_QUAD_OVF;
+        }
+    }
+    tl.fill += cnt;
+    if (cross) {
+        tl.cur = nx;
+        tl.fill -= 64u;
+    }
+}
+
+// Null entries in the unused slots of a wave's tiles: the rest of the current
+// tile and the unused tiles of its run (every lane active).
+__device__ __forceinline__ void rq_pad(const RqTiles &tl, ps4_quad *out, u64 ocap, u32 *err)
+{
+    const u32 l = lane_id();
+    const uint4 z0 = make_uint4(0u, 0u, 0u, 0u), z1 = make_uint4(0u, 0u, PS4_RID_NONE, 0u);
+    for (u32 t = 0; t <= tl.left; ++t) {
+        const u64 pos = (t == 0 ? tl.cur : tl.run + 64u * (t - 1u)) + l;
+        if (t > 0 || l >= tl.fill) {
+            if (pos < ocap) {
+                uint4 *o = (uint4 *)(out + pos);
+                o[0] = z0;
+                o[1] = z1;
+            } else {
+                *err |= ERR_QUAD_OVF;
+            }
+        }
+    }
+}
+
+// Walk the representation triples with flat indices [c0, c1) of one window
+// (lexicographic index triples x<y<z inside each S, S in bucket order) and
+// append the 4-sets of both classes to the wave's tiles (exhaustive/s4.c:324-339);
+// returns their number.  Every lane runs nit >= c1 - c0 iterations (the tile
+// appends are wave-wide), those past its range without effect.  The buckets
+// are not sorted: the three p values of each triple are put in ascending order
+// here, which is the order the classes are defined in (and makes every 4-set
+// come out a > b > c > d).  s_qv holds the q of each representation
+// (p^2 + q^2 = S), so q is read rather than recomputed; p ascending is q
+// descending.  The window must have at least one triple.
+__device__ u32 rq_walk(u32 c0, u32 c1, u32 nit, const u32 *s_sk, const u32 *s_st, const u32 *s_tp, const u32 *s_pv,
+                       const u32 *s_qv, u32 nq, u64 B0, RqTiles &tl, ps4_quad *out, u64 ocap, ps4_ctl *ctl, u32 rid,
+                       u32 verify, u32 *err)
+{
+    u32 j = 0, x = 0, y = 1, z = 2;   // a lane without triples stays on the first one
+    if (c0 < c1) {
+        u32 lo = 0, hi = nq;
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) >> 1;
+            if (s_tp[mid] <= c0) lo = mid; else hi = mid;
+        }
+        j = lo;
+        u32 tau = c0 - s_tp[j];
+        const u32 kk = s_sk[j] >> 16;
+        for (;;) {
+            const u32 m = kk - 1 - x, c = m * (m - 1) / 2;
+            if (tau < c) break;
+            tau -= c;
+            ++x;
+        }
+        y = x + 1;
+        while (tau >= kk - 1 - y) { tau -= kk - 1 - y; ++y; }
+        z = y + 1 + tau;
+    }
     u32 sk = s_sk[j], k = sk >> 16, st = s_st[j];
-    u32 x = 0;
-    for (;;) {
-        const u32 m = k - 1 - x, c = m * (m - 1) / 2;
-        if (tau < c) break;
-        tau -= c;
-        ++x;
-    }
-    u32 y = x + 1;
-    while (tau >= k - 1 - y) { tau -= k - 1 - y; ++y; }
-    u32 z = y + 1 + tau;
     const double B0d = (double)B0;
     const u32 b0 = (u32)B0;
     double Sd = B0d + (double)(sk & 0xFFFFu);
     u32 Sodd = (b0 + sk) & 1u;
     u32 n = 0;
     u32 vx = s_pv[st + x], vy = s_pv[st + y], vz = s_pv[st + z];
-    u32 wx = WRITE ? s_qv[st + x] : 0u, wy = WRITE ? s_qv[st + y] : 0u, wz = WRITE ? s_qv[st + z] : 0u;
-    for (u32 c = c0; c < c1; ++c) {
+    u32 wx = s_qv[st + x], wy = s_qv[st + y], wz = s_qv[st + z];
+    for (u32 i = 0; i < nit; ++i) {
+        const u32 c = c0 + i;
+        const bool live = c < c1;
         const u32 lo2 = min(vx, vy), hi2 = max(vx, vy);
         const u32 px = min(lo2, vz), pz = max(hi2, vz), py = max(lo2, min(hi2, vz));
         const double Px = (double)px * px, Py = (double)py * py, Pz = (double)pz * pz;
         const double sxy = Px + Py;
         const u32 par = (px + py + pz) & 1u;
         // class A: sums (Q_x,Q_y,Q_z), p-sum even, d = (P_x+P_y-P_z)/2 > 0
-        const bool ca = (par == 0) && (sxy > Pz);
+        const bool ca = live && (par == 0) && (sxy > Pz);
         // class B: sums (Q_x,Q_y,P_z), p-sum = S mod 2, 2P_z != S, d = (P_x+P_y+P_z-S)/2 > 0
-        const bool cb = (par == Sodd) && (Pz + Pz != Sd) && (sxy + Pz > Sd);
-        if (WRITE) {
-            if (ca || cb) {
-                const u32 wlo = min(wx, wy), whi = max(wx, wy);
-                const u32 qx = max(whi, wz), qz = min(wlo, wz), qy = max(wlo, min(whi, wz));
-                if (ca) {
-                    if (obase + n < ocap) {
-                        uint4 *o = (uint4 *)(out + obase + n);
-                        o[0] = make_uint4(qx, qy, qz, pz);
-                        o[1] = make_uint4(py, px, rid, 0u);
-                    } else {
-                        *err |= ERR_BOUNDS;
-                    }
-                    ++n;
-                }
-                if (cb) {
-                    if (obase + n < ocap) {
-                        uint4 *o = (uint4 *)(out + obase + n);
-                        o[0] = make_uint4(qx, qy, pz, qz);
-                        o[1] = make_uint4(py, px, rid, 0u);
-                    } else {
-                        *err |= ERR_BOUNDS;
-                    }
-                    ++n;
-                }
-                if (verify) {
-                    // the six roots of both 4-sets (exhaustive/s4.c:229-233)
-                    for (int cl = 0; cl < 2; ++cl) {
-                        if (cl == 0 ? !ca : !cb) continue;
-                        const u64 rab = qx, rac = qy, rad = cl == 0 ? qz : pz, rbc = cl == 0 ? pz : qz, rbd = py, rcd = px;
-                        const u64 va = (rab * rab + rac * rac - rbc * rbc) / 2;
-                        const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
-                        const bool ok = va > vb && vb > vc && vc > vd && vd > 0 && vb + vc == rbc * rbc &&
-                                        vb + vd == rbd * rbd && vc + vd == rcd * rcd && va + vd == rad * rad &&
-                                        va + vb + vc + vd == B0 + (sk & 0xFFFFu);
-                        if (!ok) *err |= ERR_VERIFY;
-                    }
-                }
+        const bool cb = live && (par == Sodd) && (Pz + Pz != Sd) && (sxy + Pz > Sd);
+        n += (u32)ca + (u32)cb;
+        const u32 wlo = min(wx, wy), whi = max(wx, wy);
+        const u32 qx = max(whi, wz), qz = min(wlo, wz), qy = max(wlo, min(whi, wz));
+        const uint4 tail = make_uint4(py, px, rid, 0u);
+        rq_put(tl, ca, make_uint4(qx, qy, qz, pz), tail, out, ocap, ctl, err);
+        rq_put(tl, cb, make_uint4(qx, qy, pz, qz), tail, out, ocap, ctl, err);
+        if (verify && (ca || cb)) {
+            // the six roots of both 4-sets (exhaustive/s4.c:229-233)
+            for (int cl = 0; cl < 2; ++cl) {
+                if (cl == 0 ? !ca : !cb) continue;
+                const u64 rab = qx, rac = qy, rad = cl == 0 ? qz : pz, rbc = cl == 0 ? pz : qz, rbd = py, rcd = px;
+                const u64 va = (rab * rab + rac * rac - rbc * rbc) / 2;
+                const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
+                const bool ok = va > vb && vb > vc && vc > vd && vd > 0 && vb + vc == rbc * rbc &&
+                                vb + vd == rbd * rbd && vc + vd == rcd * rcd && va + vd == rad * rad &&
+                                va + vb + vc + vd == B0 + (sk & 0xFFFFu);
+                if (!ok) *err |= ERR_VERIFY;
             }
-        } else {
-            n += (u32)ca + (u32)cb;
         }
         // the next triple in lexicographic order: z advances, a z at k moves y
         // on, a y at k - 1 moves x on, and an x at k - 2 moves to the next
         // counted S (an exec-masked block, rare per lane).  The three p (and q)
         // of the next triple are read together, so one LDS round trip per
-        // triple; every index stays inside the current or next S's bucket.
-        const bool tz = z + 1u == k, ty = tz && y + 2u == k, tn = ty && x + 3u == k;
+        // triple; every index stays inside the current or next S's bucket, and
+        // a lane past its range stays where it is.
+        const bool tz = live && z + 1u == k, ty = tz && y + 2u == k, tn = ty && x + 3u == k;
         if (tn && c + 1 < c1) {
             ++j;
             sk = s_sk[j];
@@ -345,15 +433,13 @@ __device__ u32 rq_walk(u32 c0, u32 c1, const u32 *s_sk, const u32 *s_st, const u
         }
         x = tn ? 0u : x + (ty ? 1u : 0u);
         y = ty ? x + 1u : y + (tz ? 1u : 0u);
-        z = tz ? y + 1u : z + 1u;
+        z = tz ? y + 1u : z + (live ? 1u : 0u);
         vx = s_pv[st + x];
         vy = s_pv[st + y];
         vz = s_pv[st + z];
-        if (WRITE) {
-            wx = s_qv[st + x];
-            wy = s_qv[st + y];
-            wz = s_qv[st + z];
-        }
+        wx = s_qv[st + x];
+        wy = s_qv[st + y];
+        wz = s_qv[st + z];
     }
     return n;
 }
@@ -370,10 +456,12 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
     __shared__ u32 s_w[2 * RQ_NT / 64];
     __shared__ u64 s_w64[RQ_NT / 64];
     __shared__ u32 s_item;
-    __shared__ u64 s_gbase;
 
     const u32 tid = threadIdx.x;
     u64 *scr = a.scratch + (u64)blockIdx.x * a.nb_alloc * a.bin_cap;
+    u32 err = 0;
+    RqTiles tl;   // this wave's tiles of the 4-set queue
+    rq_tiles_reset(tl, PS4_RID_NONE);
 
     for (;;) {
         // The barrier first: it separates this claim from the previous item's
@@ -386,6 +474,10 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
         if (it >= a.n_items) break;
         ps4_item item = a.items[it];
         item.rid = uni(item.rid);
+        if (item.rid != tl.rid) {   // tiles hold one interval's 4-sets
+            rq_pad(tl, a.quads, a.quad_cap, &err);
+            rq_tiles_reset(tl, item.rid);
+        }
         const u64 X0 = item.lo, X1 = item.hi, len = X1 - X0;
         const u32 nb = (u32)((len + RQ_W - 1) >> RQ_LOGW);
         // a malformed item (never produced by the host) is skipped, not walked
@@ -443,7 +535,6 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
         __syncthreads();
 
         u64 c_pairs = 0, c_sge3 = 0, c_trip = 0, c_sets4 = 0, c_maxk = 0;
-        u32 err = 0;
         for (u32 b = 0; b < nb; ++b) {
             const u64 B0 = X0 + ((u64)b << RQ_LOGW);
             u32 npts = uni(s_bin[b]);
@@ -466,27 +557,56 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                     if (i0 + u * RQ_NT < npts) atomicAdd(&s_cnt[sv[u] >> 1], 1u << ((sv[u] & 1u) << 4));
             }
             __syncthreads();
-            // pass 2: which S count, offsets (exhaustive/s4.c:300-305, 314-321)
+            // pass 2: which S count, offsets (exhaustive/s4.c:300-305, 314-321).
+            // Thread t owns the 32 S from s0 = 32 t: 16 count words, read as four
+            // 16-byte words.  Counted S (k >= 3, S != 2 mod 4) are a few per cent,
+            // so one sweep sums and maxes the counts two at a time and marks the
+            // counted S in a 32-bit mask; only the marked S are visited after.
             constexpr u32 SPT = RQ_W / RQ_NT;
+            static_assert(SPT == 32, "one 32-bit mask of counted S per thread");
             const u32 s0 = tid * SPT;
-            const u32 b0 = (u32)B0;   // S mod 4 = (b0 + s) mod 4
-            u32 nq = 0, qr = 0, tr = 0, w_pairs = 0, w_maxk = 0;
-            bool big = false;
-            for (u32 w = 0; w < SPT / 2; ++w) {
-                const u32 word = s_cnt[(s0 >> 1) + w];
+            const u32 b0 = (u32)B0;   // S mod 4 = (b0 + s) mod 4, and s0 = 0 mod 4
+            // bit 15 (low half) / 31 (high half) set where that S may count: word
+            // w holds S = s0 + 2w and s0 + 2w + 1, so the pattern alternates with w
+            u32 okm[2];
 #pragma unroll
-                for (u32 h = 0; h < 2; ++h) {
-                    const u32 k = (word >> (16 * h)) & 0xFFFFu;
-                    w_pairs += k;
-                    w_maxk = k > w_maxk ? k : w_maxk;
-                    if (k >= 3 && ((b0 + s0 + 2 * w + h) & 3u) != 2u) {
-                        ++nq;
-                        qr += k;
-                        if (k > a.cap_reps) err |= ERR_CAP_REPS;
-                        if (k > RQ_K_MAX) big = true;
-                        tr += tri3(k < RQ_K_MAX ? k : RQ_K_MAX);
-                    }
-                }
+            for (u32 par = 0; par < 2; ++par) {
+                u32 m = 0x80008000u;
+#pragma unroll
+                for (u32 h = 0; h < 2; ++h)
+                    if (((b0 + 2u * par + h) & 3u) == 2u) m &= ~(0x8000u << (16u * h));
+                okm[par] = m;
+            }
+            uint4 *cw = (uint4 *)s_cnt + tid * (SPT / 8);
+            uint4 cv[SPT / 8];
+#pragma unroll
+            for (u32 i = 0; i < SPT / 8; ++i) cv[i] = cw[i];
+            u32 w_pairs = 0, pmax = 0, cm = 0;
+#pragma unroll
+            for (u32 w = 0; w < SPT / 2; ++w) {
+                u32 &word = (w & 3u) == 0 ? cv[w >> 2].x : (w & 3u) == 1 ? cv[w >> 2].y : (w & 3u) == 2 ? cv[w >> 2].z
+                                                                                                       : cv[w >> 2].w;
+                w_pairs = PS4_SAD_U16(word, 0u, w_pairs);   // + both counts
+                pmax = PS4_MAX_U16X2(pmax, word);
+                // a count >= 3 sets the top bit of its half (counts stay below 0x8003,
+                // so no carry crosses into the high half)
+                const u32 g = (word + 0x7FFD7FFDu) & okm[w & 1u];
+                cm |= (((g >> 15) & 1u) | ((g >> 30) & 2u)) << (2u * w);
+                // the halves of S that do not count become 0xFFFF (the fill below skips
+                // them); counted halves keep k until their offset replaces it
+                word |= ~((g >> 15) * 0xFFFFu);
+            }
+            const u32 w_maxk = max(pmax & 0xFFFFu, pmax >> 16);
+            const u32 nq = (u32)__builtin_popcount(cm);
+            u32 qr = 0, tr = 0;
+            bool big = false;
+            for (u32 m = cm; m != 0u; m &= m - 1u) {
+                const u32 i = (u32)__builtin_ctz(m);
+                const u32 k = (s_cnt[(s0 + i) >> 1] >> ((i & 1u) << 4)) & 0xFFFFu;
+                qr += k;
+                if (k > a.cap_reps) err |= ERR_CAP_REPS;
+                if (k > RQ_K_MAX) big = true;
+                tr += tri3(k < RQ_K_MAX ? k : RQ_K_MAX);
             }
             c_pairs += w_pairs;
             c_maxk = w_maxk > c_maxk ? w_maxk : c_maxk;
@@ -509,26 +629,21 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                 err |= ERR_LDS_OVF;
                 continue;
             }
-            for (u32 w = 0; w < SPT / 2; ++w) {
-                const u32 word = s_cnt[(s0 >> 1) + w];
-                u32 nw = 0;
 #pragma unroll
-                for (u32 h = 0; h < 2; ++h) {
-                    const u32 k = (word >> (16 * h)) & 0xFFFFu;
-                    const u32 s = s0 + 2 * w + h;
-                    u32 o = 0xFFFFu;
-                    if (k >= 3 && ((b0 + s) & 3u) != 2u) {
-                        s_sk[b_nq] = s | (k << 16);
-                        s_st[b_nq] = b_qr;
-                        s_tp[b_nq] = b_tr;
-                        o = b_qr;
-                        ++b_nq;
-                        b_qr += k;
-                        b_tr += tri3(k);
-                    }
-                    nw |= o << (16 * h);
-                }
-                s_cnt[(s0 >> 1) + w] = nw;
+            for (u32 i = 0; i < SPT / 8; ++i) cw[i] = cv[i];
+            // the counted S in order: bucket records, and the bucket offset in
+            // place of k (the thread's own words)
+            for (u32 m = cm; m != 0u; m &= m - 1u) {
+                const u32 i = (u32)__builtin_ctz(m), s = s0 + i, sh = (i & 1u) << 4;
+                u32 *pw = s_cnt + (s >> 1);
+                const u32 word = *pw, k = (word >> sh) & 0xFFFFu;
+                s_sk[b_nq] = s | (k << 16);
+                s_st[b_nq] = b_qr;
+                s_tp[b_nq] = b_tr;
+                *pw = (word & ~(0xFFFFu << sh)) | (b_qr << sh);
+                ++b_nq;
+                b_qr += k;
+                b_tr += tri3(k);
             }
             if (tid == 0) s_tp[tot_nq] = tot_tr;
             __syncthreads();
@@ -554,32 +669,12 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                 }
             }
             __syncthreads();
-            // pass 4: triples -> 4-sets, count then write
+            // pass 4: triples -> 4-sets, appended to the wave's tiles of the queue
             const u32 T = tot_tr;   // uniform
-            const u32 c0 = (u32)(((u64)tid * T) / RQ_NT), c1 = (u32)(((u64)(tid + 1) * T) / RQ_NT);
-            const u32 n4 = rq_walk<false>(c0, c1, s_sk, s_st, s_tp, s_pv, s_qv, tot_nq, B0, nullptr, 0, 0, 0, 0, &err);
-            u32 tot4;
-            const u32 b4 = block_excl_scan<RQ_NT>(n4, s_w, &tot4);
-            tot4 = uni(tot4);
-            // the window's block of the queue is a whole number of k_ext tiles (64),
-            // padded with null entries, so that no tile mixes intervals
-            const u32 tot4p = (tot4 + 63u) & ~63u;
-            if (tid == 0)
-                s_gbase = tot4 ? (u64)atomicAdd((unsigned long long *)&a.ctl->quad_count, (unsigned long long)tot4p) : 0ull;
-            __syncthreads();
-            const u64 gb = ((u64)uni((u32)(s_gbase >> 32)) << 32) | uni((u32)s_gbase);
-            c_sets4 += n4;
-            if (gb + tot4p > a.quad_cap) {
-                err |= ERR_QUAD_OVF;
-            } else {
-                if (n4)
-                    rq_walk<true>(c0, c1, s_sk, s_st, s_tp, s_pv, s_qv, tot_nq, B0, a.quads, gb + b4, a.quad_cap, item.rid,
-                                  a.verify, &err);
-                if (tid < tot4p - tot4) {
-                    uint4 *o = (uint4 *)(a.quads + gb + tot4 + tid);
-                    o[0] = make_uint4(0u, 0u, 0u, 0u);
-                    o[1] = make_uint4(0u, 0u, PS4_RID_NONE, 0u);
-                }
+            if (T > 0) {
+                const u32 c0 = (u32)(((u64)tid * T) / RQ_NT), c1 = (u32)(((u64)(tid + 1) * T) / RQ_NT);
+                c_sets4 += rq_walk(c0, c1, (T + RQ_NT - 1) / RQ_NT, s_sk, s_st, s_tp, s_pv, s_qv, tot_nq, B0, tl,
+                                   a.quads, a.quad_cap, a.ctl, item.rid, a.verify, &err);
             }
             __syncthreads();
         }
@@ -600,6 +695,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             atomicMax(c + CT_MAX_REPS, (unsigned long long)t_maxk);
         }
     }
+    rq_pad(tl, a.quads, a.quad_cap, &err);
+    if (err) atomicOr(&a.ctl->err, err);
 }
 
 // ----------------------------------------------------------------------------
@@ -607,14 +704,14 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 // ----------------------------------------------------------------------------
 
 #define EX_NT 64          // one wave per workgroup: every barrier below is a wave's own
-#define EX_ARENA 800      // doubles of divisor lists per pass (the workgroup's LDS stays at
+#define EX_ARENA 864      // doubles of divisor lists per pass (the workgroup's LDS stays at
                           // 16 KB); one 4-set with d(D) <= 16384 needs at most 748 (nL + nH
                           // over the best prefix split, reached by the radices 2, 11, 33, 2, 11)
 #define EX_MAXF 12        // distinct odd primes of D < 2^51 (3*5*...*43 > 2^51)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
 #define EX_CHUNK 4        // tiles claimed per work-queue atomic
-#define EX_Q1 192         // LDS stack of candidates past the residue filters (rounds of 64
-                          // run while it holds >= 64, before an iteration adds at most 128)
+#define EX_Q1 128         // LDS stack of candidates past the residue filters (a round of 64
+                          // runs once it holds >= 64; each push adds at most 64)
 #define EX_Q2 128         // queue of candidates for the exact test, per workgroup in global
                           // memory, kept across passes and tiles (drained at >= 64 and at
                           // the end; a round of the first queue adds at most 64)
@@ -905,12 +1002,6 @@ __device__ __forceinline__ u64 wave_max_u64(u64 v)
     for (int m = 32; m >= 1; m >>= 1) { const u64 o = shfl_xor_u64(v, m); v = o > v ? o : v; }
     return v;
 }
-// set bits of a wave-wide mask (a ballot) in the lanes below this one
-__device__ __forceinline__ u32 lanes_below(u64 m)
-{
-    return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
-}
-
 // exclusive prefix sum over the wave; *total gets the sum (uniform)
 __device__ __forceinline__ u32 wave_excl_scan(u32 v, u32 *total)
 {
@@ -1326,14 +1417,16 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                         qn1 = b1;
                         if (qn2 >= 64u) drain2();
                     };
-                    while (qn1 >= 64u) round1();   // then at most 63 + 128 after this iteration
-                    const u64 boa = __ballot(oka), bob = __ballot(okb);
+                    // (the stack holds < 64 entries before each push of at most 64)
+                    const u64 boa = __ballot(oka);
                     if (oka) s_q1[qn1 + lanes_below(boa)] = ex_qent(A.u, ka0);
                     qn1 += (u32)__popcll(boa);
+                    if (qn1 >= 64u) round1();
+                    const u64 bob = __ballot(okb);
                     if (okb) s_q1[qn1 + lanes_below(bob)] = ex_qent(B.u, kb0);
                     qn1 += (u32)__popcll(bob);
-                    if (base + 128u >= ctot)   // the pass's records are about to be replaced
-                        while (qn1 > 0u) round1();
+                    while (qn1 >= 64u || (base + 128u >= ctot && qn1 > 0u))   // (the pass's records
+                        round1();                                              //  are about to be replaced)
                 }
                 __syncthreads();
                 if (fits) pending = false;

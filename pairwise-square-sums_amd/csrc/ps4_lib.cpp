@@ -549,22 +549,26 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         u64 remaining = 0;
         for (size_t i = next_iv; i < n; ++i) remaining += iv[i].hi - iv[i].cur;
         u64 item_max = ctx->W;
+        // null entries an item may leave in the 4-set queue: each wave of the
+        // representation kernel ends an item with up to two partly used tiles
+        const double item_pad = 128.0 * (double)(ps4_rq_threads() / 64);
         while (item_max * ctx->rq_grid < remaining && item_max < ((u64)1 << 40)) item_max <<= 1;
         for (size_t i = next_iv; i < n && n_items < ctx->items_cap && quads_expected < quads_budget; ++i) {
             Interval &v = iv[i];
             while (v.cur < v.hi && n_items < ctx->items_cap && quads_expected < quads_budget) {
                 u64 len = std::min(superwindow(ctx, v.cur), item_max);
                 if (len > v.hi - v.cur) len = v.hi - v.cur;
-                const double dq = quad_density(v.cur + len) * (double)len;
+                const double dq = quad_density(v.cur + len) * (double)len + item_pad;
                 if (n_items > 0 && quads_expected + dq > quads_budget) {
                     // shrink the last item to the remaining budget, in whole windows
-                    u64 room = (u64)((quads_budget - quads_expected) / quad_density(v.cur + len));
+                    const double left = quads_budget - quads_expected - item_pad;
+                    u64 room = left > 0.0 ? (u64)(left / quad_density(v.cur + len)) : 0;
                     room = room / ctx->W * ctx->W;
                     if (room == 0) { quads_expected = quads_budget; break; }
                     len = room;
                 }
                 ctx->h_items[n_items++] = {v.cur, v.cur + len, v.rid, 0};
-                quads_expected += quad_density(v.cur + len) * (double)len;
+                quads_expected += quad_density(v.cur + len) * (double)len + item_pad;
                 v.cur += len;
                 rid_hi = v.rid;
             }

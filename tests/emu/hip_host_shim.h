@@ -29,6 +29,11 @@
 #define __noinline__ __attribute__((noinline))
 #define __launch_bounds__(...)
 #define PS4_KEEP(...) ((void)0)
+#define PS4_MAX_U16X2(a, b)                                                                       \
+    ((std::max((uint32_t)(a) & 0xFFFFu, (uint32_t)(b) & 0xFFFFu)) | (std::max((uint32_t)(a) >> 16, (uint32_t)(b) >> 16) << 16))
+#define PS4_SAD_U16(a, b, c)                                                                           \
+    ((uint32_t)(c) + (uint32_t)std::abs((int)((uint32_t)(a) & 0xFFFFu) - (int)((uint32_t)(b) & 0xFFFFu)) + \
+     (uint32_t)std::abs((int)((uint32_t)(a) >> 16) - (int)((uint32_t)(b) >> 16)))
 #define PS4_MAD_U24(a, b, c) (((uint32_t)(a) & 0xFFFFFFu) * ((uint32_t)(b) & 0xFFFFFFu) + (uint32_t)(c))
 #define PS4_MAD_I24(a, b, c)                                                                           \
     ((uint32_t)((int64_t)((int32_t)((uint32_t)(a) << 8) >> 8) * (int64_t)((int32_t)((uint32_t)(b) << 8) >> 8) + \
