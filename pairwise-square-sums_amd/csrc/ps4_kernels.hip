@@ -540,7 +540,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             u32 npts = uni(s_bin[b]);
             if (npts > a.bin_cap) { err |= ERR_BIN_OVF; npts = a.bin_cap; }
             const u64 *pts = scr + (u64)b * a.bin_cap;
-            for (u32 i = tid; i < RQ_W / 2; i += RQ_NT) s_cnt[i] = 0;
+            for (u32 i = tid; i < RQ_W / 8; i += RQ_NT) ((uint4 *)s_cnt)[i] = make_uint4(0u, 0u, 0u, 0u);
             __syncthreads();
             // pass 1: count (exhaustive/s4.c:285-298); RQ_UNROLL points per thread
             // loaded together (independent loads in flight), then counted
@@ -1141,18 +1141,53 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                         if (e < bestv) { bestv = e; best = (u32)t; }
                     }
                     const ExPair P = ex_pair(best, va, vb, vc, vd, rab, rac, rad, rbc, rbd, rcd);
-                    // D = m1 * m2: strip the twos, then walk both odd parts through
-                    // the table side by side (two independent load chains)
+                    // D = m1 * m2: strip the twos; the first table entries of both odd
+                    // parts are read now, ahead of the residue tables
                     u64 m1 = P.ra - P.rb, m2 = P.ra + P.rb;
                     const u32 z1 = (u32)__builtin_ctzll(m1), z2 = (u32)__builtin_ctzll(m2);
                     const u32 e2 = z1 + z2;
                     m1 >>= z1;
                     m2 >>= z2;
+                    auto ftab_at = [&](u64 m) -> u64 { return m > 1 && m < a.tab_n ? a.ftab[m] : 0ull; };
+                    u64 en1 = ftab_at(m1), en2 = ftab_at(m2);
+                    // quadratic-residue masks of this 4-set (read before the factor walk,
+                    // so that the reads overlap it): bit r of M_m set iff
+                    // r^2 + (C1-A) and r^2 + (C2-A) are both squares mod m (the
+                    // filters of exhaustive/s4.c:192-194)
+                    const double d1 = (double)(i64)(P.C1 - P.A), d2 = (double)(i64)(P.C2 - P.A);
+                    // representatives mod 262080 (residues mod 64, 63, 65 follow) and
+                    // exact residues mod 46189; |C - A| < 2^51
+                    const u32 r1a = rep_m1(d1), r1b = rep_m1(d2);
+                    const u32 r2a = mod_m2(d1), r2b = mod_m2(d2);
+                    // residues index small tables: each index is checked against
+                    // its table (a wrong residue is an error, never a load outside it)
+                    u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(r1a), mod63(r1b), mod65(r1a), mod65(r1b), mod11(r2a),
+                                  mod11(r2b), mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
+                    const u32 lim[7] = {64, 63, 65, 11, 13, 17, 19};
+#pragma unroll
+                    for (int q = 0; q < 14; ++q)
+                        if (ix[q] >= lim[q >> 1]) { err |= ERR_BOUNDS; ix[q] = 0; }
+                    // (read here, combined after the walk)
+                    const u64 t64a = qr->t64[ix[0]], t64b = qr->t64[ix[1]], t63a = qr->t63[ix[2]],
+                              t63b = qr->t63[ix[3]], t65a = qr->t65[ix[4]][0], t65b = qr->t65[ix[5]][0],
+                              t65c = qr->t65[ix[4]][1], t65d = qr->t65[ix[5]][1];
+                    const u32 t11a = qr->t11[ix[6]], t11b = qr->t11[ix[7]], t13a = qr->t13[ix[8]], t13b = qr->t13[ix[9]],
+                              t17a = qr->t17[ix[10]], t17b = qr->t17[ix[11]], t19a = qr->t19[ix[12]],
+                              t19b = qr->t19[ix[13]];
+                    // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m,
+                    // i.e. u is a root of a pair sum containing A, or u^2 = 2A
+                    // (every root is below 2^26 for S < 2^50)
+                    const u64 sa = d_isqrt52(P.A), s2a = d_isqrt52(2 * P.A);
+                    psqA = (u32)sa;
+                    pf1 = (u32)P.F1;
+                    pf2 = (u32)P.F2;
+                    pf3 = (u32)P.F3;
+                    pf4 = (s2a * s2a == 2 * P.A) ? (u32)s2a : 0u;   // u >= 2: 0 never matches
+                    // the walk of D = m1 * m2 through the table (below): both odd
+                    // parts side by side (two independent load chains)
                     u64 seen = 0;
                     for (int guard = 0; m1 > 1 || m2 > 1; ++guard) {
                         if (m1 >= a.tab_n || m2 >= a.tab_n || guard >= 64) { err |= ERR_FACTORS; break; }   // never for a valid 4-set
-                        const u64 en1 = m1 > 1 ? a.ftab[m1] : 0ull;
-                        const u64 en2 = m2 > 1 ? a.ftab[m2] : 0ull;
                         if (m1 > 1) {
                             ex_add_factor(s_fac, tid, (u32)(en1 >> 32) & 0x7FFFFFFu, (u32)(en1 >> 59), nf, seen, err);
                             m1 = en1 & 0xFFFFFFFFull;
@@ -1161,7 +1196,14 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                             ex_add_factor(s_fac, tid, (u32)(en2 >> 32) & 0x7FFFFFFu, (u32)(en2 >> 59), nf, seen, err);
                             m2 = en2 & 0xFFFFFFFFull;
                         }
+                        en1 = ftab_at(m1);
+                        en2 = ftab_at(m2);
                     }
+                    pm64 = t64a & t64b;
+                    pm63 = t63a & t63b;
+                    pm65 = t65a & t65b;
+                    pmsm = (t65c & t65d & 1ull) | ((u64)(t11a & t11b) << 1) | ((u64)(t13a & t13b) << 12) |
+                           ((u64)(t17a & t17b) << 25) | ((u64)(t19a & t19b) << 42);
                     u64 ndodd = 1;
                     for (u32 i = 0; i < nf; ++i) ndodd *= (s_fac[i * EX_NT + tid] & 31u) + 1;
                     const u64 ndD = (u64)(e2 + 1) * ndodd;
@@ -1198,40 +1240,6 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                             if (nL * nH != ndp || ncand > EX_CMAX) { err |= ERR_FACTORS; ncand = 0; }   // never
                         }
                     }
-                    // quadratic-residue masks of this 4-set: bit r of M_m set iff
-                    // r^2 + (C1-A) and r^2 + (C2-A) are both squares mod m (the
-                    // filters of exhaustive/s4.c:192-194)
-                    const double d1 = (double)(i64)(P.C1 - P.A), d2 = (double)(i64)(P.C2 - P.A);
-                    // representatives mod 262080 (residues mod 64, 63, 65 follow) and
-                    // exact residues mod 46189; |C - A| < 2^51
-                    const u32 r1a = rep_m1(d1), r1b = rep_m1(d2);
-                    const u32 r2a = mod_m2(d1), r2b = mod_m2(d2);
-                    // residues index small tables: each index is checked against
-                    // its table (a wrong residue is an error, never a load outside it)
-                    u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(r1a), mod63(r1b), mod65(r1a), mod65(r1b), mod11(r2a),
-                                  mod11(r2b), mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
-                    const u32 lim[7] = {64, 63, 65, 11, 13, 17, 19};
-#pragma unroll
-                    for (int q = 0; q < 14; ++q)
-                        if (ix[q] >= lim[q >> 1]) { err |= ERR_BOUNDS; ix[q] = 0; }
-                    pm64 = qr->t64[ix[0]] & qr->t64[ix[1]];
-                    pm63 = qr->t63[ix[2]] & qr->t63[ix[3]];
-                    pm65 = qr->t65[ix[4]][0] & qr->t65[ix[5]][0];
-                    const u64 m65hi = qr->t65[ix[4]][1] & qr->t65[ix[5]][1] & 1ull;
-                    const u64 m11 = qr->t11[ix[6]] & qr->t11[ix[7]];
-                    const u64 m13 = qr->t13[ix[8]] & qr->t13[ix[9]];
-                    const u64 m17 = qr->t17[ix[10]] & qr->t17[ix[11]];
-                    const u64 m19 = qr->t19[ix[12]] & qr->t19[ix[13]];
-                    pmsm = m65hi | (m11 << 1) | (m13 << 12) | (m17 << 25) | (m19 << 42);
-                    // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m,
-                    // i.e. u is a root of a pair sum containing A, or u^2 = 2A
-                    // (every root is below 2^26 for S < 2^50)
-                    const u64 sa = d_isqrt52(P.A), s2a = d_isqrt52(2 * P.A);
-                    psqA = (u32)sa;
-                    pf1 = (u32)P.F1;
-                    pf2 = (u32)P.F2;
-                    pf3 = (u32)P.F3;
-                    pf4 = (s2a * s2a == 2 * P.A) ? (u32)s2a : 0u;   // u >= 2: 0 never matches
                 }
             }
 
