@@ -716,11 +716,18 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                           // memory, kept across passes and tiles (drained at >= 64 and at
                           // the end; a round of the first queue adds at most 64)
 
+// Two steps of the factorisation of n by the smallest-prime table: f1 = the
+// entry of n, f2 = the entry of n's cofactor (0 when that is 1), each
+// cofactor | smallest prime << 32 | its exponent << 59.
+struct __align__(16) ExFac {
+    u64 f1, f2;
+};
+
 struct ExArgs {
     const ps4_quad *quads;
     u64 quad_cap;
     ps4_ctl *ctl;
-    const u64 *ftab;        // n -> (cofactor) | (spf << 32) | (exponent << 59)
+    const ExFac *fpair;     // n -> the first two steps of n's factorisation
     u64 tab_n;              // entries in ftab / ndiv
     const uint16_t *ndiv;   // n -> d(n)
     const ps4_qrtab *qr;
@@ -1148,8 +1155,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     const u32 e2 = z1 + z2;
                     m1 >>= z1;
                     m2 >>= z2;
-                    auto ftab_at = [&](u64 m) -> u64 { return m > 1 && m < a.tab_n ? a.ftab[m] : 0ull; };
-                    u64 en1 = ftab_at(m1), en2 = ftab_at(m2);
+                    auto fac_at = [&](u64 m) -> ExFac { return m > 1 && m < a.tab_n ? a.fpair[m] : ExFac{0ull, 0ull}; };
+                    ExFac en1 = fac_at(m1), en2 = fac_at(m2);
                     // quadratic-residue masks of this 4-set (read before the factor walk,
                     // so that the reads overlap it): bit r of M_m set iff
                     // r^2 + (C1-A) and r^2 + (C2-A) are both squares mod m (the
@@ -1188,16 +1195,27 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     u64 seen = 0;
                     for (int guard = 0; m1 > 1 || m2 > 1; ++guard) {
                         if (m1 >= a.tab_n || m2 >= a.tab_n || guard >= 64) { err |= ERR_FACTORS; break; }   // never for a valid 4-set
+                        // two prime powers of each part per step
                         if (m1 > 1) {
-                            ex_add_factor(s_fac, tid, (u32)(en1 >> 32) & 0x7FFFFFFu, (u32)(en1 >> 59), nf, seen, err);
-                            m1 = en1 & 0xFFFFFFFFull;
+                            ex_add_factor(s_fac, tid, (u32)(en1.f1 >> 32) & 0x7FFFFFFu, (u32)(en1.f1 >> 59), nf, seen, err);
+                            m1 = en1.f1 & 0xFFFFFFFFull;
+                            if (m1 > 1) {
+                                ex_add_factor(s_fac, tid, (u32)(en1.f2 >> 32) & 0x7FFFFFFu, (u32)(en1.f2 >> 59), nf, seen,
+                                              err);
+                                m1 = en1.f2 & 0xFFFFFFFFull;
+                            }
                         }
                         if (m2 > 1) {
-                            ex_add_factor(s_fac, tid, (u32)(en2 >> 32) & 0x7FFFFFFu, (u32)(en2 >> 59), nf, seen, err);
-                            m2 = en2 & 0xFFFFFFFFull;
+                            ex_add_factor(s_fac, tid, (u32)(en2.f1 >> 32) & 0x7FFFFFFu, (u32)(en2.f1 >> 59), nf, seen, err);
+                            m2 = en2.f1 & 0xFFFFFFFFull;
+                            if (m2 > 1) {
+                                ex_add_factor(s_fac, tid, (u32)(en2.f2 >> 32) & 0x7FFFFFFu, (u32)(en2.f2 >> 59), nf, seen,
+                                              err);
+                                m2 = en2.f2 & 0xFFFFFFFFull;
+                            }
                         }
-                        en1 = ftab_at(m1);
-                        en2 = ftab_at(m2);
+                        en1 = fac_at(m1);
+                        en2 = fac_at(m2);
                     }
                     pm64 = t64a & t64b;
                     pm63 = t63a & t63b;
@@ -1478,12 +1496,12 @@ extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *it
 }
 
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
-                                     const u64 *ftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
+                                     const u64 *fpair, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
                                      u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq)
 {
     ExArgs a;
     a.n_rids = n_rids;
-    a.quads = quads; a.quad_cap = quad_cap; a.ctl = ctl; a.ftab = ftab; a.tab_n = tab_n; a.ndiv = ndiv; a.qr = qr;
+    a.quads = quads; a.quad_cap = quad_cap; a.ctl = ctl; a.fpair = (const ExFac *)fpair; a.tab_n = tab_n; a.ndiv = ndiv; a.qr = qr;
     a.ctr = ctr;
     a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs; a.exq = (uint4 *)exq;
     a.neg_m1 = -M1;

@@ -30,7 +30,7 @@ extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *it
                                     u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u32 n_rids, u64 *scratch, ps4_quad *quads,
                                     u64 quad_cap, u32 cap_reps, u32 verify);
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
-                                     const u64 *ftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
+                                     const u64 *fpair, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
                                      u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq);
 extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u32 n_ctr);
 extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u64 d_max, const u64 *ftab,
@@ -193,6 +193,7 @@ struct ps4_ctx {
     u32 hit_cap = 0, items_cap = 0, ctr_cap = 0;
     // device
     u64 *d_ftab = nullptr;
+    u64 *d_fpair = nullptr;   // {ftab[n], ftab[cofactor of n]}: two prime powers per read
     uint16_t *d_ndiv = nullptr;
     ps4_qrtab *d_qr = nullptr;
     ps4_ctl *d_ctl = nullptr;
@@ -287,10 +288,19 @@ int build_tables(ps4_ctx *ctx)
     for (u32 a = 0; a < 13; ++a) for (u32 r = 0; r < 13; ++r) if (isq((r * r + a) % 13, 13)) qr.t13[a] |= 1u << r;
     for (u32 a = 0; a < 17; ++a) for (u32 r = 0; r < 17; ++r) if (isq((r * r + a) % 17, 17)) qr.t17[a] |= 1u << r;
     for (u32 a = 0; a < 19; ++a) for (u32 r = 0; r < 19; ++r) if (isq((r * r + a) % 19, 19)) qr.t19[a] |= 1u << r;
+    // the extension kernel's factor walk takes two steps per read
+    std::vector<u64> fpair(2 * n, 0);
+    for (u64 i = 0; i < n; ++i) {
+        const u64 c = ftab[i] & 0xFFFFFFFFull;
+        fpair[2 * i] = ftab[i];
+        fpair[2 * i + 1] = (c > 1 && c < n) ? ftab[c] : 0ull;
+    }
     HIP_OK(hipMalloc(&ctx->d_ftab, n * sizeof(u64)));
+    HIP_OK(hipMalloc(&ctx->d_fpair, 2 * n * sizeof(u64)));
     HIP_OK(hipMalloc(&ctx->d_ndiv, n * sizeof(uint16_t)));
     HIP_OK(hipMalloc(&ctx->d_qr, sizeof(ps4_qrtab)));
     HIP_OK(hipMemcpy(ctx->d_ftab, ftab.data(), n * sizeof(u64), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ctx->d_fpair, fpair.data(), 2 * n * sizeof(u64), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(ctx->d_ndiv, ndiv.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(ctx->d_qr, &qr, sizeof qr, hipMemcpyHostToDevice));
     return 0;
@@ -601,7 +611,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
                                    PS4_ELEM_CAP, ctx->d_ecnt, ctx->n_cu * 8));
         }
         if (!noext)
-            HIP_OK(ps4_launch_ext(ctx->st, ctx->ext_grid, ctx->d_quads, ctx->quad_cap, ctx->d_ctl, ctx->d_ftab,
+            HIP_OK(ps4_launch_ext(ctx->st, ctx->ext_grid, ctx->d_quads, ctx->quad_cap, ctx->d_ctl, ctx->d_fpair,
                                   ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, rid_hi + 1, ctx->d_hits,
                                   ctx->hit_cap, ctx->cap_divs, ctx->d_exq));
         HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
@@ -719,7 +729,7 @@ extern "C" void ps4_close(ps4_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
-    void *dev[] = {ctx->d_ftab, ctx->d_ndiv, ctx->d_qr, ctx->d_ctl, ctx->d_ctr, ctx->d_items, ctx->d_scratch,
+    void *dev[] = {ctx->d_ftab, ctx->d_fpair, ctx->d_ndiv, ctx->d_qr, ctx->d_ctl, ctx->d_ctr, ctx->d_items, ctx->d_scratch,
                    ctx->d_quads, ctx->d_hits, ctx->d_exq, ctx->d_eout, ctx->d_ecnt, ctx->d_etri, ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
     for (void *p : dev) if (p) (void)hipFree(p);
     void *host[] = {ctx->h_ctl, ctx->h_ctr, ctx->h_items, ctx->h_hits};

@@ -28,6 +28,7 @@
 #define __forceinline__ inline
 #define __noinline__ __attribute__((noinline))
 #define __launch_bounds__(...)
+#define __align__(n) __attribute__((aligned(n)))
 #define PS4_KEEP(...) ((void)0)
 #define PS4_MAX_U16X2(a, b)                                                                       \
     ((std::max((uint32_t)(a) & 0xFFFFu, (uint32_t)(b) & 0xFFFFu)) | (std::max((uint32_t)(a) >> 16, (uint32_t)(b) >> 16) << 16))
