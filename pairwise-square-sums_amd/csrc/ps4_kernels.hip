@@ -897,8 +897,11 @@ __device__ __forceinline__ u32 ext_exact(double u, const ps4_quad *quads, u64 qi
 }
 
 // Add p^e to a lane's factor list (slots [slot][lane] in s_fac), merging a
-// prime already present; `seen` is a 64-bit filter of the primes stored so far.
-__device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e, u32 &nf, u64 &seen, u32 &err)
+// prime already present; `seen` is a 64-bit filter of the primes stored so far,
+// `exps` the exponents of the slots, 5 bits each (an odd prime's exponent in
+// D = (ra - rb)(ra + rb) stays below 32 for roots below 2^25).
+__device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e, u32 &nf, u64 &seen, u64 &exps,
+                                              u32 &err)
 {
     const u64 bit = 1ull << (p & 63u);
     u32 sl = nf;
@@ -907,8 +910,10 @@ __device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e,
             if ((s_fac[i * EX_NT + tid] >> 5) == p) sl = i;
     if (sl < nf) {
         s_fac[sl * EX_NT + tid] += e;
+        exps += (u64)e << (5u * sl);
     } else if (nf < EX_MAXF) {
         s_fac[nf * EX_NT + tid] = (p << 5) | e;
+        exps |= (u64)e << (5u * nf);
         ++nf;
         seen |= bit;
     } else {
@@ -1192,25 +1197,25 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     pf4 = (s2a * s2a == 2 * P.A) ? (u32)s2a : 0u;   // u >= 2: 0 never matches
                     // the walk of D = m1 * m2 through the table (below): both odd
                     // parts side by side (two independent load chains)
-                    u64 seen = 0;
+                    u64 seen = 0, exps = 0;
                     for (int guard = 0; m1 > 1 || m2 > 1; ++guard) {
                         if (m1 >= a.tab_n || m2 >= a.tab_n || guard >= 64) { err |= ERR_FACTORS; break; }   // never for a valid 4-set
                         // two prime powers of each part per step
                         if (m1 > 1) {
-                            ex_add_factor(s_fac, tid, (u32)(en1.f1 >> 32) & 0x7FFFFFFu, (u32)(en1.f1 >> 59), nf, seen, err);
+                            ex_add_factor(s_fac, tid, (u32)(en1.f1 >> 32) & 0x7FFFFFFu, (u32)(en1.f1 >> 59), nf, seen, exps, err);
                             m1 = en1.f1 & 0xFFFFFFFFull;
                             if (m1 > 1) {
                                 ex_add_factor(s_fac, tid, (u32)(en1.f2 >> 32) & 0x7FFFFFFu, (u32)(en1.f2 >> 59), nf, seen,
-                                              err);
+                                              exps, err);
                                 m1 = en1.f2 & 0xFFFFFFFFull;
                             }
                         }
                         if (m2 > 1) {
-                            ex_add_factor(s_fac, tid, (u32)(en2.f1 >> 32) & 0x7FFFFFFu, (u32)(en2.f1 >> 59), nf, seen, err);
+                            ex_add_factor(s_fac, tid, (u32)(en2.f1 >> 32) & 0x7FFFFFFu, (u32)(en2.f1 >> 59), nf, seen, exps, err);
                             m2 = en2.f1 & 0xFFFFFFFFull;
                             if (m2 > 1) {
                                 ex_add_factor(s_fac, tid, (u32)(en2.f2 >> 32) & 0x7FFFFFFu, (u32)(en2.f2 >> 59), nf, seen,
-                                              err);
+                                              exps, err);
                                 m2 = en2.f2 & 0xFFFFFFFFull;
                             }
                         }
@@ -1222,8 +1227,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     pm65 = t65a & t65b;
                     pmsm = (t65c & t65d & 1ull) | ((u64)(t11a & t11b) << 1) | ((u64)(t13a & t13b) << 12) |
                            ((u64)(t17a & t17b) << 25) | ((u64)(t19a & t19b) << 42);
-                    u64 ndodd = 1;
-                    for (u32 i = 0; i < nf; ++i) ndodd *= (s_fac[i * EX_NT + tid] & 31u) + 1;
+                    u32 ndodd = 1;   // d of the odd part (< 2^32 for D < 2^51)
+                    for (u32 i = 0; i < nf; ++i) ndodd *= (u32)(exps >> (5u * i)) % 32u + 1u;
                     const u64 ndD = (u64)(e2 + 1) * ndodd;
                     my_maxdiv = ndD;
                     if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
@@ -1245,8 +1250,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                             const float fndp = (float)ndp;
                             u32 nl = 1, bestc = 1 + ndp, bestnl = 1;
                             for (u32 s = 1; s <= nfa; ++s) {
-                                const u32 ee = e2p ? (s == 1 ? e2p : (s_fac[(s - 2) * EX_NT + tid] & 31u))
-                                                   : (s_fac[(s - 1) * EX_NT + tid] & 31u);
+                                const u32 ee = e2p ? (s == 1 ? e2p : (u32)(exps >> (5u * (s - 2u))) % 32u)
+                                                   : (u32)(exps >> (5u * (s - 1u))) % 32u;
                                 nl *= ee + 1;
                                 const u32 nh = (u32)__builtin_rintf(fndp * __builtin_amdgcn_rcpf((float)nl));
                                 const u32 c = nl + nh;
@@ -1307,6 +1312,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     double *lst = s_ar + loff;
                     u32 n = 1;
                     lst[0] = 1.0;
+                    const u32 o2 = e2p ? 1u : 0u;              // stage of the first odd prime
+                    u32 wn = nf > 0 ? s_fac[tid] : 0u;         // its factor word (read ahead)
                     for (u32 s = 0; s < nfa; ++s) {
                         if (s == sL) {   // start the H list
                             lst = s_ar + loff + nL;
@@ -1314,8 +1321,15 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                             lst[0] = h0;
                         }
                         u32 p, e;
-                        if (e2p && s == 0) { p = 2; e = e2p; }
-                        else { const u32 w = s_fac[(s - (e2p ? 1u : 0u)) * EX_NT + tid]; p = w >> 5; e = w & 31u; }
+                        if (e2p && s == 0) {
+                            p = 2;
+                            e = e2p;
+                        } else {   // this stage's factor word, and the next one's read now
+                            p = wn >> 5;
+                            e = wn & 31u;
+                            const u32 k = s + 1u - o2;
+                            if (k < nf) wn = s_fac[k * EX_NT + tid];
+                        }
                         // entry (ee, j) = lst[j] * p^ee at ee * cur + j: each entry of the
                         // list so far is read once and its e multiples are written, so
                         // the only read-after-write dependence is between stages
