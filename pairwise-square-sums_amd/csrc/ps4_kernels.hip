@@ -839,11 +839,12 @@ __device__ __forceinline__ u32 rep_m1(double x)
 // roots of pair sums containing A, and the pair roots ra > rb with
 // D = v_i - v_j = (ra - rb)(ra + rb).  Members a > b > c > d (index 0..3).
 struct ExPair {
-    u64 A, C1, C2, F1, F2, F3, ra, rb;
+    u64 A, C1, C2;
+    u32 F1, F2, F3, ra, rb;   // roots (below 2^26)
 };
 
-__device__ __forceinline__ ExPair ex_pair(u32 best, u64 va, u64 vb, u64 vc, u64 vd, u64 rab, u64 rac, u64 rad,
-                                          u64 rbc, u64 rbd, u64 rcd)
+__device__ __forceinline__ ExPair ex_pair(u32 best, u64 va, u64 vb, u64 vc, u64 vd, u32 rab, u32 rac, u32 rad,
+                                          u32 rbc, u32 rbd, u32 rcd)
 {
     // as selects (the lanes of a wave choose different pairs):
     //   best 0: A=va C1=vc C2=vd F=(rab,rac,rad) ra=rac rb=rbc   (i,j,k) = (0,1,2)
@@ -876,10 +877,10 @@ __device__ __forceinline__ u32 ext_exact(double u, const ps4_quad *quads, u64 qi
     // the drain (a pending load would make the candidate loop wait for all
     // outstanding memory operations, its queue stores included, every iteration)
     PS4_KEEP("v"(qd.r_ab), "v"(qd.r_ac), "v"(qd.r_ad), "v"(qd.r_bc), "v"(qd.r_bd), "v"(qd.r_cd), "v"(qd.rid));
-    const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
+    const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc;
     const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
     const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
-    const ExPair P = ex_pair(best, va, vb, vc, vd, rab, rac, rad, rbc, rbd, rcd);
+    const ExPair P = ex_pair(best, va, vb, vc, vd, qd.r_ab, qd.r_ac, qd.r_ad, qd.r_bc, qd.r_bd, qd.r_cd);
     const u64 uu = (u64)u;
     const u128 e = (u128)uu * uu - P.A;
     if (!d_is_square128(e + P.C1) || !d_is_square128(e + P.C2)) return 0;
@@ -1138,13 +1139,15 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // D = v_i - v_j, first minimum in the scan order of
                     // exhaustive/s4.c:163-171 (members are a > b > c > d, so the
                     // reference's swap never fires).
-                    const u64 ra_[6] = {rac, rac, rab, rab, rab, rab};
-                    const u64 rb_[6] = {rbc, rad, rbc, rad, rbd, rac};
+                    // (the roots and the table indices are 32-bit: rab + rac < tab_n < 2^27)
+                    const u32 ra_[6] = {qd.r_ac, qd.r_ac, qd.r_ab, qd.r_ab, qd.r_ab, qd.r_ab};
+                    const u32 rb_[6] = {qd.r_bc, qd.r_ad, qd.r_bc, qd.r_ad, qd.r_bd, qd.r_ac};
                     u32 nd_[12];
 #pragma unroll
                     for (int t = 0; t < 6; ++t) {
-                        nd_[2 * t] = a.ndiv[ra_[t] - rb_[t]];
-                        nd_[2 * t + 1] = a.ndiv[ra_[t] + rb_[t]];
+                        // (32-bit byte offsets: the loads take the table base from scalar registers)
+                        nd_[2 * t] = *(const uint16_t *)((const char *)a.ndiv + 2u * (ra_[t] - rb_[t]));
+                        nd_[2 * t + 1] = *(const uint16_t *)((const char *)a.ndiv + 2u * (ra_[t] + rb_[t]));
                     }
                     u32 bestv = 0xFFFFFFFFu;
 #pragma unroll
@@ -1152,15 +1155,17 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                         const u32 e = nd_[2 * t] * nd_[2 * t + 1];
                         if (e < bestv) { bestv = e; best = (u32)t; }
                     }
-                    const ExPair P = ex_pair(best, va, vb, vc, vd, rab, rac, rad, rbc, rbd, rcd);
+                    const ExPair P = ex_pair(best, va, vb, vc, vd, qd.r_ab, qd.r_ac, qd.r_ad, qd.r_bc, qd.r_bd, qd.r_cd);
                     // D = m1 * m2: strip the twos; the first table entries of both odd
                     // parts are read now, ahead of the residue tables
-                    u64 m1 = P.ra - P.rb, m2 = P.ra + P.rb;
-                    const u32 z1 = (u32)__builtin_ctzll(m1), z2 = (u32)__builtin_ctzll(m2);
+                    u32 m1 = P.ra - P.rb, m2 = P.ra + P.rb;
+                    const u32 z1 = (u32)__builtin_ctz(m1), z2 = (u32)__builtin_ctz(m2);
                     const u32 e2 = z1 + z2;
                     m1 >>= z1;
                     m2 >>= z2;
-                    auto fac_at = [&](u64 m) -> ExFac { return m > 1 && m < a.tab_n ? a.fpair[m] : ExFac{0ull, 0ull}; };
+                    auto fac_at = [&](u32 m) -> ExFac {
+                        return m > 1 && m < a.tab_n ? *(const ExFac *)((const char *)a.fpair + 16u * m) : ExFac{0ull, 0ull};
+                    };
                     ExFac en1 = fac_at(m1), en2 = fac_at(m2);
                     // quadratic-residue masks of this 4-set (read before the factor walk,
                     // so that the reads overlap it): bit r of M_m set iff
@@ -1191,9 +1196,9 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // (every root is below 2^26 for S < 2^50)
                     const u64 sa = d_isqrt52(P.A), s2a = d_isqrt52(2 * P.A);
                     psqA = (u32)sa;
-                    pf1 = (u32)P.F1;
-                    pf2 = (u32)P.F2;
-                    pf3 = (u32)P.F3;
+                    pf1 = P.F1;
+                    pf2 = P.F2;
+                    pf3 = P.F3;
                     pf4 = (s2a * s2a == 2 * P.A) ? (u32)s2a : 0u;   // u >= 2: 0 never matches
                     // the walk of D = m1 * m2 through the table (below): both odd
                     // parts side by side (two independent load chains)
@@ -1203,20 +1208,20 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                         // two prime powers of each part per step
                         if (m1 > 1) {
                             ex_add_factor(s_fac, tid, (u32)(en1.f1 >> 32) & 0x7FFFFFFu, (u32)(en1.f1 >> 59), nf, seen, exps, err);
-                            m1 = en1.f1 & 0xFFFFFFFFull;
+                            m1 = (u32)en1.f1;
                             if (m1 > 1) {
                                 ex_add_factor(s_fac, tid, (u32)(en1.f2 >> 32) & 0x7FFFFFFu, (u32)(en1.f2 >> 59), nf, seen,
                                               exps, err);
-                                m1 = en1.f2 & 0xFFFFFFFFull;
+                                m1 = (u32)en1.f2;
                             }
                         }
                         if (m2 > 1) {
                             ex_add_factor(s_fac, tid, (u32)(en2.f1 >> 32) & 0x7FFFFFFu, (u32)(en2.f1 >> 59), nf, seen, exps, err);
-                            m2 = en2.f1 & 0xFFFFFFFFull;
+                            m2 = (u32)en2.f1;
                             if (m2 > 1) {
                                 ex_add_factor(s_fac, tid, (u32)(en2.f2 >> 32) & 0x7FFFFFFu, (u32)(en2.f2 >> 59), nf, seen,
                                               exps, err);
-                                m2 = en2.f2 & 0xFFFFFFFFull;
+                                m2 = (u32)en2.f2;
                             }
                         }
                         en1 = fac_at(m1);
