@@ -3,13 +3,13 @@
 //
 // k_rq   one workgroup per superwindow of S values (persistent, work queue):
 //        walks every lattice column p once, bins the points (p,q), p<=q,
-//        p^2+q^2 in the superwindow into 16K-wide windows in a per-workgroup
+//        p^2+q^2 in the superwindow into 4096-wide windows in a per-workgroup
 //        global scratch, then per window builds the representation buckets in
-//        LDS (count, scan, fill, sort -- process_chunk passes 1-3,
+//        LDS (count, scan, fill -- process_chunk passes 1-3,
 //        exhaustive/s4.c:285-310) and enumerates the representation triples of
 //        every S with >= 3 representations, S != 2 mod 4, into 4-sets (pass 4,
-//        :311-341).  Triples are spread evenly over the threads; 4-sets go to a
-//        global queue in a count-then-write pair of sweeps (one atomic per window).
+//        :311-341).  Triples are spread evenly over the threads; each wave
+//        appends its 4-sets to its own 64-entry tiles of a global queue.
 // k_ext  one single-wave workgroup per tile of 64 queued 4-sets (persistent,
 //        work queue): per lane the pair choice (:163-171), the factorisation of
 //        D = (ra-rb)(ra+rb) by a resident smallest-prime table, the closed-form
@@ -231,10 +231,11 @@ __device__ __forceinline__ u32 tri3(u32 k) { return __umulhi((k * (k - 1) >> 1) 
 // ----------------------------------------------------------------------------
 
 #define RQ_NT 256
-#define RQ_LOGW 13
+#define RQ_LOGW 12          // windows of 4096 S: 29 KB of LDS, 5 workgroups (20 waves) per CU
 #define RQ_W (1u << RQ_LOGW)
-#define RQ_PV_CAP 2560
-#define RQ_QS_CAP 1024
+#define RQ_PV_CAP 1536      // representations of the counted S of a window (about 800 at
+                            // S ~ 10^15, at most about 900 in samples; one S holds <= 512)
+#define RQ_QS_CAP 512       // counted S of a window (about 140, at most about 160 in samples)
 #define RQ_NB_MAX 1024
 #define RQ_K_MAX 2000u      // largest bucket whose triples a window indexes (C(2000,3) < 2^31)
 #define RQ_UNROLL 4         // window points a thread loads at once in the bucket passes
@@ -513,8 +514,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                         const u32 bin = (u32)(o >> RQ_LOGW);
                         if (bin >= nb) break;     // o < len implies bin < nb
                         const u32 slot = atomicAdd(&s_bin[bin], 1u);
-                        // a point: offset in its window | p << 13 | q << 38 (p < 2^25 and
-                        // q < 2^26 for S < 2^50)
+                        // a point: offset in its window | p << RQ_LOGW | q << 38 (p < 2^25
+                        // and q < 2^26 for S < 2^50)
                         if (slot < a.bin_cap)
                             scr[(u64)bin * a.bin_cap + slot] = (qq << 38) | (p << RQ_LOGW) | (o & (RQ_W - 1));
                         o += 2 * qq + 1;
@@ -558,12 +559,12 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             }
             __syncthreads();
             // pass 2: which S count, offsets (exhaustive/s4.c:300-305, 314-321).
-            // Thread t owns the 32 S from s0 = 32 t: 16 count words, read as four
-            // 16-byte words.  Counted S (k >= 3, S != 2 mod 4) are a few per cent,
+            // Thread t owns the SPT = W / 256 S from s0 = SPT t: SPT / 2 count
+            // words, read as 16-byte words.  Counted S (k >= 3, S != 2 mod 4) are a few per cent,
             // so one sweep sums and maxes the counts two at a time and marks the
             // counted S in a 32-bit mask; only the marked S are visited after.
             constexpr u32 SPT = RQ_W / RQ_NT;
-            static_assert(SPT == 32, "one 32-bit mask of counted S per thread");
+            static_assert(SPT % 8 == 0 && SPT <= 32, "whole 16-byte words, one 32-bit mask of counted S per thread");
             const u32 s0 = tid * SPT;
             const u32 b0 = (u32)B0;   // S mod 4 = (b0 + s) mod 4, and s0 = 0 mod 4
             // bit 15 (low half) / 31 (high half) set where that S may count: word
@@ -613,7 +614,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             c_sge3 += nq;
             c_trip += tr;
             // two scans: counted S and their representations packed in one word
-            // (each sum stays below 2^16: at most 8192 S and bin_cap <= 32768
+            // (each sum stays below 2^16: at most RQ_W S and bin_cap <= 4 RQ_W
             // points per window), and the triples with bit 31 set by a bucket too
             // large for the 32-bit triple index (without one, the sum of C(k,3)
             // over <= 2560 representations stays below 2^31)

@@ -770,7 +770,7 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     ctx->debug_sync = env_u64("PS4_DEBUG_SYNC", 0) != 0;
     // superwindows up to 2*sqrt(S_hi_max), at most 256 windows
     u32 nb = 1;
-    while ((u64)ctx->W * nb < 2 * isqrt64(ctx->S_hi_max) + 1 && nb < 256) nb <<= 1;
+    while ((u64)ctx->W * nb < 2 * isqrt64(ctx->S_hi_max) + 1 && nb < 512) nb <<= 1;
     ctx->nb_alloc = std::min<u32>(nb, (u32)ps4_rq_nb_max());
     ctx->bin_cap = ctx->W / 2 + 1024;
     // 4-set queue: enough for the whole range when it is small, else 2^30 entries

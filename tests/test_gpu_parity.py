@@ -251,6 +251,23 @@ def test_cli_drop_in(tmp_path, golden):
     assert sha_sorted(lines) == rec["sorted_sha256"]
     ck = (tmp_path / "c.ckpt").read_text().splitlines()
     assert ck[0] == "s4ckpt v1" and ck[-1] == "DONE"
+    # the audit chain's checks of a completed block (exhaustive/compact.py:29-45, 106-110):
+    # checkpoint parameters, one 13-field row per thread covering its whole region, the
+    # rows' counters summing to TOTAL, and the thread lines agreeing with the rows
+    lo, hi, nt = 1, 10**7, 4
+    assert tuple(map(int, ck[1].split())) == (lo, hi, nt, 1 << 20)
+    assert len(ck) == nt + 3
+    ckrows = [list(map(int, line.split())) for line in ck[2:-1]]
+    for i, row in enumerate(ckrows):
+        start, end = lo + (hi - lo) * i // nt, lo + (hi - lo) * (i + 1) // nt
+        assert len(row) == 13 and row[:3] == [i, end, end] and row[3] == end - start and min(row) >= 0
+    order = ("S_done", "pairs", "S_ge3", "triples", "4-sets", "5-sets", "6-sets", "multi-ext-4sets", "ext_cand",
+             "ext_sqrt")
+    for j, key in enumerate(order):
+        assert sum(row[3 + j] for row in ckrows) == kv[key], key
+    threads = re.findall(r"thread\s+(\d+): S \[(\d+),(\d+)\) 4-sets (\d+) 5-sets (\d+)", err)
+    assert threads == [(str(i), str(lo + (hi - lo) * i // nt), str(row[2]), str(row[7]), str(row[8]))
+                       for i, row in enumerate(ckrows)]
     # a finished checkpoint: --resume exits at once
     r2 = _run_cli([1, 10**7, 4, 20, "--out", out, "--ckpt", tmp_path / "c.ckpt", "--resume"], tmp_path)
     assert r2.returncode == 0 and b"nothing to do" in r2.stderr
