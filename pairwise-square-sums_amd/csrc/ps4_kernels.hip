@@ -713,6 +713,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 #define EX_CHUNK 4        // tiles claimed per work-queue atomic
 #define EX_Q1 128         // LDS stack of candidates past the residue filters (a round of 64
                           // runs once it holds >= 64; each push adds at most 64)
+#define EX_WG_SCRATCH (EX_Q2 + EX_MAXF * EX_NT / 4)   // 16-byte words of global scratch per
+                          // workgroup: the exact-test queue, then the factor lists
 #define EX_Q2 128         // queue of candidates for the exact test, per workgroup in global
                           // memory, kept across passes and tiles (drained at >= 64 and at
                           // the end; a round of the first queue adds at most 64)
@@ -734,7 +736,8 @@ struct ExArgs {
     const ps4_qrtab *qr;
     u64 *ctr;
     ps4_hit *hits;
-    uint4 *exq;             // EX_Q2 entries per workgroup: {u (52 bits) | pair choice << 52, 4-set index}
+    uint4 *exq;             // EX_WG_SCRATCH words per workgroup: EX_Q2 entries {u (52 bits) |
+                            // pair choice << 52, 4-set index}, then EX_MAXF * EX_NT factor words
     u32 hit_cap;
     u32 cap_divs;
     u32 n_rids;             // counter rows in ctr
@@ -898,23 +901,23 @@ __device__ __forceinline__ u32 ext_exact(double u, const ps4_quad *quads, u64 qi
     return 0;
 }
 
-// Add p^e to a lane's factor list (slots [slot][lane] in s_fac), merging a
+// Add p^e to a lane's factor list (slots [slot][lane] in fac), merging a
 // prime already present; `seen` is a 64-bit filter of the primes stored so far,
 // `exps` the exponents of the slots, 5 bits each (an odd prime's exponent in
 // D = (ra - rb)(ra + rb) stays below 32 for roots below 2^25).
-__device__ __forceinline__ void ex_add_factor(u32 *s_fac, u32 tid, u32 p, u32 e, u32 &nf, u64 &seen, u64 &exps,
+__device__ __forceinline__ void ex_add_factor(u32 *fac, u32 tid, u32 p, u32 e, u32 &nf, u64 &seen, u64 &exps,
                                               u32 &err)
 {
     const u64 bit = 1ull << (p & 63u);
     u32 sl = nf;
     if (seen & bit)
         for (u32 i = 0; i < nf; ++i)
-            if ((s_fac[i * EX_NT + tid] >> 5) == p) sl = i;
+            if ((fac[i * EX_NT + tid] >> 5) == p) sl = i;
     if (sl < nf) {
-        s_fac[sl * EX_NT + tid] += e;
+        fac[sl * EX_NT + tid] += e;
         exps += (u64)e << (5u * sl);
     } else if (nf < EX_MAXF) {
-        s_fac[nf * EX_NT + tid] = (p << 5) | e;
+        fac[nf * EX_NT + tid] = (p << 5) | e;
         exps |= (u64)e << (5u * nf);
         ++nf;
         seen |= bit;
@@ -1055,7 +1058,6 @@ __device__ __forceinline__ void ex_flush(u64 *ctr, u32 n_rids, u32 rid, u64 cand
 __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIMD: <= 168 VGPRs
 {
     __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
-    __shared__ u32 s_fac[EX_MAXF * EX_NT];     // odd prime factors of D: p << 5 | e, [slot][lane]
     __shared__ uint4 s_par[EX_NT * EX_PAR_WORDS];   // parameters of the 4-sets of the current pass
     __shared__ uint4 s_mem[EX_NT];             // their e > 0 and member bounds
     __shared__ u32 s_aux[EX_NT];               // isqrt(2A) or 0, | lane << 26
@@ -1070,7 +1072,11 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
     const u64 ntiles = (nquads + EX_NT - 1) / EX_NT;
     const u64 nchunks = (ntiles + EX_CHUNK - 1) / EX_CHUNK;
     const ps4_qrtab *qr = a.qr;
-    uint4 *exq2 = a.exq + (u64)blockIdx.x * EX_Q2;   // candidates for the exact test
+    uint4 *exq2 = a.exq + (u64)blockIdx.x * EX_WG_SCRATCH;   // candidates for the exact test
+    // odd prime factors of D, p << 5 | e, [slot][lane]: in the workgroup's global
+    // scratch (L2-resident; each lane reads only what it wrote), so that the LDS
+    // stays at 13 KB and 12 waves fit a CU
+    u32 *fac = (u32 *)(exq2 + EX_Q2);
     u32 qn2 = 0;                                     // entries in it
 
     // this wave's counters of interval acc_rid (lane partial sums)
@@ -1208,19 +1214,19 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                         if (m1 >= a.tab_n || m2 >= a.tab_n || guard >= 64) { err |= ERR_FACTORS; break; }   // never for a valid 4-set
                         // two prime powers of each part per step
                         if (m1 > 1) {
-                            ex_add_factor(s_fac, tid, (u32)(en1.f1 >> 32) & 0x7FFFFFFu, (u32)(en1.f1 >> 59), nf, seen, exps, err);
+                            ex_add_factor(fac, tid, (u32)(en1.f1 >> 32) & 0x7FFFFFFu, (u32)(en1.f1 >> 59), nf, seen, exps, err);
                             m1 = (u32)en1.f1;
                             if (m1 > 1) {
-                                ex_add_factor(s_fac, tid, (u32)(en1.f2 >> 32) & 0x7FFFFFFu, (u32)(en1.f2 >> 59), nf, seen,
+                                ex_add_factor(fac, tid, (u32)(en1.f2 >> 32) & 0x7FFFFFFu, (u32)(en1.f2 >> 59), nf, seen,
                                               exps, err);
                                 m1 = (u32)en1.f2;
                             }
                         }
                         if (m2 > 1) {
-                            ex_add_factor(s_fac, tid, (u32)(en2.f1 >> 32) & 0x7FFFFFFu, (u32)(en2.f1 >> 59), nf, seen, exps, err);
+                            ex_add_factor(fac, tid, (u32)(en2.f1 >> 32) & 0x7FFFFFFu, (u32)(en2.f1 >> 59), nf, seen, exps, err);
                             m2 = (u32)en2.f1;
                             if (m2 > 1) {
-                                ex_add_factor(s_fac, tid, (u32)(en2.f2 >> 32) & 0x7FFFFFFu, (u32)(en2.f2 >> 59), nf, seen,
+                                ex_add_factor(fac, tid, (u32)(en2.f2 >> 32) & 0x7FFFFFFu, (u32)(en2.f2 >> 59), nf, seen,
                                               exps, err);
                                 m2 = (u32)en2.f2;
                             }
@@ -1319,7 +1325,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     u32 n = 1;
                     lst[0] = 1.0;
                     const u32 o2 = e2p ? 1u : 0u;              // stage of the first odd prime
-                    u32 wn = nf > 0 ? s_fac[tid] : 0u;         // its factor word (read ahead)
+                    u32 wn = nf > 0 ? fac[tid] : 0u;         // its factor word (read ahead)
                     for (u32 s = 0; s < nfa; ++s) {
                         if (s == sL) {   // start the H list
                             lst = s_ar + loff + nL;
@@ -1334,7 +1340,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                             p = wn >> 5;
                             e = wn & 31u;
                             const u32 k = s + 1u - o2;
-                            if (k < nf) wn = s_fac[k * EX_NT + tid];
+                            if (k < nf) wn = fac[k * EX_NT + tid];
                         }
                         // entry (ee, j) = lst[j] * p^ee at ee * cur + j: each entry of the
                         // list so far is read once and its e multiples are written, so
@@ -1550,7 +1556,7 @@ extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u
 
 extern "C" int ps4_rq_threads(void) { return RQ_NT; }
 extern "C" int ps4_ext_threads(void) { return EX_NT; }
-extern "C" int ps4_ext_queue_cap(void) { return 2 * EX_Q2; }   // u64 words per workgroup
+extern "C" int ps4_ext_queue_cap(void) { return 2 * EX_WG_SCRATCH; }   // u64 words per workgroup
 extern "C" int ps4_rq_window(void) { return (int)RQ_W; }
 extern "C" int ps4_rq_nb_max(void) { return RQ_NB_MAX; }
 
