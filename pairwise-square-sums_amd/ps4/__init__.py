@@ -127,15 +127,17 @@ class Engine:
 
     def search(self, lo, hi):
         """Census of S in [lo, hi): (counter dict, list of result lines)."""
-        chunks = []
+        lines = []
 
+        # each call holds whole lines; the library makes a batch's lines while the
+        # next batch runs, so splitting them here overlaps the device work
         def sink(_user, text, n):
-            chunks.append(ctypes.string_at(text, n))
+            lines.extend(ctypes.string_at(text, n).decode().splitlines())
             return 0
         cb = SINK(sink)
         c = Counters()
         self._check(self.lib.ps4_search(self.ctx, int(lo), int(hi), cb, None, ctypes.byref(c)))
-        return c.as_dict(), b"".join(chunks).decode().splitlines()
+        return c.as_dict(), lines
 
     def search_batch(self, los, his):
         """Independent intervals in one persistent launch: list of (counters, lines)."""
@@ -145,13 +147,13 @@ class Engine:
         texts = [[] for _ in range(n)]
         users = (ctypes.c_void_p * n)(*range(1, n + 1))
 
-        def sink(user, text, nb):
-            texts[int(user) - 1].append(ctypes.string_at(text, nb))
+        def sink(user, text, nb):   # whole lines per call (as in search)
+            texts[int(user) - 1].extend(ctypes.string_at(text, nb).decode().splitlines())
             return 0
         cb = SINK(sink)
         cs = (Counters * n)()
         self._check(self.lib.ps4_search_batch(self.ctx, n, lo, hi, cb, users, cs))
-        return [(cs[i].as_dict(), b"".join(texts[i]).decode().splitlines()) for i in range(n)]
+        return [(cs[i].as_dict(), texts[i]) for i in range(n)]
 
     def search_elements(self, n_max):
         """Element-range search: (sorted "[a,b,c,d]" lines of every 4-set with
