@@ -1061,7 +1061,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
     __shared__ uint4 s_par[EX_NT * EX_PAR_WORDS];   // parameters of the 4-sets of the current pass
     __shared__ uint4 s_mem[EX_NT];             // their e > 0 and member bounds
     __shared__ u32 s_aux[EX_NT];               // isqrt(2A) or 0, | lane << 26
-    __shared__ uint4 s_bits4[EX_CMAX / 128];   // bit g: a 4-set's candidates start at g
+    __shared__ uint4 s_bits4[EX_CMAX / 128];   // bit g: a 4-set's candidates start at g + 1
     __shared__ u64 s_q1[EX_Q1];                // candidates past the residue filters: u | record << 52
     u32 *s_bits = (u32 *)s_bits4;
 
@@ -1313,7 +1313,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     pw[2] = make_uint4((u32)pm65, (u32)(pm65 >> 32), (u32)pmsm, (u32)(pmsm >> 32) | (best << 29));
                     s_mem[slot] = make_uint4(psqA, pf1, pf2, pf3);
                     s_aux[slot] = pf4 | (tid << 26);
-                    atomicOr(&s_bits[coff >> 5], 1u << (coff & 31u));
+                    if (coff > 0u) atomicOr(&s_bits[(coff - 1u) >> 5], 1u << ((coff - 1u) & 31u));
                     // mixed-radix lists over the factor sequence: stage s multiplies
                     // the list so far by p_s^1 .. p_s^e (earlier stages are the less
                     // significant digits, so index n and n_max - n are complements).
@@ -1372,31 +1372,33 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 // one after are read, and the next iteration's mask words once this
                 // iteration's tests have used theirs (LDS reads complete in order,
                 // so each wait covers only older reads).
-                u32 kp = 0;   // 4-set starts before the next fetched base
-                const u64 lane_mask = (2ull << tid) - 1ull;   // bits 0..tid
-                u32 qn1 = 0;                                  // entries in the first queue
+                u32 kp = 0;     // marks before the next fetched base
+                u32 qn1 = 0;    // entries in the first queue
                 // the records of candidates base + tid and base + 64 + tid from the
-                // bitmap words of base (the 4-set of candidate g starts at or before
-                // g, minus one); past the end of the pass a lane evaluates record 0
-                // at its first candidate (in range) and discards it
-                u32 ka, kb, ia, ib;       // ia, ib: the records read (ka, kb or 0)
+                // bitmap words of base: every 4-set but the first marks the candidate
+                // before its first, so the record of candidate g is the number of
+                // marks below g (a ballot-style prefix count, v_mbcnt); past the end
+                // of the pass a lane evaluates record 0 at its first candidate (in
+                // range) and discards it
+                u32 ka, kb, ia, ib;       // ia, ib: byte offsets of the records read (of ka, kb or 0)
                 uint4 A0, A1, A2, B0, B1, B2;   // their words
                 auto fetch = [&](u32 base, const uint4 &wv) {
                     const u64 wa = (u64)wv.x | ((u64)wv.y << 32), wb = (u64)wv.z | ((u64)wv.w << 32);
-                    ka = kp + (u32)__popcll(wa & lane_mask) - 1u;
+                    ka = kp + lanes_below(wa);
                     kp += (u32)__popcll(wa);
-                    kb = kp + (u32)__popcll(wb & lane_mask) - 1u;
+                    kb = kp + lanes_below(wb);
                     kp += (u32)__popcll(wb);
-                    ia = (base + tid < ctot ? ka : 0u) & 63u;   // (a record index: the mask only
-                    ib = (base + 64u + tid < ctot ? kb : 0u) & 63u;   //  tells the compiler its range)
-                    A0 = s_par[ia * EX_PAR_WORDS];
-                    B0 = s_par[ib * EX_PAR_WORDS];
+                    // byte offsets of the records (record indices are below 64)
+                    ia = PS4_MAD_U24(base + tid < ctot ? ka : 0u, EX_PAR_WORDS * 16u, 0u);
+                    ib = PS4_MAD_U24(base + 64u + tid < ctot ? kb : 0u, EX_PAR_WORDS * 16u, 0u);
+                    A0 = *(const uint4 *)((const char *)s_par + ia);
+                    B0 = *(const uint4 *)((const char *)s_par + ib);
                 };
                 auto fetch_masks = [&]() {
-                    A1 = s_par[ia * EX_PAR_WORDS + 1];
-                    A2 = s_par[ia * EX_PAR_WORDS + 2];
-                    B1 = s_par[ib * EX_PAR_WORDS + 1];
-                    B2 = s_par[ib * EX_PAR_WORDS + 2];
+                    A1 = *(const uint4 *)((const char *)s_par + ia + 16u);
+                    A2 = *(const uint4 *)((const char *)s_par + ia + 32u);
+                    B1 = *(const uint4 *)((const char *)s_par + ib + 16u);
+                    B2 = *(const uint4 *)((const char *)s_par + ib + 32u);
                 };
                 // (read unconditionally: beyond the pass the bitmap holds stale bits,
                 // which only choose records of candidates that are discarded)
