@@ -59,7 +59,7 @@ PEAK_LANE_OPS = 256 * 64 * 2.4e9      # 256 CUs x 64 lanes x 2.4 GHz, 32-bit int
 # HBM bytes per k_ext launch, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
 # [1,1e10) census (4 launches), each counter scaled by what it reads for a 2 GiB device copy
 # (profiles/census_1e10_counters.md); reported for that workload only
-K_EXT_TRAFFIC = {(1, 10**10): 22.88e9}
+K_EXT_TRAFFIC = {(1, 10**10): 36.18e9}
 GOLDEN = os.path.join(REPO, "tests", "golden", "golden.json")
 
 

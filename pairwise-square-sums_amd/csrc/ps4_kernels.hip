@@ -1055,6 +1055,12 @@ __device__ __forceinline__ void ex_flush(u64 *ctr, u32 n_rids, u32 rid, u64 cand
 // unordered pair f != g exactly once, and f + g needs no ordering.
 // Counters are summed per wave over consecutive tiles of one interval and
 // added to the device counters when the interval changes and at the end.
+// kLdsFac: the lanes' factor lists in LDS (16 KB per workgroup, 10 waves per
+// CU) rather than in the workgroup's global scratch (13 KB, 12 waves per CU).
+// The host takes the LDS form for batches with S above about 3*10^10, where
+// the larger factor tables make the scratch's L2 traffic cost more than the
+// two extra waves gain (-2.7 % time at S ~ 3*10^11, +1.7 % below 10^10).
+template <bool kLdsFac>
 __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIMD: <= 168 VGPRs
 {
     __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
@@ -1073,10 +1079,15 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
     const u64 nchunks = (ntiles + EX_CHUNK - 1) / EX_CHUNK;
     const ps4_qrtab *qr = a.qr;
     uint4 *exq2 = a.exq + (u64)blockIdx.x * EX_WG_SCRATCH;   // candidates for the exact test
-    // odd prime factors of D, p << 5 | e, [slot][lane]: in the workgroup's global
-    // scratch (L2-resident; each lane reads only what it wrote), so that the LDS
-    // stays at 13 KB and 12 waves fit a CU
-    u32 *fac = (u32 *)(exq2 + EX_Q2);
+    // odd prime factors of D, p << 5 | e, [slot][lane]: in LDS, or in the
+    // workgroup's global scratch (L2-resident; each lane reads only what it wrote)
+    u32 *fac;
+    if constexpr (kLdsFac) {
+        __shared__ u32 s_fac[EX_MAXF * EX_NT];
+        fac = s_fac;
+    } else {
+        fac = (u32 *)(exq2 + EX_Q2);
+    }
     u32 qn2 = 0;                                     // entries in it
 
     // this wave's counters of interval acc_rid (lane partial sums)
@@ -1525,7 +1536,7 @@ extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *it
 
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
                                      const u64 *fpair, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
-                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq)
+                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq, int lds_fac)
 {
     ExArgs a;
     a.n_rids = n_rids;
@@ -1533,7 +1544,10 @@ extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *q
     a.ctr = ctr;
     a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs; a.exq = (uint4 *)exq;
     a.neg_m1 = -M1;
-    hipLaunchKernelGGL(k_ext, dim3(grid), dim3(EX_NT), 0, st, a);
+    if (lds_fac)
+        hipLaunchKernelGGL(k_ext<true>, dim3(grid), dim3(EX_NT), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_ext<false>, dim3(grid), dim3(EX_NT), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1563,7 +1577,7 @@ extern "C" int ps4_rq_window(void) { return (int)RQ_W; }
 extern "C" int ps4_rq_nb_max(void) { return RQ_NB_MAX; }
 
 extern "C" void *ps4_k_rq_ptr(void) { return (void *)&k_rq; }
-extern "C" void *ps4_k_ext_ptr(void) { return (void *)&k_ext; }
+extern "C" void *ps4_k_ext_ptr(int lds_fac) { return lds_fac ? (void *)&k_ext<true> : (void *)&k_ext<false>; }
 
 // ----------------------------------------------------------------------------
 // k_triple: d with d+a, d+b, d+c square for given square triples {a,b,c}

@@ -31,7 +31,7 @@ extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *it
                                     u64 quad_cap, u32 cap_reps, u32 verify);
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
                                      const u64 *fpair, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
-                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq);
+                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq, int lds_fac);
 extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u32 n_ctr);
 extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u64 d_max, const u64 *ftab,
                                         u64 tab_n, u64 *d_out, u32 cap, u32 *count, u32 *err);
@@ -45,7 +45,7 @@ extern "C" int ps4_ext_queue_cap(void);
 extern "C" int ps4_rq_window(void);
 extern "C" int ps4_rq_nb_max(void);
 extern "C" void *ps4_k_rq_ptr(void);
-extern "C" void *ps4_k_ext_ptr(void);
+extern "C" void *ps4_k_ext_ptr(int lds_fac);
 
 namespace {
 
@@ -188,6 +188,7 @@ struct ps4_ctx {
     u32 cap_reps = 512, cap_divs = 16384, cap_ext = 64;
     u64 S_hi_max = 0, tab_n = 0;
     u32 n_cu = 0, rq_grid = 0, ext_grid = 0;
+    u32 ext_grid_lds = 0;      // workgroups of the extension kernel's LDS-factor form
     u32 W = 0, nb_alloc = 0, bin_cap = 0;
     u64 quad_cap = 0;
     u32 hit_cap = 0, items_cap = 0, ctr_cap = 0;
@@ -553,6 +554,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         const double quads_budget =
             0.8 * (double)(ctx->elem_max ? std::min<u64>(ctx->quad_cap, PS4_ELEM_CAP) : ctx->quad_cap) * budget_scale;
         u32 n_items = 0;
+        u64 s_top = 0;   // largest S of the batch
         u32 rid_lo = iv[next_iv].rid, rid_hi = rid_lo;
         // little work left (small queries): shorter superwindows, so that the
         // representation kernel still has about one per workgroup
@@ -578,6 +580,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
                     len = room;
                 }
                 ctx->h_items[n_items++] = {v.cur, v.cur + len, v.rid, 0};
+                s_top = std::max(s_top, v.cur + len);
                 quads_expected += quad_density(v.cur + len) * (double)len + item_pad;
                 v.cur += len;
                 rid_hi = v.rid;
@@ -610,10 +613,13 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
             HIP_OK(ps4_launch_elem(ctx->st, ctx->d_quads, ctx->d_ctl, ctx->quad_cap, ctx->elem_max, ctx->d_eout,
                                    PS4_ELEM_CAP, ctx->d_ecnt, ctx->n_cu * 8));
         }
+        // the extension kernel's factor lists: in LDS above S ~ 3*10^10 (larger
+        // factor tables), in global scratch with two more waves per CU below
+        const bool lds_fac = s_top > (1ull << 35);
         if (!noext)
-            HIP_OK(ps4_launch_ext(ctx->st, ctx->ext_grid, ctx->d_quads, ctx->quad_cap, ctx->d_ctl, ctx->d_fpair,
-                                  ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, rid_hi + 1, ctx->d_hits,
-                                  ctx->hit_cap, ctx->cap_divs, ctx->d_exq));
+            HIP_OK(ps4_launch_ext(ctx->st, lds_fac ? ctx->ext_grid_lds : ctx->ext_grid, ctx->d_quads, ctx->quad_cap,
+                                  ctx->d_ctl, ctx->d_fpair, ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, rid_hi + 1,
+                                  ctx->d_hits, ctx->hit_cap, ctx->cap_divs, ctx->d_exq, lds_fac ? 1 : 0));
         HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
         if (!noext)
             if (int rc = dbg("k_ext")) return rc;
@@ -761,11 +767,13 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     HIP_OK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
     for (hipEvent_t &e : ctx->ev) HIP_OK(hipEventCreate(&e));
     if (int rc = build_tables(ctx)) return rc;
-    int b_rq = 0, b_ext = 0;
+    int b_rq = 0, b_ext = 0, b_ext_lds = 0;
     HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b_rq, ps4_k_rq_ptr(), ps4_rq_threads(), 0));
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b_ext, ps4_k_ext_ptr(), ps4_ext_threads(), 0));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b_ext, ps4_k_ext_ptr(0), ps4_ext_threads(), 0));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b_ext_lds, ps4_k_ext_ptr(1), ps4_ext_threads(), 0));
     ctx->rq_grid = (u32)env_u64("PS4_RQ_GRID", (u64)ctx->n_cu * (u64)std::max(1, b_rq));
     ctx->ext_grid = (u32)env_u64("PS4_EXT_GRID", (u64)ctx->n_cu * (u64)std::max(1, b_ext));
+    ctx->ext_grid_lds = std::min(ctx->ext_grid, ctx->n_cu * (u32)std::max(1, b_ext_lds));   // (the scratch holds ext_grid)
     ctx->W = (u32)ps4_rq_window();
     ctx->debug_sync = env_u64("PS4_DEBUG_SYNC", 0) != 0;
     // superwindows up to 2*sqrt(S_hi_max), at most 256 windows
