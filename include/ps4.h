@@ -129,6 +129,38 @@ int ps4_search_elements(ps4_ctx *ctx, uint64_t n_max, ps4_sink sink, void *user,
  * exhaustive/s4.c:496). */
 int ps4_kernel_time(ps4_ctx *ctx, double *ms_rq, double *ms_ext, uint64_t *batches, int reset);
 
+/* Launch shape and batch plan of later searches: what the autotuner
+ * (pairwise-square-sums_amd/ps4/autotune.py) varies.  Results never depend on
+ * these, only the time does.  No reference counterpart: the shape knobs of s4
+ * are its thread count and log2 chunk (exhaustive/s4.c:424-426, 451-456). */
+typedef struct {
+    uint32_t rq_grid;        /* workgroups of the persistent representation kernel (k_rq)          */
+    uint32_t ext_grid;       /* single-wave workgroups of the extension kernel (k_ext), factor
+                                lists in global scratch                                             */
+    uint32_t ext_grid_lds;   /* the same for k_ext's form with the factor lists in LDS              */
+    uint32_t sw_shift;       /* subrange split: superwindows 2^-sw_shift of the default length (a
+                                power of two near 2 sqrt(S)), never shorter than one 4096-S window  */
+    uint32_t batch_pct;      /* a batch is planned to fill this percentage of the 4-set queue       */
+    uint32_t n_cu;           /* the device's compute units (reported by ps4_get_tuning; ignored by
+                                ps4_set_tuning)                                                     */
+    uint64_t lds_fac_above;  /* batches reaching above this S run k_ext's LDS-list form             */
+} ps4_tuning;
+
+/* The values in force (*cur) and the largest each field may take (*max; grids:
+ * the workgroups resident at once, which sized the scratch at ps4_open).
+ * Either pointer may be NULL. */
+int ps4_get_tuning(ps4_ctx *ctx, ps4_tuning *cur, ps4_tuning *max);
+
+/* Set every field; a field of 0 selects its default (resident maximum for the
+ * grids, 0 for sw_shift, 80 for batch_pct, 2^35 for lds_fac_above).  A value
+ * above its maximum is PS4_E_ARG and changes nothing. */
+int ps4_set_tuning(ps4_ctx *ctx, const ps4_tuning *t);
+
+/* Parse a tuning file: "key = value" lines, keys named as the ps4_tuning
+ * fields, '#' starts a comment; fields not named are 0.  PS4_E_ARG on an
+ * unknown key or a malformed value.  Needs no context and no device. */
+int ps4_parse_tuning(const char *text, ps4_tuning *t);
+
 /* Close a context and free its device memory. */
 void ps4_close(ps4_ctx *ctx);
 

@@ -4,7 +4,7 @@
  * include/ps4.h.
  *
  *   ps4 S_lo S_hi [threads] [log2 chunk] [--out FILE] [--ckpt FILE] [--resume]
- *                 [--noext] [--verify] [--dump4 N] [--test-sixset] [--device N]
+ *                 [--noext] [--verify] [--dump4 N] [--test-sixset] [--device N] [--tune FILE]
  *   ps4 --elements N [--out FILE] [--device N]
  *
  * --elements N (no s4 counterpart: the element-range form of the search) prints
@@ -12,6 +12,10 @@
  * lines in ascending order, and on stderr one line
  * "ELEMENTS: n_max=N sets=... square_triples=..." (square triples {a<b<c}
  * with c <= N).
+ *
+ * --tune FILE (no s4 counterpart) applies a tuning file written by the
+ * autotuner (ps4/autotune.py): launch shape and batch plan only, the output is
+ * the same with or without it.
  *
  * Same result lines on stdout (or appended to --out), same stderr reports
  * (header, progress, per-thread lines, TOTAL, 5-set histogram, CAPACITY), same
@@ -161,7 +165,24 @@ static int read_ckpt(ull S_lo, ull S_hi, int T, ull W, region_t *rg)
     return done ? 2 : 1;
 }
 
-static int run_elements(ull n_max, int device, const char *out)
+/* Apply a tuning file (ps4_parse_tuning grammar); 0 or the ps4 error code. */
+static int apply_tuning(ps4_ctx *ctx, const char *path)
+{
+    if (!path) return 0;
+    FILE *f = fopen(path, "r");
+    if (!f) { perror(path); return PS4_E_ARG; }
+    char buf[4096];
+    const size_t n = fread(buf, 1, sizeof buf - 1, f);
+    fclose(f);
+    buf[n] = 0;
+    ps4_tuning t;
+    int rc = ps4_parse_tuning(buf, &t);
+    if (!rc) rc = ps4_set_tuning(ctx, &t);
+    if (rc) fprintf(stderr, "ps4: tuning file %s: %s %s\n", path, ps4_strerror(rc), ps4_last_error(ctx));
+    return rc;
+}
+
+static int run_elements(ull n_max, int device, const char *out, const char *tune)
 {
     if (out) { out_fp = fopen(out, "a"); if (!out_fp) { perror(out); return 1; } }
     else out_fp = stdout;
@@ -172,6 +193,7 @@ static int run_elements(ull n_max, int device, const char *out)
     ps4_ctx *ctx = NULL;
     int rc = ps4_open(&cfg, &ctx);
     if (rc) { fprintf(stderr, "FATAL: %s: %s\n", ps4_strerror(rc), ctx ? ps4_last_error(ctx) : ""); return 2; }
+    if (apply_tuning(ctx, tune)) { ps4_close(ctx); return 1; }
     uint64_t ns = 0, nt = 0;
     rc = ps4_search_elements(ctx, n_max, sink_out, NULL, &ns, &nt);
     if (rc) {
@@ -192,7 +214,7 @@ int main(int argc, char **argv)
     int T = 0, pos = 0, device = 0, opt_noext = 0, opt_verify = 0, opt_resume = 0, opt_test6 = 0;
     long long opt_dump4 = 0;
     ull opt_elements = 0;
-    const char *opt_out = NULL;
+    const char *opt_out = NULL, *opt_tune = NULL;
     for (int i = 1; i < argc; i++) {
         if (!strcmp(argv[i], "--noext")) opt_noext = 1;
         else if (!strcmp(argv[i], "--verify")) opt_verify = 1;
@@ -202,6 +224,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "--out") && i + 1 < argc) opt_out = argv[++i];
         else if (!strcmp(argv[i], "--ckpt") && i + 1 < argc) opt_ckpt = argv[++i];
         else if (!strcmp(argv[i], "--device") && i + 1 < argc) device = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "--tune") && i + 1 < argc) opt_tune = argv[++i];
         else if (!strcmp(argv[i], "--elements") && i + 1 < argc) opt_elements = strtoull(argv[++i], 0, 10);
         else {
             pos++;
@@ -211,7 +234,7 @@ int main(int argc, char **argv)
             else if (pos == 4) W = 1ull << atoi(argv[i]);
         }
     }
-    if (opt_elements) return run_elements(opt_elements, device, opt_out);
+    if (opt_elements) return run_elements(opt_elements, device, opt_out, opt_tune);
     if (S_lo < 1) S_lo = 1;
     if (S_hi < S_lo) { fprintf(stderr, "S_hi < S_lo\n"); return 1; }
     if (T <= 0) {
@@ -257,6 +280,7 @@ int main(int argc, char **argv)
     ps4_ctx *ctx = NULL;
     int rc = ps4_open(&cfg, &ctx);
     if (rc) { fprintf(stderr, "ps4: cannot open device %d: %s\n", device, ps4_strerror(rc)); return 1; }
+    if (apply_tuning(ctx, opt_tune)) { ps4_close(ctx); return 1; }
     ps4_set_alert(ctx, alert_out, NULL);
 
     /* blocks: whole chunks, about 2^33 S, so positions stay on chunk starts */

@@ -26,6 +26,9 @@ typedef struct {
  * with the 4-sets of one superwindow; the slots a superwindow leaves unused
  * carry this rid (and zero roots). */
 #define PS4_RID_NONE 0xFFFFFFFFu
+// largest tuning values: superwindow shift, batch fill of the 4-set queue (%)
+#define PS4_SW_SHIFT_MAX 8u
+#define PS4_BATCH_PCT_MAX 95u
 // element-range mode: qualifying 4-sets one batch can return (the batch's
 // 4-set budget is held below it)
 #define PS4_ELEM_CAP (1u << 25)

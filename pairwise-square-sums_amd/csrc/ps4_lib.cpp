@@ -189,6 +189,12 @@ struct ps4_ctx {
     u64 S_hi_max = 0, tab_n = 0;
     u32 n_cu = 0, rq_grid = 0, ext_grid = 0;
     u32 ext_grid_lds = 0;      // workgroups of the extension kernel's LDS-factor form
+    // tuning: the grids above may be lowered to these resident maxima (which
+    // sized the scratch); superwindow shift, batch fill, k_ext form threshold
+    u32 rq_grid_max = 0, ext_grid_max = 0, ext_grid_lds_max = 0;
+    u32 sw_shift = 0;
+    double batch_frac = 0.8;
+    u64 lds_fac_above = 1ull << 35;
     u32 W = 0, nb_alloc = 0, bin_cap = 0;
     u64 quad_cap = 0;
     u32 hit_cap = 0, items_cap = 0, ctr_cap = 0;
@@ -340,7 +346,7 @@ u64 superwindow(const ps4_ctx *ctx, u64 S)
     u64 sw = ctx->W;
     const u64 swmax = (u64)ctx->W * ctx->nb_alloc;
     while (sw < target && sw < swmax) sw <<= 1;
-    return sw;
+    return std::max<u64>(sw >> ctx->sw_shift, ctx->W);
 }
 
 struct Interval {
@@ -552,7 +558,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         double quads_expected = 0.0;
         // element mode: a batch's 4-sets must also fit the buffer of qualifying ones
         const double quads_budget =
-            0.8 * (double)(ctx->elem_max ? std::min<u64>(ctx->quad_cap, PS4_ELEM_CAP) : ctx->quad_cap) * budget_scale;
+            ctx->batch_frac * (double)(ctx->elem_max ? std::min<u64>(ctx->quad_cap, PS4_ELEM_CAP) : ctx->quad_cap) * budget_scale;
         u32 n_items = 0;
         u64 s_top = 0;   // largest S of the batch
         u32 rid_lo = iv[next_iv].rid, rid_hi = rid_lo;
@@ -565,10 +571,20 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         // representation kernel ends an item with up to two partly used tiles
         const double item_pad = 128.0 * (double)(ps4_rq_threads() / 64);
         while (item_max * ctx->rq_grid < remaining && item_max < ((u64)1 << 40)) item_max <<= 1;
+        // at least two superwindows per representation workgroup in a batch: at
+        // high S a batch's 4-set budget covers fewer S, and superwindows of
+        // ~2 sqrt(S) would leave workgroups idle (at S ~ 9*10^11: 820 per batch
+        // for 1024 workgroups)
+        auto sw_at = [&](u64 S) {
+            u64 sw = superwindow(ctx, S);
+            const double span = quads_budget / quad_density(S);
+            while (sw > ctx->W && (double)sw * 2.0 * (double)ctx->rq_grid > span) sw >>= 1;
+            return sw;
+        };
         for (size_t i = next_iv; i < n && n_items < ctx->items_cap && quads_expected < quads_budget; ++i) {
             Interval &v = iv[i];
             while (v.cur < v.hi && n_items < ctx->items_cap && quads_expected < quads_budget) {
-                u64 len = std::min(superwindow(ctx, v.cur), item_max);
+                u64 len = std::min(sw_at(v.cur), item_max);
                 if (len > v.hi - v.cur) len = v.hi - v.cur;
                 const double dq = quad_density(v.cur + len) * (double)len + item_pad;
                 if (n_items > 0 && quads_expected + dq > quads_budget) {
@@ -615,7 +631,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         }
         // the extension kernel's factor lists: in LDS above S ~ 3*10^10 (larger
         // factor tables), in global scratch with two more waves per CU below
-        const bool lds_fac = s_top > (1ull << 35);
+        const bool lds_fac = s_top > ctx->lds_fac_above;
         if (!noext)
             HIP_OK(ps4_launch_ext(ctx->st, lds_fac ? ctx->ext_grid_lds : ctx->ext_grid, ctx->d_quads, ctx->quad_cap,
                                   ctx->d_ctl, ctx->d_fpair, ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, rid_hi + 1,
@@ -774,6 +790,9 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     ctx->rq_grid = (u32)env_u64("PS4_RQ_GRID", (u64)ctx->n_cu * (u64)std::max(1, b_rq));
     ctx->ext_grid = (u32)env_u64("PS4_EXT_GRID", (u64)ctx->n_cu * (u64)std::max(1, b_ext));
     ctx->ext_grid_lds = std::min(ctx->ext_grid, ctx->n_cu * (u32)std::max(1, b_ext_lds));   // (the scratch holds ext_grid)
+    ctx->rq_grid_max = ctx->rq_grid;
+    ctx->ext_grid_max = ctx->ext_grid;
+    ctx->ext_grid_lds_max = ctx->ext_grid_lds;
     ctx->W = (u32)ps4_rq_window();
     ctx->debug_sync = env_u64("PS4_DEBUG_SYNC", 0) != 0;
     // superwindows up to 2*sqrt(S_hi_max), at most 256 windows
@@ -818,6 +837,83 @@ extern "C" int ps4_open(const ps4_config *cfg, ps4_ctx **ctx_out)
         return rc;
     }
     *ctx_out = ctx;
+    return 0;
+}
+
+extern "C" int ps4_get_tuning(ps4_ctx *ctx, ps4_tuning *cur, ps4_tuning *max)
+{
+    if (!ctx) return PS4_E_ARG;
+    if (cur) {
+        memset(cur, 0, sizeof *cur);
+        cur->rq_grid = ctx->rq_grid;
+        cur->ext_grid = ctx->ext_grid;
+        cur->ext_grid_lds = ctx->ext_grid_lds;
+        cur->sw_shift = ctx->sw_shift;
+        cur->batch_pct = (u32)std::lround(ctx->batch_frac * 100.0);
+        cur->lds_fac_above = ctx->lds_fac_above;
+        cur->n_cu = ctx->n_cu;
+    }
+    if (max) {
+        memset(max, 0, sizeof *max);
+        max->rq_grid = ctx->rq_grid_max;
+        max->ext_grid = ctx->ext_grid_max;
+        max->ext_grid_lds = ctx->ext_grid_lds_max;
+        max->sw_shift = PS4_SW_SHIFT_MAX;
+        max->batch_pct = PS4_BATCH_PCT_MAX;
+        max->lds_fac_above = ~0ull;
+        max->n_cu = ctx->n_cu;
+    }
+    return 0;
+}
+
+extern "C" int ps4_set_tuning(ps4_ctx *ctx, const ps4_tuning *t)
+{
+    if (!ctx || !t) return PS4_E_ARG;
+    if (t->rq_grid > ctx->rq_grid_max || t->ext_grid > ctx->ext_grid_max || t->ext_grid_lds > ctx->ext_grid_lds_max ||
+        t->sw_shift > PS4_SW_SHIFT_MAX || t->batch_pct > PS4_BATCH_PCT_MAX) {
+        ctx->last_error = "tuning value above its maximum (ps4_get_tuning)";
+        return PS4_E_ARG;
+    }
+    ctx->rq_grid = t->rq_grid ? t->rq_grid : ctx->rq_grid_max;
+    ctx->ext_grid = t->ext_grid ? t->ext_grid : ctx->ext_grid_max;
+    ctx->ext_grid_lds = t->ext_grid_lds ? t->ext_grid_lds : ctx->ext_grid_lds_max;
+    ctx->sw_shift = t->sw_shift;
+    ctx->batch_frac = (t->batch_pct ? t->batch_pct : 80u) / 100.0;
+    ctx->lds_fac_above = t->lds_fac_above ? t->lds_fac_above : (1ull << 35);
+    return 0;
+}
+
+extern "C" int ps4_parse_tuning(const char *text, ps4_tuning *t)
+{
+    if (!text || !t) return PS4_E_ARG;
+    memset(t, 0, sizeof *t);
+    const char *p = text;
+    while (*p) {
+        const char *eol = strchr(p, '\n');
+        std::string line(p, eol ? (size_t)(eol - p) : strlen(p));
+        p = eol ? eol + 1 : p + line.size();
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        auto trim = [](std::string x) {
+            const size_t a = x.find_first_not_of(" \t\r"), b = x.find_last_not_of(" \t\r");
+            return a == std::string::npos ? std::string() : x.substr(a, b - a + 1);
+        };
+        line = trim(line);
+        if (line.empty()) continue;
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos) return PS4_E_ARG;
+        const std::string key = trim(line.substr(0, eq)), val = trim(line.substr(eq + 1));
+        if (val.empty() || val.find_first_not_of("0123456789") != std::string::npos || val.size() > 19) return PS4_E_ARG;
+        const u64 v = strtoull(val.c_str(), nullptr, 10);
+        if (key == "lds_fac_above") { t->lds_fac_above = v; continue; }
+        if (v > 0xFFFFFFFFull) return PS4_E_ARG;
+        if (key == "rq_grid") t->rq_grid = (u32)v;
+        else if (key == "ext_grid") t->ext_grid = (u32)v;
+        else if (key == "ext_grid_lds") t->ext_grid_lds = (u32)v;
+        else if (key == "sw_shift") t->sw_shift = (u32)v;
+        else if (key == "batch_pct") t->batch_pct = (u32)v;
+        else return PS4_E_ARG;
+    }
     return 0;
 }
 

@@ -48,6 +48,44 @@ class Config(ctypes.Structure):
                 ("cap_ext", ctypes.c_uint32)]
 
 
+class Tuning(ctypes.Structure):
+    """Launch shape and batch plan (ps4_tuning): results never depend on it."""
+    _fields_ = [("rq_grid", ctypes.c_uint32), ("ext_grid", ctypes.c_uint32), ("ext_grid_lds", ctypes.c_uint32),
+                ("sw_shift", ctypes.c_uint32), ("batch_pct", ctypes.c_uint32), ("n_cu", ctypes.c_uint32),
+                ("lds_fac_above", ctypes.c_uint64)]
+    KEYS = ("rq_grid", "ext_grid", "ext_grid_lds", "sw_shift", "batch_pct", "lds_fac_above")
+
+    def as_dict(self):
+        d = {k: int(getattr(self, k)) for k in self.KEYS}
+        if self.n_cu:
+            d["n_cu"] = int(self.n_cu)
+        return d
+
+    @classmethod
+    def from_dict(cls, d):
+        d = {k: v for k, v in d.items() if k != "n_cu"}
+        unknown = set(d) - set(cls.KEYS)
+        if unknown:
+            raise ValueError(f"unknown tuning keys {sorted(unknown)}")
+        return cls(**{k: int(v) for k, v in d.items()})
+
+
+def parse_tuning(text):
+    """Parse a tuning file's text with the library's own parser (no device needed)."""
+    lib = load()
+    t = Tuning()
+    rc = lib.ps4_parse_tuning(text.encode(), ctypes.byref(t))
+    if rc:
+        raise PS4Error(rc, "tuning file")
+    return t.as_dict()
+
+
+def format_tuning(d, comment=""):
+    """The text of a tuning file for the dict d (the grammar ps4_parse_tuning reads)."""
+    head = "".join(f"# {c}\n" for c in comment.splitlines() if c)
+    return head + "".join(f"{k} = {int(d[k])}\n" for k in Tuning.KEYS if k in d)
+
+
 SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t)
 ALERT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p)
 
@@ -81,6 +119,9 @@ def load():
     lib.ps4_search_elements.argtypes = [P, u64, SINK, P, u64p, u64p]
     lib.ps4_set_alert.argtypes = [P, ALERT, P]
     lib.ps4_set_dump4.argtypes = [P, ctypes.c_int64]
+    lib.ps4_get_tuning.argtypes = [P, ctypes.POINTER(Tuning), ctypes.POINTER(Tuning)]
+    lib.ps4_set_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+    lib.ps4_parse_tuning.argtypes = [ctypes.c_char_p, ctypes.POINTER(Tuning)]
     lib.ps4_close.argtypes = [P]
     lib.ps4_close.restype = None
     lib.ps4_strerror.argtypes = [ctypes.c_int]
@@ -97,7 +138,8 @@ class Engine:
     """One device context (resident tables for S < S_hi_max)."""
 
     def __init__(self, S_hi_max, device=0, noext=False, verify=False, test6=False, dump4=0,
-                 cap_reps=0, cap_divs=0, cap_ext=0):
+                 cap_reps=0, cap_divs=0, cap_ext=0, tune=None):
+        """tune: a tuning dict or the path of a tuning file (ps4/autotune.py)."""
         self.lib = load()
         cfg = Config(device, int(S_hi_max), (NOEXT if noext else 0) | (VERIFY if verify else 0) |
                      (TEST6 if test6 else 0), int(dump4), cap_reps, cap_divs, cap_ext)
@@ -107,6 +149,8 @@ class Engine:
             raise PS4Error(rc, "ps4_open")
         self.alerts = []
         self._alert_cb = None
+        if tune is not None:
+            self.set_tuning(tune)
 
     def _check(self, rc):
         if rc:
@@ -121,6 +165,20 @@ class Engine:
         else:
             self._alert_cb = ALERT(0)
         self._check(self.lib.ps4_set_alert(self.ctx, self._alert_cb, None))
+
+    def get_tuning(self):
+        """(values in force, maxima) as dicts."""
+        cur, mx = Tuning(), Tuning()
+        self._check(self.lib.ps4_get_tuning(self.ctx, ctypes.byref(cur), ctypes.byref(mx)))
+        return cur.as_dict(), mx.as_dict()
+
+    def set_tuning(self, tune):
+        """Apply a tuning dict (missing or 0 fields: defaults) or a tuning file's path."""
+        if isinstance(tune, (str, os.PathLike)):
+            with open(tune) as f:
+                tune = parse_tuning(f.read())
+        t = Tuning.from_dict(tune)
+        self._check(self.lib.ps4_set_tuning(self.ctx, ctypes.byref(t)))
 
     def set_dump4(self, n):
         self._check(self.lib.ps4_set_dump4(self.ctx, int(n)))

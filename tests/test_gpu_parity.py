@@ -390,6 +390,77 @@ def test_search_batch_uneven_subranges(eng):
         assert sorted(alone[1]) == sorted(lines), (a, b)
 
 
+def test_tuning_never_changes_results(eng):
+    """Launch shape and batch plan are timing only: extreme tunings (one
+    workgroup per CU, 8x finer subrange split, tiny batches, k_ext's LDS form
+    from S = 2^20 or never) give the default results on a census and on
+    uneven subranges; values above the maxima are rejected."""
+    from ps4 import workloads
+    base, mx = eng.get_tuning()
+    n_cu = mx["n_cu"]
+    assert n_cu > 0 and base["rq_grid"] == mx["rq_grid"] and base["batch_pct"] == 80
+    ranges = workloads.batched_ranges(24, seed=3, top=10**11, e0=3, e1=7)
+    los, his = [a for a, _ in ranges], [b for _, b in ranges]
+    say("tuning: default results")
+    ref_c = eng.search(2 * 10**9, 2 * 10**9 + 2 * 10**7)
+    ref_b = eng.search_batch(los, his)
+    tunings = [
+        {"rq_grid": n_cu, "ext_grid": n_cu, "ext_grid_lds": n_cu, "sw_shift": 3, "batch_pct": 5,
+         "lds_fac_above": 1 << 20},
+        {"rq_grid": mx["rq_grid"] - n_cu, "ext_grid": mx["ext_grid"] - n_cu, "sw_shift": 1, "batch_pct": 95,
+         "lds_fac_above": 1 << 50},
+    ]
+    try:
+        for t in tunings:
+            say(f"tuning: {t}")
+            eng.set_tuning(t)
+            cur, _ = eng.get_tuning()
+            assert all(cur[k] == v for k, v in t.items())
+            assert eng.search(2 * 10**9, 2 * 10**9 + 2 * 10**7) == ref_c
+            got = eng.search_batch(los, his)
+            for r, (c, lines), (c0, l0) in zip(ranges, got, ref_b):
+                assert c == c0, (r, t)
+                assert sorted(lines) == sorted(l0), (r, t)
+        for over in ({"rq_grid": mx["rq_grid"] + 1}, {"ext_grid": mx["ext_grid"] + 1}, {"sw_shift": 9},
+                     {"batch_pct": 96}):
+            with pytest.raises(ps4.PS4Error):
+                eng.set_tuning(over)
+    finally:
+        eng.set_tuning({})
+    assert eng.get_tuning()[0] == base
+
+
+def test_autotune_runs_and_keeps_results(eng, tmp_path):
+    """The autotuner on a small sample: every trial's results must equal the
+    default's (it raises otherwise); its file reads back and applies."""
+    from ps4 import autotune
+    say("autotune on 16 subranges")
+    try:
+        best, rep = autotune.tune(eng, n=16, seed=5, top=10**10, e0=3, e1=6, reps=1, log=say)
+        assert rep["trials"] and all(t["seconds"] > 0 for t in rep["trials"])
+        f = tmp_path / "tune.txt"
+        f.write_text(ps4.format_tuning(best, "test"))
+        eng.set_tuning(str(f))
+        cur, _ = eng.get_tuning()
+        assert {k: cur[k] for k in best} == best
+    finally:
+        eng.set_tuning({})
+
+
+def test_cli_tune_file(tmp_path, golden):
+    """ps4 --tune FILE: same output as without; a bad file is a usage error."""
+    good = tmp_path / "t.txt"
+    good.write_text("sw_shift = 2\nbatch_pct = 30\n")
+    r = _run_cli([1, 10**7, "--tune", good, "--out", tmp_path / "o.out"], tmp_path)
+    assert r.returncode == 0, r.stderr.decode()
+    rec = next(x for x in golden["runs"] if (x["lo"], x["hi"]) == (1, 10**7))
+    assert sha_sorted((tmp_path / "o.out").read_text().splitlines()) == rec["sorted_sha256"]
+    bad = tmp_path / "b.txt"
+    bad.write_text("threads = 4\n")
+    r = _run_cli([1, 10**6, "--tune", bad], tmp_path)
+    assert r.returncode == 1
+
+
 def test_serving_results_same_alone_and_in_the_mix(eng):
     """Requests through the continuous-batching loop (arriving while others run)
     give exactly the results of the same requests run alone."""
