@@ -62,6 +62,7 @@ PEAK_LANE_OPS = 256 * 64 * 2.4e9      # 256 CUs x 64 lanes x 2.4 GHz, 32-bit int
 K_EXT_TRAFFIC = {(1, 10**10): 36.18e9}
 GOLDEN = os.path.join(REPO, "tests", "golden", "golden.json")
 TUNE = None   # tuning file (--tune): launch shape and batch plan of every engine opened here
+GRAPH = False  # --graph: every batch launched as one captured hipGraph
 
 
 def log(*a):
@@ -179,7 +180,7 @@ def bench_batched(args):
     ranges = workloads.batched_ranges(4096, seed=42, top=10**11)
     los, his = [a for a, _ in ranges], [b for _, b in ranges]
     total = sum(b - a for a, b in ranges)
-    eng = ps4.Engine(S_hi_max=10**11 + 1, tune=TUNE)
+    eng = ps4.Engine(S_hi_max=10**11 + 1, tune=TUNE, graph=GRAPH)
     eng.search(1, 10**6)                     # warm the device
     log(f"batched: {len(ranges)} subranges, {total:.3e} S in one ps4_search_batch")
     eng.kernel_time(reset=True)
@@ -216,7 +217,7 @@ def bench_band(args):
     from ps4 import workloads
     lo, hi, width = 9 * 10**11, 10**12, 10**8
     sub = workloads.band_ranges(lo, hi, width)
-    eng = ps4.Engine(S_hi_max=hi, tune=TUNE)
+    eng = ps4.Engine(S_hi_max=hi, tune=TUNE, graph=GRAPH)
     eng.search(lo, lo + 10**6)
     eng.kernel_time(reset=True)
     log(f"band: [{lo},{hi}) as {len(sub)} subranges in one ps4_search_batch")
@@ -275,7 +276,7 @@ def bench_serving(args):
     from ps4 import serve, workloads
     n = args.requests
     stream = workloads.serving_stream(n, rate=200.0, seed=7)
-    eng = ps4.Engine(S_hi_max=10**12 + 10**8 + 1, tune=TUNE)
+    eng = ps4.Engine(S_hi_max=10**12 + 10**8 + 1, tune=TUNE, graph=GRAPH)
     eng.search(10**12 - 10**6, 10**12)   # warm the device
     srv = serve.Server(eng)
     log(f"serving: {n} requests at 200/s (open loop)")
@@ -334,7 +335,7 @@ def bench_elements(args):
     (filtered by largest element afterwards)."""
     import ps4
     n_max = 10**7
-    eng = ps4.Engine(S_hi_max=4 * n_max, tune=TUNE)
+    eng = ps4.Engine(S_hi_max=4 * n_max, tune=TUNE, graph=GRAPH)
     eng.search_elements(10**5)   # warm the device
     times = []
     for _ in range(3):
@@ -379,13 +380,14 @@ def main():
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--workload", choices=["plain", "batched", "band", "serving", "elements"], default="plain")
     ap.add_argument("--tune", default=None, help="tuning file from ps4/autotune.py (launch shape only)")
+    ap.add_argument("--graph", action="store_true", help="launch every batch as one captured hipGraph")
     ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once (a plain copy of known size that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters in a profiled run)")
     args = ap.parse_args()
-    global TUNE
-    TUNE = args.tune
+    global TUNE, GRAPH
+    TUNE, GRAPH = args.tune, args.graph
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the census shards by S range "
               f"across GPUs (disjoint S intervals, one engine per device), but this bench measures one device.", file=sys.stderr)
@@ -397,7 +399,7 @@ def main():
         return
     import ps4
     lo, hi = args.lo, args.hi
-    eng = ps4.Engine(S_hi_max=hi, tune=TUNE)
+    eng = ps4.Engine(S_hi_max=hi, tune=TUNE, graph=GRAPH)
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
     for w in range(args.warmup):

@@ -41,6 +41,9 @@ extern "C" {
 #define PS4_NOEXT   0x1u    /* --noext: count 4-sets, do not extend (:242)                          */
 #define PS4_VERIFY  0x2u    /* --verify: re-check the six roots of every 4-set (:229-233)           */
 #define PS4_TEST6   0x4u    /* --test-sixset: exercise the 6-set alert on the first M: line (:255-260) */
+#define PS4_GRAPH   0x8u    /* launch each batch's sequence (copy, reset, kernels, read-backs) as one
+                               captured hipGraph, updated in place from batch to batch; the kernel
+                               time is then reported as one figure (ms_rq).  No s4 counterpart    */
 
 /* Opaque context: resident tables, device buffers, streams. */
 typedef struct ps4_ctx ps4_ctx;
@@ -57,7 +60,7 @@ typedef struct {
     int device;            /* HIP device ordinal                                              */
     uint64_t S_hi_max;     /* largest S_hi any later call may use; sizes the factor tables
                               (exhaustive/s4.c:439-447: n < 2*isqrt(S_hi)+64)                   */
-    uint32_t flags;        /* PS4_NOEXT | PS4_VERIFY | PS4_TEST6                                */
+    uint32_t flags;        /* PS4_NOEXT | PS4_VERIFY | PS4_TEST6 | PS4_GRAPH                    */
     int64_t dump4_limit;   /* emit the first dump4_limit 4-sets per search as "4:" lines
                               (--dump4 N, :237-241; 0 = none)                                   */
     uint32_t cap_reps, cap_divs, cap_ext;   /* capacity limits; 0 = the reference's 512 / 16384 / 64

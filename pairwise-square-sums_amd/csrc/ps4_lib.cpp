@@ -224,6 +224,7 @@ struct ps4_ctx {
     double ms_rq = 0.0, ms_ext = 0.0;
     u64 n_batches = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t gexec = nullptr;   // PS4_GRAPH: the batch sequence, updated per batch
     // triple queries
     u64 *d_tabc = nullptr, *d_tout = nullptr;
     u32 *d_tcnt = nullptr, *d_terr = nullptr;
@@ -603,6 +604,9 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
             }
         }
         const u32 n_ctr = (rid_hi + 1) * CT_N;
+        // PS4_GRAPH: the sequence below is captured and launched as one graph
+        const bool graph = (ctx->flags & PS4_GRAPH) && !ctx->debug_sync;
+        if (graph) HIP_OK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
         HIP_OK(hipMemcpyAsync(ctx->d_items, ctx->h_items, n_items * sizeof(ps4_item), hipMemcpyHostToDevice, ctx->st));
         // PS4_DEBUG_SYNC=1: wait after every launch and name the kernel that failed
         auto dbg = [&](const char *what) -> int {
@@ -618,11 +622,11 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         };
         HIP_OK(ps4_launch_reset(ctx->st, ctx->d_ctl, ctx->d_ctr, n_ctr));
         if (int rc = dbg("k_reset")) return rc;
-        HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
+        if (!graph) HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
         HIP_OK(ps4_launch_rq(ctx->st, ctx->rq_grid, ctx->d_items, n_items, ctx->bin_cap, ctx->nb_alloc, ctx->d_ctl,
                              ctx->d_ctr, rid_hi + 1, ctx->d_scratch, ctx->d_quads, ctx->quad_cap, ctx->cap_reps,
                              (ctx->flags & PS4_VERIFY) ? 1u : 0u));
-        HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
+        if (!graph) HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
         if (int rc = dbg("k_rq")) return rc;
         if (ctx->elem_max) {
             HIP_OK(ps4_launch_fill_u32(ctx->st, ctx->d_ecnt, 1, 0));
@@ -636,11 +640,35 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
             HIP_OK(ps4_launch_ext(ctx->st, lds_fac ? ctx->ext_grid_lds : ctx->ext_grid, ctx->d_quads, ctx->quad_cap,
                                   ctx->d_ctl, ctx->d_fpair, ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, rid_hi + 1,
                                   ctx->d_hits, ctx->hit_cap, ctx->cap_divs, ctx->d_exq, lds_fac ? 1 : 0));
-        HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
+        if (!graph) HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
         if (!noext)
             if (int rc = dbg("k_ext")) return rc;
         HIP_OK(hipMemcpyAsync(ctx->h_ctl, ctx->d_ctl, sizeof(ps4_ctl), hipMemcpyDeviceToHost, ctx->st));
         HIP_OK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, (size_t)n_ctr * sizeof(u64), hipMemcpyDeviceToHost, ctx->st));
+        if (graph) {
+            // same node structure from batch to batch (only the arguments and
+            // sizes change), so the executable graph is updated in place; it is
+            // built anew when the structure changes (an update that fails)
+            hipGraph_t g = nullptr;
+            HIP_OK(hipStreamEndCapture(ctx->st, &g));
+            if (ctx->gexec) {
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult res;
+                if (hipGraphExecUpdate(ctx->gexec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(ctx->gexec);
+                    ctx->gexec = nullptr;
+                }
+            }
+            hipError_t e = hipSuccess;
+            if (!ctx->gexec) e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_OK(e);
+            HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
+            HIP_OK(hipGraphLaunch(ctx->gexec, ctx->st));
+            HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
+            HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
+        }
         if (have_pending) {
             have_pending = false;
             if (int rc = emit(pending)) return fail(rc);
@@ -649,6 +677,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         float t_rq = 0.f, t_ext = 0.f;
         HIP_OK(hipEventElapsedTime(&t_rq, ctx->ev[0], ctx->ev[1]));
         HIP_OK(hipEventElapsedTime(&t_ext, ctx->ev[1], ctx->ev[2]));
+        if (graph) { t_rq += t_ext; t_ext = 0.f; }   // one figure for the whole graph
         ctx->ms_rq += t_rq;
         ctx->ms_ext += t_ext;
         ++ctx->n_batches;
@@ -756,6 +785,7 @@ extern "C" void ps4_close(ps4_ctx *ctx)
     for (void *p : dev) if (p) (void)hipFree(p);
     void *host[] = {ctx->h_ctl, ctx->h_ctr, ctx->h_items, ctx->h_hits};
     for (void *p : host) if (p) (void)hipHostFree(p);
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     for (hipEvent_t e : ctx->ev) if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libps4.so")
 CLI_PATH = os.path.join(PKG_DIR, "lib", "ps4")
 REPO_DIR = os.path.dirname(PKG_DIR)
 
-NOEXT, VERIFY, TEST6 = 0x1, 0x2, 0x4
+NOEXT, VERIFY, TEST6, GRAPH = 0x1, 0x2, 0x4, 0x8
 ERRORS = {0: "PS4_OK", -1: "PS4_E_ARG", -2: "PS4_E_CAPACITY", -3: "PS4_E_HIP", -4: "PS4_E_OOM",
           -5: "PS4_E_VERIFY", -6: "PS4_E_SINK", -7: "PS4_E_INTERNAL"}
 # TOTAL-line keys of the reference (exhaustive/s4.c:498-500), in order
@@ -138,11 +138,13 @@ class Engine:
     """One device context (resident tables for S < S_hi_max)."""
 
     def __init__(self, S_hi_max, device=0, noext=False, verify=False, test6=False, dump4=0,
-                 cap_reps=0, cap_divs=0, cap_ext=0, tune=None):
-        """tune: a tuning dict or the path of a tuning file (ps4/autotune.py)."""
+                 cap_reps=0, cap_divs=0, cap_ext=0, tune=None, graph=False):
+        """tune: a tuning dict or the path of a tuning file (ps4/autotune.py);
+        graph: launch each batch as one captured hipGraph (PS4_GRAPH)."""
         self.lib = load()
         cfg = Config(device, int(S_hi_max), (NOEXT if noext else 0) | (VERIFY if verify else 0) |
-                     (TEST6 if test6 else 0), int(dump4), cap_reps, cap_divs, cap_ext)
+                     (TEST6 if test6 else 0) | (GRAPH if graph else 0), int(dump4), cap_reps, cap_divs,
+                     cap_ext)
         self.ctx = ctypes.c_void_p()
         rc = self.lib.ps4_open(ctypes.byref(cfg), ctypes.byref(self.ctx))
         if rc:

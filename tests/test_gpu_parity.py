@@ -461,6 +461,41 @@ def test_cli_tune_file(tmp_path, golden):
     assert r.returncode == 1
 
 
+def test_graph_launches_equal_eager(eng):
+    """PS4_GRAPH (each batch's copy, reset, kernels and read-backs as one captured
+    hipGraph, updated in place between batches): every replay equals the eager
+    engine's result, with eager work and synchronisations between the replays;
+    a census of several batches (small batch fill) and uneven subranges."""
+    from ps4 import workloads
+    ranges = workloads.batched_ranges(24, seed=11, top=10**11, e0=3, e1=7)
+    los, his = [a for a, _ in ranges], [b for _, b in ranges]
+    say("graph: eager references")
+    ref_c = eng.search(3 * 10**9, 3 * 10**9 + 3 * 10**7)
+    ref_b = eng.search_batch(los, his)
+    os.environ["PS4_QUAD_CAP"] = str(1 << 22)   # a small 4-set queue: several batches per call
+    try:
+        g = ps4.Engine(S_hi_max=10**11 + 1, graph=True)
+    finally:
+        del os.environ["PS4_QUAD_CAP"]
+    try:
+        g.set_tuning({"batch_pct": 30})         # the graph is updated between batches
+        for rep in range(3):
+            say(f"graph: replay {rep}")
+            assert g.search(3 * 10**9, 3 * 10**9 + 3 * 10**7) == ref_c
+            got = g.search_batch(los, his)
+            for r, (c, lines), (c0, l0) in zip(ranges, got, ref_b):
+                assert c == c0, (r, rep)
+                assert sorted(lines) == sorted(l0), (r, rep)
+            assert eng.search(10**6, 2 * 10**6)[0]["counters"]["S_done"] == 10**6   # eager work between
+        assert g.kernel_time()[2] >= 12   # several graph launches per call
+        g.set_tuning({})
+        res, lines = g.search(1, 10**8)
+        one, l1 = eng.search(1, 10**8)
+        assert res == one and sorted(lines) == sorted(l1)
+    finally:
+        g.close()
+
+
 def test_serving_results_same_alone_and_in_the_mix(eng):
     """Requests through the continuous-batching loop (arriving while others run)
     give exactly the results of the same requests run alone."""
