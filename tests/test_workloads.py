@@ -50,3 +50,46 @@ def test_serving_stream_requests():
             assert 0 < a < b < c <= 10**9 and d_max == 10**12
             for x, y in ((a, b), (a, c), (b, c)):
                 assert math.isqrt(x + y) ** 2 == x + y
+
+
+# ---- near-miss post-processing of M: lines (ps4/nearmiss.py)
+from ps4 import nearmiss  # noqa: E402
+
+# exhaustive/test_verify.py:31-32: the reference's known M: line
+KNOWN_M = ("M: [28860026400,17124487200,4431625200,162503200] ext=[4981255200,2553991200,1674390681] "
+           "S=50578642000")
+
+
+def test_square_part_brute_force():
+    def brute(n):
+        return max(d * d for d in range(1, math.isqrt(n) + 1) if n % (d * d) == 0)
+    for n in list(range(1, 3000)) + [2**40, 3**2 * 7**4 * 1000003**2, 999983 * 1000003, 10007**2 * 6, 10**12 + 39]:
+        if n < 3000 or n in (2**40,):
+            assert nearmiss.square_part(n) == brute(n), n
+    assert nearmiss.square_part(3**2 * 7**4 * 1000003**2) == 3**2 * 7**4 * 1000003**2
+    assert nearmiss.square_part(999983 * 1000003) == 1
+    assert nearmiss.square_part(10007**2 * 6) == 10007**2
+
+
+def test_split_known_mline():
+    S, near, sols = nearmiss.split_mline(KNOWN_M)
+    assert S == 50578642000 and not sols
+    assert len(near) == 3                      # three extension pairs, none a solution
+    for prim, pair in near.items():
+        assert list(prim) == sorted(prim) and set(pair) <= set(prim)
+        assert nearmiss.square_part(math.gcd(*prim)) == 1      # primitive
+        assert not nearmiss.is_square(pair[0] + pair[1])
+        # the only non-square pair sum is the pair's (14 of 15 square)
+        bad = [(u, v) for i, u in enumerate(prim) for v in prim[i + 1:] if not nearmiss.is_square(u + v)]
+        assert bad == [pair]
+        assert 0 <= nearmiss.euler_count(prim, pair) <= 4
+    near_all, sols_all = nearmiss.classify(["5: [1,2,3,4,5] S=1", KNOWN_M, KNOWN_M])
+    assert set(near_all) == set(near) and not sols_all
+
+
+def test_split_rejects_bad_lines():
+    import pytest
+    for bad in ("5: [1,2,3,4,5] S=15", "M: [1,2,3,4] ext=[5] S=10",
+                KNOWN_M.replace("4431625200", "4431625201").replace("S=50578642000", "S=50578642001")):
+        with pytest.raises(ValueError):
+            nearmiss.split_mline(bad)
