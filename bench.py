@@ -9,6 +9,8 @@ is the input; nothing is cached between steps.  Prints one JSON line.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
                     [--lo 1] [--hi 10000000000] [--no-cpu] [--no-latency]
     python bench.py --workload batched|band|serving|elements
+    (any of them) [--tune FILE]  tuning file from ps4/autotune.py
+                  [--graph]      every batch launched as one captured hipGraph
 
 The default workload is the plain census above.  The others print their own
 JSON line:
