@@ -496,6 +496,25 @@ def test_graph_launches_equal_eager(eng):
         g.close()
 
 
+def test_near_misses_of_the_census_mlines(eng, golden):
+    """Every M: line of the [1,10^9) census (59, as in the reference's run) splits
+    into near misses with 14 of 15 square sums (checked inside split_mline) and
+    no 6-set solution (the reference finds none below 10^13); scaled copies
+    share their primitive forms (18 distinct ones here)."""
+    from ps4 import nearmiss
+    say("near misses of [1,1e9) M-lines")
+    res, lines = eng.search(1, 10**9)
+    mlines = [x for x in lines if x.startswith("M:")]
+    assert len(mlines) == res["counters"]["multi-ext-4sets"] == 59
+    near, sols = nearmiss.classify(mlines)
+    # M-lines that are square multiples of one another share their primitive near misses
+    pairs = sum(math.comb(len(nearmiss.parse_mline(x)[1]), 2) for x in mlines)
+    assert not sols and 0 < len(near) <= pairs
+    assert all(nearmiss.split_mline(x)[1] for x in mlines)      # each line gives a near miss
+    for prim, (pair, eu, S) in near.items():
+        assert set(pair) <= set(prim) and 0 <= eu <= 4 and S < 10**9
+
+
 def test_serving_results_same_alone_and_in_the_mix(eng):
     """Requests through the continuous-batching loop (arriving while others run)
     give exactly the results of the same requests run alone."""
