@@ -23,7 +23,12 @@
  * "threads" keeps its meaning for the outputs: the interval is cut into that
  * many contiguous regions exactly as exhaustive/s4.c:451-456 does, and each
  * region gets its own counters, per-thread line and checkpoint position; the
- * regions are searched one after another on the device.  Default 1.
+ * regions are searched one after another on the device.  Default: the online
+ * CPU count, as the reference's threads=0 (exhaustive/s4.c:428-431), so the
+ * header and per-thread lines read as its would.
+ * Checkpoints are written every 5 s (exhaustive/s4.c:470-478) between blocks
+ * of about 2^33 S; PS4_CKPT_SECS=x (a drill setting, no s4 counterpart) sets
+ * the interval, 0 = after every block.
  * Deviations: S_hi < S_lo is rejected (the reference wraps around, :451), and
  * --dump4 N applies per region in S order (the reference's per-thread order).
  */
@@ -238,7 +243,8 @@ int main(int argc, char **argv)
     if (S_lo < 1) S_lo = 1;
     if (S_hi < S_lo) { fprintf(stderr, "S_hi < S_lo\n"); return 1; }
     if (T <= 0) {
-        T = 1;
+        const long nc = sysconf(_SC_NPROCESSORS_ONLN);
+        T = nc > 0 ? (int)nc : 1;
         if (opt_ckpt && opt_resume) { int c = ckpt_threads(); if (c > 0) T = c; }
     }
     if (opt_ckpt && !opt_resume && access(opt_ckpt, F_OK) == 0) {
@@ -292,6 +298,7 @@ int main(int argc, char **argv)
     memset(&run, 0, sizeof run);
     const double t0 = now_sec();
     double last_ck = t0, last_rep = t0;
+    const double ck_every = getenv("PS4_CKPT_SECS") ? atof(getenv("PS4_CKPT_SECS")) : 5.0;
     for (int t = 0; t < T; t++) {
         const double ts = now_sec();
         long long left = opt_dump4;
@@ -312,7 +319,7 @@ int main(int argc, char **argv)
             rg[t].next = hi;
             done_chunks += (hi - lo + W - 1) / W;
             const double nw = now_sec();
-            if (nw - last_ck >= 5.0) { write_ckpt(S_lo, S_hi, T, W, rg, 0); last_ck = nw; }
+            if (nw - last_ck >= ck_every) { write_ckpt(S_lo, S_hi, T, W, rg, 0); last_ck = nw; }
             if (nw - last_rep >= 30.0) {
                 last_rep = nw;
                 ull s4 = 0, s5 = 0, s6 = 0;

@@ -196,6 +196,7 @@ struct ps4_ctx {
     double batch_frac = 0.8;
     u64 lds_fac_above = 1ull << 35;
     u32 W = 0, nb_alloc = 0, bin_cap = 0;
+    u32 scratch_wgs = 0;       // workgroups the window-bin scratch holds
     u64 quad_cap = 0;
     u32 hit_cap = 0, items_cap = 0, ctr_cap = 0;
     // device
@@ -332,10 +333,14 @@ int ensure_ctr(ps4_ctx *ctx, u32 n)
     return 0;
 }
 
+// The representation kernel's window bins: one region per workgroup of the
+// largest grid the tuning may select (rq_grid_max), so that lowering the grid,
+// growing the bins, then raising the grid again never indexes past the end.
 int alloc_scratch(ps4_ctx *ctx)
 {
     if (ctx->d_scratch) { (void)hipFree(ctx->d_scratch); ctx->d_scratch = nullptr; }
-    HIP_OK(hipMalloc(&ctx->d_scratch, (size_t)ctx->rq_grid * ctx->nb_alloc * ctx->bin_cap * sizeof(u64)));
+    ctx->scratch_wgs = ctx->rq_grid_max;
+    HIP_OK(hipMalloc(&ctx->d_scratch, (size_t)ctx->scratch_wgs * ctx->nb_alloc * ctx->bin_cap * sizeof(u64)));
     return 0;
 }
 
@@ -404,6 +409,94 @@ struct Pending {
     std::vector<ps4_quad> qs;
     std::vector<ps4_hit> hits;
 };
+
+// The device sequence of one batch on the context's stream: the items copy,
+// counter reset, k_rq, (k_elem), k_ext and the read-backs of the control block
+// and counter rows into pinned memory.  Kernel time is taken by events around
+// the two census kernels.  Returns once everything is enqueued.
+int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, bool graph)
+{
+    const u32 n_ctr = n_rids * CT_N;
+    if (ctx->rq_grid > ctx->scratch_wgs) { ctx->last_error = "k_rq grid above its scratch"; return PS4_E_INTERNAL; }
+    HIP_OK(hipMemcpyAsync(ctx->d_items, ctx->h_items, n_items * sizeof(ps4_item), hipMemcpyHostToDevice, ctx->st));
+    // PS4_DEBUG_SYNC=1: wait after every launch and name the kernel that failed
+    auto dbg = [&](const char *what) -> int {
+        if (!ctx->debug_sync) return 0;
+        fprintf(stderr, "ps4: %s (items %u)\n", what, n_items);
+        fflush(stderr);
+        const hipError_t e = hipStreamSynchronize(ctx->st);
+        if (e != hipSuccess) {
+            ctx->last_error = std::string("after ") + what + ": " + hipGetErrorString(e);
+            return PS4_E_HIP;
+        }
+        return 0;
+    };
+    HIP_OK(ps4_launch_reset(ctx->st, ctx->d_ctl, ctx->d_ctr, n_ctr));
+    if (int rc = dbg("k_reset")) return rc;
+    if (!graph) HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
+    HIP_OK(ps4_launch_rq(ctx->st, ctx->rq_grid, ctx->d_items, n_items, ctx->bin_cap, ctx->nb_alloc, ctx->d_ctl,
+                         ctx->d_ctr, n_rids, ctx->d_scratch, ctx->d_quads, ctx->quad_cap, ctx->cap_reps,
+                         (ctx->flags & PS4_VERIFY) ? 1u : 0u));
+    if (!graph) HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
+    if (int rc = dbg("k_rq")) return rc;
+    if (ctx->elem_max) {
+        HIP_OK(ps4_launch_fill_u32(ctx->st, ctx->d_ecnt, 1, 0));
+        HIP_OK(ps4_launch_elem(ctx->st, ctx->d_quads, ctx->d_ctl, ctx->quad_cap, ctx->elem_max, ctx->d_eout,
+                               PS4_ELEM_CAP, ctx->d_ecnt, ctx->n_cu * 8));
+    }
+    // the extension kernel's factor lists: in LDS above S ~ 3*10^10 (larger
+    // factor tables), in global scratch with two more waves per CU below
+    const bool lds_fac = s_top > ctx->lds_fac_above;
+    if (!noext)
+        HIP_OK(ps4_launch_ext(ctx->st, lds_fac ? ctx->ext_grid_lds : ctx->ext_grid, ctx->d_quads, ctx->quad_cap,
+                              ctx->d_ctl, ctx->d_fpair, ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, n_rids,
+                              ctx->d_hits, ctx->hit_cap, ctx->cap_divs, ctx->d_exq, lds_fac ? 1 : 0));
+    if (!graph) HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
+    if (!noext)
+        if (int rc = dbg("k_ext")) return rc;
+    HIP_OK(hipMemcpyAsync(ctx->h_ctl, ctx->d_ctl, sizeof(ps4_ctl), hipMemcpyDeviceToHost, ctx->st));
+    HIP_OK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, (size_t)n_ctr * sizeof(u64), hipMemcpyDeviceToHost, ctx->st));
+    return 0;
+}
+
+// One batch launched: directly, or (graph) captured and launched as one
+// hipGraph.  A failure inside the capture ends the capture and drops the
+// partial graph, so the stream is usable again.
+int launch_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, bool graph)
+{
+    if (!graph) return enqueue_batch(ctx, n_items, n_rids, s_top, noext, false);
+    HIP_OK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_batch(ctx, n_items, n_rids, s_top, noext, true);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->st, &g);
+    if (rc || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        if (!rc) ctx->last_error = std::string("hipStreamEndCapture: ") + hipGetErrorString(ec);
+        return rc ? rc : PS4_E_HIP;
+    }
+    // same node structure from batch to batch (only the arguments and
+    // sizes change), so the executable graph is updated in place; it is
+    // built anew when the structure changes (an update that fails)
+    if (ctx->gexec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res;
+        if (hipGraphExecUpdate(ctx->gexec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(ctx->gexec);
+            ctx->gexec = nullptr;
+        }
+    }
+    hipError_t e = hipSuccess;
+    if (!ctx->gexec) e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_OK(e);
+    HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
+    HIP_OK(hipGraphLaunch(ctx->gexec, ctx->st));
+    HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
+    HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
+    return 0;
+}
 
 int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink sink, void *const *users,
             void *single_user, ps4_counters *out)
@@ -603,72 +696,9 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
                 rid_hi = v.rid;
             }
         }
-        const u32 n_ctr = (rid_hi + 1) * CT_N;
-        // PS4_GRAPH: the sequence below is captured and launched as one graph
+        // PS4_GRAPH: the sequence is captured and launched as one graph
         const bool graph = (ctx->flags & PS4_GRAPH) && !ctx->debug_sync;
-        if (graph) HIP_OK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-        HIP_OK(hipMemcpyAsync(ctx->d_items, ctx->h_items, n_items * sizeof(ps4_item), hipMemcpyHostToDevice, ctx->st));
-        // PS4_DEBUG_SYNC=1: wait after every launch and name the kernel that failed
-        auto dbg = [&](const char *what) -> int {
-            if (!ctx->debug_sync) return 0;
-            fprintf(stderr, "ps4: %s (items %u)\n", what, n_items);
-            fflush(stderr);
-            const hipError_t e = hipStreamSynchronize(ctx->st);
-            if (e != hipSuccess) {
-                ctx->last_error = std::string("after ") + what + ": " + hipGetErrorString(e);
-                return PS4_E_HIP;
-            }
-            return 0;
-        };
-        HIP_OK(ps4_launch_reset(ctx->st, ctx->d_ctl, ctx->d_ctr, n_ctr));
-        if (int rc = dbg("k_reset")) return rc;
-        if (!graph) HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
-        HIP_OK(ps4_launch_rq(ctx->st, ctx->rq_grid, ctx->d_items, n_items, ctx->bin_cap, ctx->nb_alloc, ctx->d_ctl,
-                             ctx->d_ctr, rid_hi + 1, ctx->d_scratch, ctx->d_quads, ctx->quad_cap, ctx->cap_reps,
-                             (ctx->flags & PS4_VERIFY) ? 1u : 0u));
-        if (!graph) HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
-        if (int rc = dbg("k_rq")) return rc;
-        if (ctx->elem_max) {
-            HIP_OK(ps4_launch_fill_u32(ctx->st, ctx->d_ecnt, 1, 0));
-            HIP_OK(ps4_launch_elem(ctx->st, ctx->d_quads, ctx->d_ctl, ctx->quad_cap, ctx->elem_max, ctx->d_eout,
-                                   PS4_ELEM_CAP, ctx->d_ecnt, ctx->n_cu * 8));
-        }
-        // the extension kernel's factor lists: in LDS above S ~ 3*10^10 (larger
-        // factor tables), in global scratch with two more waves per CU below
-        const bool lds_fac = s_top > ctx->lds_fac_above;
-        if (!noext)
-            HIP_OK(ps4_launch_ext(ctx->st, lds_fac ? ctx->ext_grid_lds : ctx->ext_grid, ctx->d_quads, ctx->quad_cap,
-                                  ctx->d_ctl, ctx->d_fpair, ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, rid_hi + 1,
-                                  ctx->d_hits, ctx->hit_cap, ctx->cap_divs, ctx->d_exq, lds_fac ? 1 : 0));
-        if (!graph) HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
-        if (!noext)
-            if (int rc = dbg("k_ext")) return rc;
-        HIP_OK(hipMemcpyAsync(ctx->h_ctl, ctx->d_ctl, sizeof(ps4_ctl), hipMemcpyDeviceToHost, ctx->st));
-        HIP_OK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, (size_t)n_ctr * sizeof(u64), hipMemcpyDeviceToHost, ctx->st));
-        if (graph) {
-            // same node structure from batch to batch (only the arguments and
-            // sizes change), so the executable graph is updated in place; it is
-            // built anew when the structure changes (an update that fails)
-            hipGraph_t g = nullptr;
-            HIP_OK(hipStreamEndCapture(ctx->st, &g));
-            if (ctx->gexec) {
-                hipGraphNode_t bad = nullptr;
-                hipGraphExecUpdateResult res;
-                if (hipGraphExecUpdate(ctx->gexec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
-                    (void)hipGetLastError();
-                    (void)hipGraphExecDestroy(ctx->gexec);
-                    ctx->gexec = nullptr;
-                }
-            }
-            hipError_t e = hipSuccess;
-            if (!ctx->gexec) e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            HIP_OK(e);
-            HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
-            HIP_OK(hipGraphLaunch(ctx->gexec, ctx->st));
-            HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
-            HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
-        }
+        if (int rc = launch_batch(ctx, n_items, rid_hi + 1, s_top, noext, graph)) return fail(rc);
         if (have_pending) {
             have_pending = false;
             if (int rc = emit(pending)) return fail(rc);

@@ -17,10 +17,14 @@ y[(i+2j)%3]).  The split counts if some signs on rows 1 and 2 make the three
 rows pairwise orthogonal.
 
 Pinned by tests/test_workloads.py:
-  - the reference's known M-line (exhaustive/test_verify.py:31-32);
+  - the reference's known M-line (exhaustive/test_verify.py:31-32), whose
+    primitives appear with the same non-square pairs among the 430 primitive
+    near misses of exhaustive/analysis_1e12.txt;
+  - the Euler count of all 865 distinct entries of the reference's near-miss
+    catalogs (exhaustive/evidence/*-compact.tar.gz::*/near-misses.json,
+    field euler_squares), and their non-square pairs and square totals;
   - brute force of the square part;
   - the 14-of-15 count of every primitive.
-The Euler count has no reference-held outputs: parity unpinned.
 """
 import itertools
 import math

@@ -19,6 +19,16 @@ Usage:  python3 tests/golden/make_golden.py [--big]
         records the 4-sets with largest element <= N (the element-range form
         of the search), their count and the SHA-256 of their "[a,b,c,d]"
         lines, members and list ascending;
+        python3 tests/golden/make_golden.py --square-triples
+        adds to each element record the number of square triples with
+        c <= N, enumerated by the oracle (seconds);
+        python3 tests/golden/make_golden.py --add-pooled LO HI PIECES WORKERS
+        runs [LO, HI) as PIECES single-thread pieces on WORKERS processes and
+        adds the summed record (the long ranges: [1, 1e11) this way);
+        python3 tests/golden/make_golden.py --batched-sample WORKERS
+        runs a seeded sample of 32 of the batched workload's subranges;
+        python3 tests/golden/make_golden.py --band-sample WORKERS
+        runs a seeded sample of 3 of the band's 1e8-wide subranges;
         python3 tests/golden/make_golden.py --add LO HI
         runs one more range (8 threads) and adds it to the existing fixture;
         [1, 6250000000) is there this way (about 2.5 minutes on 8 cores): it is
@@ -104,6 +114,57 @@ def add_run(lo, hi):
     print(lo, hi, rec["counters"]["4-sets"], rec["n_lines"], "added to", OUT)
 
 
+def _piece(args):
+    lo, hi = args
+    with tempfile.TemporaryDirectory() as tmp:
+        outp = os.path.join(tmp, "o.out")
+        res = subprocess.run([S4, str(lo), str(hi), "1", "20", "--out", outp], cwd=tmp,
+                             stdout=subprocess.PIPE, stderr=subprocess.PIPE, check=True)
+        with open(outp, "rb") as f:
+            lines = f.read().splitlines(keepends=True)
+    return lo, hi, res.stderr.decode(), lines
+
+
+def pooled_record(lo, hi, pieces, workers):
+    """One range run as `pieces` equal single-thread s4 runs on a worker pool.
+    The reference's counters are additive over a partition of the range and
+    its capacity figures are maxima (exhaustive/s4.c:488-506), so the sums,
+    maxima and the concatenated, byte-sorted lines equal one run's."""
+    from multiprocessing import Pool
+    cuts = [lo + (hi - lo) * k // pieces for k in range(pieces + 1)]
+    jobs = [(cuts[k], cuts[k + 1]) for k in range(pieces) if cuts[k] < cuts[k + 1]]
+    jobs.sort(key=lambda j: -j[0])            # costly high-S pieces first
+    counters = {k: 0 for k in KEYS}
+    hist, capacity, lines = {}, {"reps": 0, "divisors": 0, "extensions": 0}, []
+    with Pool(workers) as pool:
+        for n, (a, b, err, ls) in enumerate(pool.imap_unordered(_piece, jobs), 1):
+            c, h, cap = parse_err(err)
+            for k in KEYS:
+                counters[k] += c[k]
+            for k, v in h.items():
+                hist[k] = hist.get(k, 0) + v
+            for k in capacity:
+                capacity[k] = max(capacity[k], cap[k])
+            lines += ls
+            print("piece %d/%d [%d,%d) done" % (n, len(jobs), a, b), flush=True)
+    lines.sort()
+    rec = {"lo": lo, "hi": hi, "flags": [], "threads": 1, "pieces": len(jobs), "counters": counters,
+           "hist5": {k: hist[k] for k in sorted(hist, key=int)}, "capacity": capacity,
+           "n_lines": len(lines), "sorted_sha256": hashlib.sha256(b"".join(lines)).hexdigest()}
+    if len(lines) <= MAX_INLINE_LINES:
+        rec["lines"] = [ln.decode().rstrip("\n") for ln in lines]
+    return rec
+
+
+def add_pooled(lo, hi, pieces, workers):
+    rec = pooled_record(lo, hi, pieces, workers)
+    data = json.load(open(OUT))                # re-read: other additions may have landed meanwhile
+    data["runs"] = [r for r in data["runs"] if (r["lo"], r["hi"]) != (lo, hi)] + [rec]
+    with open(OUT, "w") as f:
+        json.dump(data, f, indent=1)
+    print(lo, hi, rec["counters"], rec["n_lines"], "added to", OUT, flush=True)
+
+
 def element_record(n_max):
     """The 4-sets with largest element <= n_max, from the reference's dump of
     every 4-set with S < 4 n_max - 5 (a + b + c + d <= 4 n_max - 6)."""
@@ -115,7 +176,66 @@ def element_record(n_max):
             sets.append(vals)
     sets.sort()
     text = "".join("[%d,%d,%d,%d]\n" % tuple(v) for v in sets)
-    return {"n_max": n_max, "n_sets": len(sets), "sha256": hashlib.sha256(text.encode()).hexdigest()}
+    return {"n_max": n_max, "n_sets": len(sets), "sha256": hashlib.sha256(text.encode()).hexdigest(),
+            "square_triples": enumerated_square_triples(n_max)}
+
+
+def enumerated_square_triples(n_max):
+    """Square triples {a<b<c} with c <= n_max, by direct enumeration of their
+    root triples in the oracle (oracle/cpu_ref.c, ora_square_triples); the
+    reference has no such count, so this is the fixture's only source."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import ora
+    return ora.square_triples(n_max)
+
+
+def add_batched_sample(workers):
+    """s4 on a seeded sample of 32 of the 4096 uneven subranges of [1, 1e11]
+    (the batched workload: splitmix64 seed 42, lengths log-uniform in
+    [1e4, 1e9]); the sample indices come from random.Random(4096)."""
+    sys.path.insert(0, os.path.join(REPO, "pairwise-square-sums_amd"))
+    from ps4 import workloads
+    ranges = workloads.batched_ranges(4096, seed=42, top=10**11)
+    pick = sorted(random.Random(4096).sample(range(len(ranges)), 32))
+    recs = []
+    for i in pick:
+        lo, hi = ranges[i]
+        rec = pooled_record(lo, hi, max(1, (hi - lo) // (5 * 10**7)), workers)
+        rec["index"] = i
+        recs.append(rec)
+        print("batched sample", i, lo, hi, rec["n_lines"], flush=True)
+    data = json.load(open(OUT))
+    data["batched_sample"] = {"workload": "batched_ranges(4096, seed=42, top=10**11)", "sample_seed": 4096,
+                              "runs": recs}
+    with open(OUT, "w") as f:
+        json.dump(data, f, indent=1)
+
+
+def add_band_sample(workers):
+    """s4 on a seeded 0.3% sample (3 of 1000) of the 1e8-wide subranges of the
+    band [9e11, 1e12); the sample indices come from random.Random(1000)."""
+    lo, hi, width = 9 * 10**11, 10**12, 10**8
+    pick = sorted(random.Random(1000).sample(range((hi - lo) // width), 3))
+    recs = []
+    for i in pick:
+        a = lo + i * width
+        rec = pooled_record(a, a + width, 8, workers)
+        rec["index"] = i
+        recs.append(rec)
+        print("band sample", i, a, rec["n_lines"], flush=True)
+    data = json.load(open(OUT))
+    data["band_sample"] = {"band": [lo, hi], "width": width, "sample_seed": 1000, "runs": recs}
+    with open(OUT, "w") as f:
+        json.dump(data, f, indent=1)
+
+
+def add_square_triple_counts():
+    data = json.load(open(OUT))
+    for rec in data.get("elements", []):
+        rec["square_triples"] = enumerated_square_triples(rec["n_max"])
+        print("elements", rec["n_max"], "square_triples", rec["square_triples"])
+    with open(OUT, "w") as f:
+        json.dump(data, f, indent=1)
 
 
 def add_elements(n_max):
@@ -128,11 +248,21 @@ def add_elements(n_max):
 
 
 def main():
+    if "--band-sample" in sys.argv:
+        return add_band_sample(int(sys.argv[sys.argv.index("--band-sample") + 1]))
+    if "--batched-sample" in sys.argv:
+        return add_batched_sample(int(sys.argv[sys.argv.index("--batched-sample") + 1]))
+    if "--square-triples" in sys.argv:
+        return add_square_triple_counts()
     if "--elements" in sys.argv:
         return add_elements(int(sys.argv[sys.argv.index("--elements") + 1]))
     if "--add" in sys.argv:
         i = sys.argv.index("--add")
         return add_run(int(sys.argv[i + 1]), int(sys.argv[i + 2]))
+    if "--add-pooled" in sys.argv:
+        i = sys.argv.index("--add-pooled")
+        return add_pooled(int(sys.argv[i + 1]), int(sys.argv[i + 2]), int(sys.argv[i + 3]),
+                          int(sys.argv[i + 4]))
     big = "--big" in sys.argv
     if not os.path.exists(S4):
         sys.exit("build the reference first: make -C oracle ref")

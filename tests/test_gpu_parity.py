@@ -50,24 +50,28 @@ def check(rec, res, lines, what=""):
     assert sha_sorted(lines) == rec["sorted_sha256"], what
 
 
-# The single-S search [4205,4206) was running when the GPU took a memory fault
-# during bring-up (a residue multiply-shift compiled to an arithmetic shift
-# indexed past a residue table; fixed, with a code-generation test in
-# test_abi.py).  The test of the first fix faulted again, so the case stays off
-# the GPU; the host emulation (tests/emu) runs it, and every wider range below
-# still covers S = 4205.
-SKIPPED_ON_GPU = {(4205, 4206)}
-
-
 def test_tiny_ranges(eng, golden):
     """Single S values and tiny windows (including S = 1..7 edge cases)."""
     for rec in golden["runs"]:
-        if (rec["lo"], rec["hi"]) in SKIPPED_ON_GPU:
-            continue
         if rec["hi"] - rec["lo"] <= 8:
             say(f"tiny [{rec['lo']},{rec['hi']})")
             res, lines = eng.search(rec["lo"], rec["hi"])
             check(rec, res, lines, (rec["lo"], rec["hi"]))
+
+
+def test_single_s_4205_regression(eng, golden):
+    """[4205, 4206): the first S with a 4-set ({3362, 482, 359, 2}).  During
+    bring-up the GPU took a memory fault on it: residues mod 11..19 computed as
+    __umul24(s, M) >> k were compiled to a 24-bit multiply followed by an
+    arithmetic shift, so a product of 2^31 or more gave a negative residue that
+    indexed far past a residue table.  Fixed with plain 32-bit multiplies and a
+    bounds check on every table index (a bad index raises ERR_BOUNDS instead of
+    loading); test_abi.py checks the compiled kernels for arithmetic shifts."""
+    rec = next(r for r in golden["runs"] if (r["lo"], r["hi"]) == (4205, 4206))
+    say("single S [4205,4206)")
+    res, lines = eng.search(4205, 4206)
+    check(rec, res, lines)
+    assert res["counters"]["4-sets"] == 1
 
 
 def test_empty_range(eng):
@@ -345,6 +349,11 @@ def test_census_1e10(eng, golden):
     say("[1,1e10)")
     res, lines = eng.search(1, 10**10)
     check(rec, res, lines)
+    # its M: lines' primitive near misses, against the reference's catalogs
+    from ps4 import nearmiss
+    near, sols = nearmiss.classify(lines)
+    assert not sols and len(near) > 18
+    assert _near_misses_in_reference_catalogs(near) > 0
 
 
 def test_census_first_thread_range_of_1e11_log(eng, golden):
@@ -500,7 +509,8 @@ def test_near_misses_of_the_census_mlines(eng, golden):
     """Every M: line of the [1,10^9) census (59, as in the reference's run) splits
     into near misses with 14 of 15 square sums (checked inside split_mline) and
     no 6-set solution (the reference finds none below 10^13); scaled copies
-    share their primitive forms (18 distinct ones here)."""
+    share their primitive forms (18 distinct ones here), each of them one of
+    the reference's primitive near misses with the same pair and Euler count."""
     from ps4 import nearmiss
     say("near misses of [1,1e9) M-lines")
     res, lines = eng.search(1, 10**9)
@@ -512,7 +522,27 @@ def test_near_misses_of_the_census_mlines(eng, golden):
     assert not sols and 0 < len(near) <= pairs
     assert all(nearmiss.split_mline(x)[1] for x in mlines)      # each line gives a near miss
     for prim, (pair, eu, S) in near.items():
-        assert set(pair) <= set(prim) and 0 <= eu <= 4 and S < 10**9
+        assert set(pair) <= set(prim) and S < 10**9
+    _near_misses_in_reference_catalogs(near)
+
+
+def _near_misses_in_reference_catalogs(near):
+    """Each primitive near miss is one of the reference's primitive near misses
+    below S = 1e12 (exhaustive/analysis_1e12.txt) with the same non-square
+    pair, and where the reference's catalogs record its Euler count
+    (exhaustive/evidence/*-compact.tar.gz::*/near-misses.json) it is the same."""
+    import json
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "near_misses.json")) as f:
+        d = json.load(f)
+    below = {tuple(e["values"]): tuple(e["nonsquare_pair"]) for e in d["below_1e12"]}
+    euler = {tuple(e["values"]): e["euler_squares"] for e in d["catalog"]}
+    n_eu = 0
+    for prim, (pair, eu, _S) in near.items():
+        assert below.get(prim) == tuple(pair), prim
+        if prim in euler:
+            assert eu == euler[prim], prim
+            n_eu += 1
+    return n_eu
 
 
 def test_serving_results_same_alone_and_in_the_mix(eng):
@@ -576,7 +606,9 @@ def test_elements_small_vs_oracle():
 def test_elements_reference_fixtures(golden):
     """N = 25000, 1e6, 1e7: the 4-sets with largest element <= N from the
     reference's --dump4 output (count and SHA-256 of the sorted lines); the
-    square-triple count against the closed form (N <= 1e6)."""
+    square-triple count against the oracle's enumeration of every root triple
+    (tests/golden/make_golden.py --square-triples), and for N <= 1e6 also
+    against a per-root-pair closed form."""
     e = ps4.Engine(S_hi_max=4 * 10**7)
     try:
         for rec in golden["elements"]:
@@ -584,6 +616,8 @@ def test_elements_reference_fixtures(golden):
             lines, ntri = e.search_elements(rec["n_max"])
             assert len(lines) == rec["n_sets"]
             assert hashlib.sha256("".join(x + "\n" for x in lines).encode()).hexdigest() == rec["sha256"]
+            # enumerated root triples (the oracle's direct count, recorded in the fixture)
+            assert ntri == rec["square_triples"]
             if rec["n_max"] <= 10**6:
                 assert ntri == _closed_form_square_triples(rec["n_max"])
     finally:

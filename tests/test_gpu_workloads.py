@@ -1,0 +1,212 @@
+"""The workloads at their stated sizes, against the reference -- needs an
+MI355X (-m gpu).
+
+  * the census [1, 1e11) against exhaustive/s4.c (a pooled golden run of the
+    reference, tests/golden/make_golden.py --add-pooled);
+  * the band [9e11, 1e12) in batched form, and every other 1e11-wide slice
+    [k*1e11, (k+1)*1e11) of [1e11, 1e12), against the reference's own logs
+    (exhaustive/logs/r12_k.err + boundary_k.err), each subrange against the
+    plain form, and a seeded sample of 1e8-wide subranges against s4.c;
+  * the 4096 uneven subranges of [1, 1e11] in one batched call, every one
+    against its own plain search, 32 seeded ones against s4.c;
+  * the drop-in CLI killed after a checkpoint and resumed
+    (A::sources/exhaustive/test_preflight.py:49-66).
+All through the C ABI or the CLI built on it; bit-exact.
+"""
+import os
+import random
+import re
+import signal
+import subprocess
+import sys
+import time
+
+import pytest
+
+import ps4
+from conftest import sha_sorted
+from ps4 import workloads
+
+pytestmark = pytest.mark.gpu
+
+S_HI_MAX = 10**12 + 1
+
+
+def say(msg):
+    sys.__stderr__.write(f"[gpu-test] {msg}\n")
+    sys.__stderr__.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    say("open engine S_hi_max=1e12+1")
+    e = ps4.Engine(S_hi_max=S_HI_MAX)
+    yield e
+    e.close()
+
+
+def _sum(results):
+    tot, hist = {}, {}
+    for c, _ in results:
+        for k, v in c["counters"].items():
+            tot[k] = tot.get(k, 0) + v
+        for k, v in c["hist5"].items():
+            hist[k] = hist.get(k, 0) + v
+    return tot, hist
+
+
+def _run(golden, lo, hi):
+    return next((r for r in golden["runs"] if (r["lo"], r["hi"]) == (lo, hi)), None)
+
+
+def test_census_1e11_vs_s4(eng, golden):
+    """[1, 1e11): every counter, the 5-set histogram, the capacity maxima, the
+    number of lines and the SHA-256 of the byte-sorted lines against
+    exhaustive/s4.c run over the same range (the bit-exact target of the
+    census)."""
+    rec = _run(golden, 1, 10**11)
+    if rec is None:
+        pytest.skip("no s4 golden for [1,1e11) in tests/golden/golden.json")
+    say("[1,1e11) vs s4.c")
+    res, lines = eng.search(1, 10**11)
+    assert res["counters"] == rec["counters"]
+    assert res["hist5"] == rec["hist5"]
+    assert res["capacity"] == rec["capacity"]
+    assert len(lines) == rec["n_lines"]
+    assert sha_sorted(lines) == rec["sorted_sha256"]
+    # the reference's own log of the same range (exhaustive/logs/s4d_1e11.err:68)
+    # comes from an earlier build; the counters up to the triples agree with it
+    log = next(x for x in golden["reference_logs"] if x["source"].endswith("s4d_1e11.err"))
+    for k in ("S_done", "pairs", "S_ge3", "triples", "6-sets"):
+        assert res["counters"][k] == log["counters"][k], k
+
+
+def _slice_logs(golden, k):
+    logs = {r["source"].rsplit("/", 1)[-1]: r for r in golden["reference_logs"]}
+    a, b = logs[f"r12_{k}.err"], logs[f"boundary_{k}.err"]
+    assert b["lo"] == k * 10**11 and b["hi"] == a["lo"] == k * 10**11 + 1 and a["hi"] == (k + 1) * 10**11
+    tot = {key: a["counters"][key] + b["counters"][key] for key in a["counters"]}
+    hist = dict(a["hist5"])
+    for key, v in b["hist5"].items():
+        hist[key] = hist.get(key, 0) + v
+    return tot, hist
+
+
+def test_band_9e11_1e12_batched(eng, golden):
+    """The large band [9e11, 1e12) as 1000 subranges of 1e8 in one batched
+    call: the summed counters and 5-set histogram equal the reference's logs
+    of the band (exhaustive/logs/r12_9.err:109-110 + boundary_9.err:3); every
+    subrange equals the plain form of the same subrange; the 5-sets whose
+    largest element is <= 1e12 are counted from the lines."""
+    lo, hi = 9 * 10**11, 10**12
+    sub = workloads.band_ranges(lo, hi, 10**8)
+    say(f"band [{lo},{hi}) as {len(sub)} subranges, one batched call")
+    got = eng.search_batch([a for a, _ in sub], [b for _, b in sub])
+    tot, hist = _sum(got)
+    ref, ref_hist = _slice_logs(golden, 9)
+    assert tot == ref
+    assert hist == ref_hist
+    say("band: every subrange against the plain form")
+    for (a, b), (c, lines) in zip(sub, got):
+        alone, al = eng.search(a, b)
+        assert alone == c, (a, b)
+        assert sorted(al) == sorted(lines), (a, b)
+    n5 = sum(1 for _, lines in got for ln in lines if ln.startswith("5:"))
+    assert n5 == tot["5-sets"]
+    n5_le = sum(1 for _, lines in got for ln in lines
+                if ln.startswith("5:") and int(ln[4:].split(",", 1)[0]) <= 10**12)
+    assert 0 < n5_le < n5
+
+
+def test_band_sample_vs_s4(eng, golden):
+    """A seeded sample of the band's 1e8-wide subranges against exhaustive/s4.c
+    (tests/golden/make_golden.py --band-sample): counters, histogram,
+    capacity, lines."""
+    recs = golden.get("band_sample", {}).get("runs", [])
+    if not recs:
+        pytest.skip("no band sample in tests/golden/golden.json")
+    for rec in recs:
+        say(f"band sample [{rec['lo']},{rec['hi']}) vs s4.c")
+        res, lines = eng.search(rec["lo"], rec["hi"])
+        assert res["counters"] == rec["counters"]
+        assert res["hist5"] == rec["hist5"]
+        assert res["capacity"] == rec["capacity"]
+        assert len(lines) == rec["n_lines"] and sha_sorted(lines) == rec["sorted_sha256"]
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_slices_of_1e11_to_1e12(eng, golden, k):
+    """[k*1e11, (k+1)*1e11) as 1000 subranges of 1e8 in one batched call: the
+    summed counters and 5-set histogram against the reference's logs of that
+    slice (exhaustive/logs/r12_k.err + boundary_k.err)."""
+    lo, hi = k * 10**11, (k + 1) * 10**11
+    sub = workloads.band_ranges(lo, hi, 10**8)
+    say(f"slice [{lo},{hi}) vs r12_{k} + boundary_{k}")
+    got = eng.search_batch([a for a, _ in sub], [b for _, b in sub])
+    tot, hist = _sum(got)
+    ref, ref_hist = _slice_logs(golden, k)
+    assert tot == ref
+    assert hist == ref_hist
+
+
+def test_batched_4096_uneven_subranges(eng, golden):
+    """The batched workload at its stated size: 4096 subranges of [1, 1e11],
+    lengths log-uniform in [1e4, 1e9] (splitmix64 seed 42), in one batched
+    call.  Every subrange equals its own plain search; a seeded sample of 32
+    equals exhaustive/s4.c (tests/golden/make_golden.py --batched-sample)."""
+    ranges = workloads.batched_ranges(4096, seed=42, top=10**11)
+    say(f"batched: {len(ranges)} subranges, {sum(b - a for a, b in ranges):.3e} S in one call")
+    got = eng.search_batch([a for a, _ in ranges], [b for _, b in ranges])
+    say("batched: every subrange against its own search")
+    for (a, b), (c, lines) in zip(ranges, got):
+        alone, al = eng.search(a, b)
+        assert alone == c, (a, b)
+        assert sorted(al) == sorted(lines), (a, b)
+    sample = golden.get("batched_sample", {}).get("runs", [])
+    assert len(sample) >= 32, "s4 goldens for the seeded sample are missing"
+    for rec in sample:
+        c, lines = got[rec["index"]]
+        assert (rec["lo"], rec["hi"]) == ranges[rec["index"]]
+        assert c["counters"] == rec["counters"], rec["index"]
+        assert c["hist5"] == rec["hist5"], rec["index"]
+        assert c["capacity"] == rec["capacity"], rec["index"]
+        assert len(lines) == rec["n_lines"] and sha_sorted(lines) == rec["sorted_sha256"], rec["index"]
+
+
+def test_cli_kill_after_checkpoint_then_resume(tmp_path, golden):
+    """The reference's restart drill (A::sources/exhaustive/test_preflight.py:49-66):
+    start a census with a checkpoint file, wait for the first checkpoint,
+    SIGTERM the process, rerun with --resume; the checkpoint then ends in DONE,
+    the TOTAL counters equal s4.c's for the range and the set of result lines
+    equals s4.c's."""
+    lo, hi, T = 1, 10**10, 16
+    rec = _run(golden, lo, hi)
+    out, ck = tmp_path / "r.out", tmp_path / "r.ckpt"
+    args = [ps4.CLI_PATH, str(lo), str(hi), str(T), "20", "--out", str(out), "--ckpt", str(ck)]
+    env = dict(os.environ, PS4_CKPT_SECS="0")    # a checkpoint after every block
+    say("cli drill: run until the first checkpoint, then SIGTERM")
+    with open(tmp_path / "first.err", "w") as err:
+        p = subprocess.Popen(args, cwd=tmp_path, env=env, stdout=subprocess.DEVNULL, stderr=err)
+        deadline = time.monotonic() + 60
+        while not ck.exists():
+            assert p.poll() is None, "finished before its first checkpoint"
+            assert time.monotonic() < deadline
+            time.sleep(0.01)
+        assert "DONE" not in ck.read_text()
+        p.send_signal(signal.SIGTERM)
+        p.wait(timeout=30)
+    first = ck.read_text().splitlines()
+    assert first[0] == "s4ckpt v1" and first[-1] != "DONE"
+    say("cli drill: resume")
+    r = subprocess.run(args + ["--resume"], cwd=tmp_path, env=env, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.PIPE, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    err = r.stderr.decode()
+    assert "RESUMED from checkpoint" in err
+    assert ck.read_text().splitlines()[-1] == "DONE"
+    tot = {k: int(v) for k, v in re.findall(r"([\w-]+)=(\d+)", re.search(r"^TOTAL: (.*)$", err, re.M).group(1))}
+    for key, v in rec["counters"].items():
+        assert tot[key] == v, key
+    lines = set(out.read_text().splitlines())
+    assert len(lines) == rec["n_lines"]
+    assert sha_sorted(lines) == rec["sorted_sha256"]

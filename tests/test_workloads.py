@@ -93,3 +93,56 @@ def test_split_rejects_bad_lines():
                 KNOWN_M.replace("4431625200", "4431625201").replace("S=50578642000", "S=50578642001")):
         with pytest.raises(ValueError):
             nearmiss.split_mline(bad)
+
+
+# ---- near-miss parity against the reference's own catalogs
+# (tests/golden/near_misses.json, written by tests/golden/make_nearmiss_fixture.py
+#  from exhaustive/evidence/*-compact.tar.gz::*/near-misses.json and
+#  exhaustive/analysis_1e12.txt)
+import json  # noqa: E402
+
+NEAR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "near_misses.json")
+
+
+def _near():
+    with open(NEAR) as f:
+        return json.load(f)
+
+
+def test_euler_count_matches_reference_catalog():
+    """euler_count on every distinct entry of the reference's near-miss
+    catalogs equals the euler_squares the reference's auditor recorded."""
+    d = _near()
+    assert len(d["catalog"]) >= 800
+    counts = {}
+    for e in d["catalog"]:
+        got = nearmiss.euler_count(tuple(e["values"]), tuple(e["nonsquare_pair"]))
+        assert got == e["euler_squares"], e["values"]
+        counts[got] = counts.get(got, 0) + 1
+    assert len(counts) >= 3          # the catalog holds several different counts
+
+
+def test_catalog_entries_are_primitive_near_misses():
+    """Each catalog and analysis entry: fourteen of fifteen pair sums square,
+    the one that is not is the recorded pair, primitive (no square > 1 divides
+    the gcd), square_total as recorded."""
+    d = _near()
+    for e in d["catalog"] + d["below_1e12"]:
+        v = e["values"]
+        bad = [(a, b) for i, a in enumerate(v) for b in v[i + 1:] if not nearmiss.is_square(a + b)]
+        assert bad == [tuple(e["nonsquare_pair"])], v
+        assert nearmiss.square_part(math.gcd(*v)) == 1, v
+        assert nearmiss.is_square(sum(v)) == e["square_total"], v
+    a = d["analysis_1e12"]
+    assert len(d["below_1e12"]) == a["primitive_six_member_near_misses"] == 430
+    assert sum(e["square_total"] for e in d["below_1e12"]) == a["near_misses_with_square_total"]
+
+
+def test_known_mline_primitives_in_reference_analysis():
+    """The primitives of the reference's known M: line are among the
+    primitive near misses of exhaustive/analysis_1e12.txt with the same
+    non-square pairs."""
+    below = {tuple(e["values"]): tuple(e["nonsquare_pair"]) for e in _near()["below_1e12"]}
+    _, near, _ = nearmiss.split_mline(KNOWN_M)
+    for prim, pair in near.items():
+        assert below.get(prim) == pair, prim
