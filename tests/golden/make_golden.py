@@ -116,12 +116,23 @@ def add_run(lo, hi):
 
 def _piece(args):
     lo, hi = args
+    cache = os.environ.get("GOLDEN_PIECE_CACHE")   # a directory: finished pieces survive a restart
+    if cache:
+        path = os.path.join(cache, f"{lo}_{hi}.json")
+        if os.path.exists(path):
+            with open(path) as f:
+                d = json.load(f)
+            return lo, hi, d["err"], [x.encode() for x in d["lines"]]
     with tempfile.TemporaryDirectory() as tmp:
         outp = os.path.join(tmp, "o.out")
         res = subprocess.run([S4, str(lo), str(hi), "1", "20", "--out", outp], cwd=tmp,
                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, check=True)
         with open(outp, "rb") as f:
             lines = f.read().splitlines(keepends=True)
+    if cache:
+        with open(path + ".tmp", "w") as f:
+            json.dump({"err": res.stderr.decode(), "lines": [x.decode() for x in lines]}, f)
+        os.replace(path + ".tmp", path)
     return lo, hi, res.stderr.decode(), lines
 
 

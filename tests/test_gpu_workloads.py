@@ -179,11 +179,13 @@ def test_cli_kill_after_checkpoint_then_resume(tmp_path, golden):
     SIGTERM the process, rerun with --resume; the checkpoint then ends in DONE,
     the TOTAL counters equal s4.c's for the range and the set of result lines
     equals s4.c's."""
-    lo, hi, T = 1, 10**10, 16
+    lo, hi, T = 1, 10**11, 16
     rec = _run(golden, lo, hi)
+    if rec is None:
+        pytest.skip("no s4 golden for [1,1e11) in tests/golden/golden.json")
     out, ck = tmp_path / "r.out", tmp_path / "r.ckpt"
     args = [ps4.CLI_PATH, str(lo), str(hi), str(T), "20", "--out", str(out), "--ckpt", str(ck)]
-    env = dict(os.environ, PS4_CKPT_SECS="0")    # a checkpoint after every block
+    env = dict(os.environ, PS4_CKPT_SECS="0")    # a checkpoint after every block (a few seconds in all)
     say("cli drill: run until the first checkpoint, then SIGTERM")
     with open(tmp_path / "first.err", "w") as err:
         p = subprocess.Popen(args, cwd=tmp_path, env=env, stdout=subprocess.DEVNULL, stderr=err)
