@@ -708,36 +708,37 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
 #define EX_ARENA 864      // doubles of divisor lists per pass (the workgroup's LDS stays at
                           // 16 KB); one 4-set with d(D) <= 16384 needs at most 748 (nL + nH
                           // over the best prefix split, reached by the radices 2, 11, 33, 2, 11)
-#define EX_MAXF 12        // distinct odd primes of D < 2^51 (3*5*...*43 > 2^51)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
 #define EX_CHUNK 4        // tiles claimed per work-queue atomic
 #define EX_Q1 128         // LDS stack of candidates past the residue filters (a round of 64
                           // runs once it holds >= 64; each push adds at most 64)
-#define EX_WG_SCRATCH (EX_Q2 + EX_MAXF * EX_NT / 4)   // 16-byte words of global scratch per
-                          // workgroup: the exact-test queue, then the factor lists
 #define EX_Q2 128         // queue of candidates for the exact test, per workgroup in global
                           // memory, kept across passes and tiles (drained at >= 64 and at
                           // the end; a round of the first queue adds at most 64)
+#define EX_OSLOT 7        // distinct primes of an odd n < 2^26 (3*5*7*...*23 > 2^26)
+#define EX_NPOS 15        // stage positions of D': the power of two, then EX_OSLOT slots for the
+                          // odd part of ra - rb and EX_OSLOT for that of ra + rb
 
-// Two steps of the factorisation of n by the smallest-prime table: f1 = the
-// entry of n, f2 = the entry of n's cofactor (0 when that is 1), each
-// cofactor | smallest prime << 32 | its exponent << 59.
-struct __align__(16) ExFac {
-    u64 f1, f2;
+// The odd-factor table: for odd n < tab_n, entry (n - 1) / 2 is two 16-byte
+// words {w0..w7}: w0..w6 the distinct prime powers of n as p << 5 | e in
+// ascending order of p (0 past the last), w7 = k | d(n) << 4 with k the
+// number of distinct primes.  One gather factors an odd part completely.
+struct __align__(16) ExOdd {
+    uint4 a, b;
 };
 
 struct ExArgs {
     const ps4_quad *quads;
     u64 quad_cap;
     ps4_ctl *ctl;
-    const ExFac *fpair;     // n -> the first two steps of n's factorisation
-    u64 tab_n;              // entries in ftab / ndiv
+    const ExOdd *oftab;     // odd n -> its factorisation
+    u64 tab_n;              // entries in ndiv (n < tab_n); oftab holds the odd n below it
     const uint16_t *ndiv;   // n -> d(n)
     const ps4_qrtab *qr;
     u64 *ctr;
     ps4_hit *hits;
-    uint4 *exq;             // EX_WG_SCRATCH words per workgroup: EX_Q2 entries {u (52 bits) |
-                            // pair choice << 52, 4-set index}, then EX_MAXF * EX_NT factor words
+    uint4 *exq;             // EX_Q2 words per workgroup: entries {u (52 bits) | pair choice << 52,
+                            // 4-set index}
     u32 hit_cap;
     u32 cap_divs;
     u32 n_rids;             // counter rows in ctr
@@ -901,29 +902,76 @@ __device__ __forceinline__ u32 ext_exact(double u, const ps4_quad *quads, u64 qi
     return 0;
 }
 
-// Add p^e to a lane's factor list (slots [slot][lane] in fac), merging a
-// prime already present; `seen` is a 64-bit filter of the primes stored so far,
-// `exps` the exponents of the slots, 5 bits each (an odd prime's exponent in
-// D = (ra - rb)(ra + rb) stays below 32 for roots below 2^25).
-__device__ __forceinline__ void ex_add_factor(u32 *fac, u32 tid, u32 p, u32 e, u32 &nf, u64 &seen, u64 &exps,
-                                              u32 &err)
+// The entry of odd n in the odd-factor table (n < tab_n, checked by the caller).
+__device__ __forceinline__ ExOdd ex_odd_at(const ExOdd *oftab, u32 n)
 {
-    const u64 bit = 1ull << (p & 63u);
-    u32 sl = nf;
-    if (seen & bit)
-        for (u32 i = 0; i < nf; ++i)
-            if ((fac[i * EX_NT + tid] >> 5) == p) sl = i;
-    if (sl < nf) {
-        fac[sl * EX_NT + tid] += e;
-        exps += (u64)e << (5u * sl);
-    } else if (nf < EX_MAXF) {
-        fac[nf * EX_NT + tid] = (p << 5) | e;
-        exps |= (u64)e << (5u * nf);
-        ++nf;
-        seen |= bit;
-    } else {
-        err |= ERR_FACTORS;
+    return oftab[n >> 1];
+}
+
+// Does p divide the odd o (p, o < 2^26)?  The refined reciprocal has relative
+// error far below 2^-40, so when p | o the rounded quotient is o / p exactly,
+// and the fused multiply-add then gives 0 exactly; otherwise it cannot.
+__device__ __forceinline__ bool ex_divides(u32 p, double o)
+{
+    const double dp = (double)p;
+    double r = __builtin_amdgcn_rcp(dp);
+    r = __builtin_fma(__builtin_fma(-dp, r, 1.0), r, r);
+    const double q = __builtin_rint(o * r);
+    return p > 1u && __builtin_fma(-q, dp, o) == 0.0;
+}
+
+// The prime powers of D' in stage order, as words p << 5 | e (0: no stage):
+// position 0 the power of two 2^e2p, positions 1..7 those of the odd part o1
+// of ra - rb (entry f1), positions 8..14 those of the odd part of ra + rb
+// (entry f2) whose prime does not divide o1.  A prime dividing both (it then
+// divides ra and rb) keeps one stage, in o1's position, with the two exponents
+// added.  *nd gets d of the odd part of D.
+__device__ __forceinline__ void ex_stages(const ExOdd &f1, const ExOdd &f2, u32 o1, u32 e2p, u32 st[EX_NPOS], u32 *nd)
+{
+    const u32 w1[EX_OSLOT] = {f1.a.x, f1.a.y, f1.a.z, f1.a.w, f1.b.x, f1.b.y, f1.b.z};
+    const u32 w2[EX_OSLOT] = {f2.a.x, f2.a.y, f2.a.z, f2.a.w, f2.b.x, f2.b.y, f2.b.z};
+    st[0] = e2p ? (2u << 5) | e2p : 0u;
+#pragma unroll
+    for (int i = 0; i < EX_OSLOT; ++i) st[1 + i] = w1[i];
+    u32 d = (f1.b.w >> 4) * (f2.b.w >> 4);
+    const double o = (double)o1;
+#pragma unroll
+    for (int j = 0; j < EX_OSLOT; ++j) {
+        u32 w = w2[j];
+        if (ex_divides(w >> 5, o)) {   // a prime of both parts (rare)
+            const u32 p = w >> 5, eb = w & 31u;
+#pragma unroll
+            for (int i = 0; i < EX_OSLOT; ++i) {
+                const u32 v = st[1 + i];
+                if ((v >> 5) == p) {
+                    // d: the factors (ea + 1)(eb + 1) of the two parts become ea + eb + 1
+                    const u32 ea = v & 31u;
+                    d = d / ((ea + 1u) * (eb + 1u)) * (ea + eb + 1u);
+                    st[1 + i] = v + eb;
+                }
+            }
+            w = 0u;
+        }
+        st[1 + EX_OSLOT + j] = w;
     }
+    *nd = d;
+}
+
+// The largest value of v (0..7) over the wave, uniform.
+__device__ __forceinline__ u32 wave_max7(u32 v)
+{
+    u32 m = 0;
+    for (u32 t = 7; t > 0; --t)
+        if (__ballot(v >= t) != 0ull) { m = t; break; }
+    return m;
+}
+
+// Stage position s is walked by the loops below when it can hold a stage of
+// some lane: the power of two, and the first n1 / n2 slots of the two odd parts
+// (n1, n2: the largest numbers of distinct primes of the parts in the wave).
+__device__ __forceinline__ bool ex_pos_live(int s, u32 n1, u32 n2)
+{
+    return s == 0 || (s <= EX_OSLOT ? (u32)(s - 1) < n1 : (u32)(s - 1 - EX_OSLOT) < n2);
 }
 
 // One candidate: 4-set record k of the pass, candidate index t of that 4-set.
@@ -1040,11 +1088,11 @@ __device__ __forceinline__ void ex_flush(u64 *ctr, u32 n_rids, u32 rid, u64 cand
 }
 
 // Workgroup = one wave; a tile is 64 queued 4-sets, one per lane for the setup:
-//   validation, pair choice, factorisation of D by the smallest-prime table
-//   (the two factors ra - rb and ra + rb walked side by side), d(D), the
-//   candidate count ncand = d(D')/2 (exhaustive/s4.c:172-186), the split of
-//   the factor sequence into L and H parts, residue masks, e > 0 and e-member
-//   bounds.
+//   validation, pair choice, the factorisation of D = 2^e2 * o1 * o2 (o1, o2
+//   the odd parts of ra - rb and ra + rb) by one gather of each odd part's
+//   entry in the odd-factor table, d(D), the candidate count ncand = d(D')/2
+//   (exhaustive/s4.c:172-186), the split of the stage sequence into L and H
+//   parts, residue masks, e > 0 and e-member bounds.
 // Then passes: the lists L and H (all divisors are L[i]*H[j]) of as many
 // 4-sets as the arena holds, and their candidates spread over the lanes with
 // stride 64 (lane l takes candidates l, l+64, ...; the 4-set of a candidate
@@ -1055,12 +1103,6 @@ __device__ __forceinline__ void ex_flush(u64 *ctr, u32 n_rids, u32 rid, u64 cand
 // unordered pair f != g exactly once, and f + g needs no ordering.
 // Counters are summed per wave over consecutive tiles of one interval and
 // added to the device counters when the interval changes and at the end.
-// kLdsFac: the lanes' factor lists in LDS (16 KB per workgroup, 10 waves per
-// CU) rather than in the workgroup's global scratch (13 KB, 12 waves per CU).
-// The host takes the LDS form for batches with S above about 3*10^10, where
-// the larger factor tables make the scratch's L2 traffic cost more than the
-// two extra waves gain (-2.7 % time at S ~ 3*10^11, +1.7 % below 10^10).
-template <bool kLdsFac>
 __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIMD: <= 168 VGPRs
 {
     __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
@@ -1078,16 +1120,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
     const u64 ntiles = (nquads + EX_NT - 1) / EX_NT;
     const u64 nchunks = (ntiles + EX_CHUNK - 1) / EX_CHUNK;
     const ps4_qrtab *qr = a.qr;
-    uint4 *exq2 = a.exq + (u64)blockIdx.x * EX_WG_SCRATCH;   // candidates for the exact test
-    // odd prime factors of D, p << 5 | e, [slot][lane]: in LDS, or in the
-    // workgroup's global scratch (L2-resident; each lane reads only what it wrote)
-    u32 *fac;
-    if constexpr (kLdsFac) {
-        __shared__ u32 s_fac[EX_MAXF * EX_NT];
-        fac = s_fac;
-    } else {
-        fac = (u32 *)(exq2 + EX_Q2);
-    }
+    uint4 *exq2 = a.exq + (u64)blockIdx.x * EX_Q2;   // candidates for the exact test
     u32 qn2 = 0;                                     // entries in it
 
     // this wave's counters of interval acc_rid (lane partial sums)
@@ -1134,7 +1167,11 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
             }
             u64 my_cand = 0, my_maxdiv = 0;
             u32 my_esq = 0, my_unc = 0;   // ext_sqrt: residue-filter passes, minus the queued ones that fail the rest
-            u32 nf = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0, best = 0;
+            u32 e2 = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0, ndp = 0, best = 0;
+            u32 o1 = 1, o2 = 1, n1 = 0, n2 = 0;   // odd parts of ra - rb, ra + rb and their numbers of primes
+            u32 st[EX_NPOS];                      // stage words of D' (ex_stages)
+#pragma unroll
+            for (int q = 0; q < EX_NPOS; ++q) st[q] = 0u;
             u64 pm64 = 0, pm63 = 0, pm65 = 0, pmsm = 0;
             u32 psqA = 0, pf1 = 0, pf2 = 0, pf3 = 0, pf4 = 0;
             if (valid) {
@@ -1174,19 +1211,16 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                         if (e < bestv) { bestv = e; best = (u32)t; }
                     }
                     const ExPair P = ex_pair(best, va, vb, vc, vd, qd.r_ab, qd.r_ac, qd.r_ad, qd.r_bc, qd.r_bd, qd.r_cd);
-                    // D = m1 * m2: strip the twos; the first table entries of both odd
-                    // parts are read now, ahead of the residue tables
-                    u32 m1 = P.ra - P.rb, m2 = P.ra + P.rb;
+                    // D = m1 * m2 = 2^e2 * o1 * o2: the odd parts are factored by one
+                    // gather each, issued ahead of the residue tables
+                    const u32 m1 = P.ra - P.rb, m2 = P.ra + P.rb;
                     const u32 z1 = (u32)__builtin_ctz(m1), z2 = (u32)__builtin_ctz(m2);
-                    const u32 e2 = z1 + z2;
-                    m1 >>= z1;
-                    m2 >>= z2;
-                    auto fac_at = [&](u32 m) -> ExFac {
-                        return m > 1 && m < a.tab_n ? *(const ExFac *)((const char *)a.fpair + 16u * m) : ExFac{0ull, 0ull};
-                    };
-                    ExFac en1 = fac_at(m1), en2 = fac_at(m2);
-                    // quadratic-residue masks of this 4-set (read before the factor walk,
-                    // so that the reads overlap it): bit r of M_m set iff
+                    e2 = z1 + z2;
+                    o1 = m1 >> z1;
+                    o2 = m2 >> z2;
+                    const ExOdd f1 = ex_odd_at(a.oftab, o1), f2 = ex_odd_at(a.oftab, o2);
+                    // quadratic-residue masks of this 4-set (read together with the
+                    // odd-factor entries, so that the reads overlap): bit r of M_m set iff
                     // r^2 + (C1-A) and r^2 + (C2-A) are both squares mod m (the
                     // filters of exhaustive/s4.c:192-194)
                     const double d1 = (double)(i64)(P.C1 - P.A), d2 = (double)(i64)(P.C2 - P.A);
@@ -1218,75 +1252,51 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     pf2 = P.F2;
                     pf3 = P.F3;
                     pf4 = (s2a * s2a == 2 * P.A) ? (u32)s2a : 0u;   // u >= 2: 0 never matches
-                    // the walk of D = m1 * m2 through the table (below): both odd
-                    // parts side by side (two independent load chains)
-                    u64 seen = 0, exps = 0;
-                    for (int guard = 0; m1 > 1 || m2 > 1; ++guard) {
-                        if (m1 >= a.tab_n || m2 >= a.tab_n || guard >= 64) { err |= ERR_FACTORS; break; }   // never for a valid 4-set
-                        // two prime powers of each part per step
-                        if (m1 > 1) {
-                            ex_add_factor(fac, tid, (u32)(en1.f1 >> 32) & 0x7FFFFFFu, (u32)(en1.f1 >> 59), nf, seen, exps, err);
-                            m1 = (u32)en1.f1;
-                            if (m1 > 1) {
-                                ex_add_factor(fac, tid, (u32)(en1.f2 >> 32) & 0x7FFFFFFu, (u32)(en1.f2 >> 59), nf, seen,
-                                              exps, err);
-                                m1 = (u32)en1.f2;
-                            }
-                        }
-                        if (m2 > 1) {
-                            ex_add_factor(fac, tid, (u32)(en2.f1 >> 32) & 0x7FFFFFFu, (u32)(en2.f1 >> 59), nf, seen, exps, err);
-                            m2 = (u32)en2.f1;
-                            if (m2 > 1) {
-                                ex_add_factor(fac, tid, (u32)(en2.f2 >> 32) & 0x7FFFFFFu, (u32)(en2.f2 >> 59), nf, seen,
-                                              exps, err);
-                                m2 = (u32)en2.f2;
-                            }
-                        }
-                        en1 = fac_at(m1);
-                        en2 = fac_at(m2);
-                    }
                     pm64 = t64a & t64b;
                     pm63 = t63a & t63b;
                     pm65 = t65a & t65b;
                     pmsm = (t65c & t65d & 1ull) | ((u64)(t11a & t11b) << 1) | ((u64)(t13a & t13b) << 12) |
                            ((u64)(t17a & t17b) << 25) | ((u64)(t19a & t19b) << 42);
-                    u32 ndodd = 1;   // d of the odd part (< 2^32 for D < 2^51)
-                    for (u32 i = 0; i < nf; ++i) ndodd *= (u32)(exps >> (5u * i)) % 32u + 1u;
-                    const u64 ndD = (u64)(e2 + 1) * ndodd;
-                    my_maxdiv = ndD;
-                    if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
                     // Divisor pairs f < g of D' = D (D odd, u = (f+g)/2) or D/4
                     // (4 | D, u = f+g); none when D = 2 mod 4.  Their number is the
                     // reference's ext_cand.
+                    half = (e2 == 0);
+                    e2p = e2 >= 2 ? e2 - 2 : 0u;
+                    u32 ndodd;   // d of the odd part (< 2^32 for D < 2^51)
+                    ex_stages(f1, f2, o1, e2p, st, &ndodd);
+                    n1 = f1.b.w & 15u;
+                    n2 = f2.b.w & 15u;
+                    const u64 ndD = (u64)(e2 + 1) * ndodd;
+                    my_maxdiv = ndD;
+                    if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
                     if (e2 != 1 && ndD <= a.cap_divs) {
-                        half = (e2 == 0);
-                        e2p = half ? 0u : e2 - 2;
-                        const u32 ndp = (u32)((u64)(e2p + 1) * ndodd);
+                        ndp = (e2p + 1) * ndodd;
                         ncand = ndp / 2;
                         my_cand = ncand;
-                        if (ncand > 0) {
-                            // split the factor sequence (2 first when e2' > 0, then
-                            // the odd primes) into a prefix L and the rest H,
-                            // minimising nL + nH; ndp / nl is exact (nl divides
-                            // ndp < 2^24) by a rounded single-precision quotient
-                            const u32 nfa = nf + (e2p ? 1u : 0u);
-                            const float fndp = (float)ndp;
-                            u32 nl = 1, bestc = 1 + ndp, bestnl = 1;
-                            for (u32 s = 1; s <= nfa; ++s) {
-                                const u32 ee = e2p ? (s == 1 ? e2p : (u32)(exps >> (5u * (s - 2u))) % 32u)
-                                                   : (u32)(exps >> (5u * (s - 1u))) % 32u;
-                                nl *= ee + 1;
-                                const u32 nh = (u32)__builtin_rintf(fndp * __builtin_amdgcn_rcpf((float)nl));
-                                const u32 c = nl + nh;
-                                if (c < bestc) { bestc = c; sL = s; bestnl = nl; }
-                            }
-                            nL = bestnl;
-                            nH = bestc - bestnl;
-                            lsize = bestc;
-                            if (nL * nH != ndp || ncand > EX_CMAX) { err |= ERR_FACTORS; ncand = 0; }   // never
-                        }
                     }
                 }
+            }
+            // Split the stage sequence into a prefix L (positions < sL) and the
+            // rest H, minimising |L| + |H| = nL + nH over the prefixes; ndp / nl
+            // is exact (nl divides ndp < 2^24) by a rounded single-precision
+            // quotient.  Positions no lane of the wave uses are skipped (uniform).
+            const u32 n1max = wave_max7(n1), n2max = wave_max7(n2);
+            if (ncand > 0) {
+                const float fndp = (float)ndp;
+                u32 nl = 1, bestc = 1 + ndp, bestnl = 1;
+                sL = 0;
+#pragma unroll
+                for (int q = 0; q < EX_NPOS; ++q) {
+                    if (!ex_pos_live(q, n1max, n2max)) continue;
+                    nl *= (st[q] & 31u) + 1u;
+                    const u32 nh = (u32)__builtin_rintf(fndp * __builtin_amdgcn_rcpf((float)nl));
+                    const u32 c = nl + nh;
+                    if (c < bestc) { bestc = c; sL = (u32)q + 1u; bestnl = nl; }
+                }
+                nL = bestnl;
+                nH = bestc - bestnl;
+                lsize = bestc;
+                if (nL * nH != ndp || nl != ndp || ncand > EX_CMAX) { err |= ERR_FACTORS; ncand = 0; }   // never
             }
 
             // the next tile's entries: loaded now, used after this tile's passes
@@ -1303,6 +1313,14 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
             //      then all their candidates spread over the lanes
             bool pending = ncand > 0;
             while (__ballot(pending) != 0ull) {
+                // the odd-factor entries again (cheaper than keeping the stage words
+                // in registers across the candidate loop): issued before the scans
+                // and the barrier, used by the list building after them
+                ExOdd g1, g2;
+                if (pending) {
+                    g1 = ex_odd_at(a.oftab, o1);
+                    g2 = ex_odd_at(a.oftab, o2);
+                }
                 u32 ltot, ptot, ctot;
                 const u32 loff = wave_excl_scan(pending ? lsize : 0u, &ltot);
                 const u32 coff = wave_excl_scan(pending ? ncand : 0u, &ptot);
@@ -1325,39 +1343,35 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     s_mem[slot] = make_uint4(psqA, pf1, pf2, pf3);
                     s_aux[slot] = pf4 | (tid << 26);
                     if (coff > 0u) atomicOr(&s_bits[(coff - 1u) >> 5], 1u << ((coff - 1u) & 31u));
-                    // mixed-radix lists over the factor sequence: stage s multiplies
-                    // the list so far by p_s^1 .. p_s^e (earlier stages are the less
-                    // significant digits, so index n and n_max - n are complements).
-                    // When u = (f + g) / 2 the H list holds halves (exact doubles),
-                    // so that u = L[i] * H[j] + L[i'] * H[j'] in both cases.
+                    // mixed-radix lists over the stages: stage s multiplies the list
+                    // so far by p_s^1 .. p_s^e (earlier stages are the less
+                    // significant digits, so index n and n_max - n are complements);
+                    // positions before sL build L, the rest H.  When u = (f + g) / 2
+                    // the H list holds halves (exact doubles), so that
+                    // u = L[i] * H[j] + L[i'] * H[j'] in both cases.
+                    u32 sw[EX_NPOS], nd2;
+                    ex_stages(g1, g2, o1, e2p, sw, &nd2);
                     const double h0 = half ? 0.5 : 1.0;
-                    const u32 nfa = nf + (e2p ? 1u : 0u);
                     double *lst = s_ar + loff;
                     u32 n = 1;
                     lst[0] = 1.0;
-                    const u32 o2 = e2p ? 1u : 0u;              // stage of the first odd prime
-                    u32 wn = nf > 0 ? fac[tid] : 0u;         // its factor word (read ahead)
-                    for (u32 s = 0; s < nfa; ++s) {
-                        if (s == sL) {   // start the H list
+                    bool inH = false;
+#pragma unroll
+                    for (int q = 0; q < EX_NPOS; ++q) {
+                        if (!ex_pos_live(q, n1max, n2max)) continue;
+                        if (!inH && (u32)q >= sL) {   // start the H list
                             lst = s_ar + loff + nL;
                             n = 1;
                             lst[0] = h0;
+                            inH = true;
                         }
-                        u32 p, e;
-                        if (e2p && s == 0) {
-                            p = 2;
-                            e = e2p;
-                        } else {   // this stage's factor word, and the next one's read now
-                            p = wn >> 5;
-                            e = wn & 31u;
-                            const u32 k = s + 1u - o2;
-                            if (k < nf) wn = fac[k * EX_NT + tid];
-                        }
+                        const u32 e = sw[q] & 31u;
+                        if (e == 0u) continue;
                         // entry (ee, j) = lst[j] * p^ee at ee * cur + j: each entry of the
                         // list so far is read once and its e multiples are written, so
                         // the only read-after-write dependence is between stages
                         const u32 cur = n;
-                        const double dp = (double)p;
+                        const double dp = (double)(sw[q] >> 5);
                         u32 j = 0;
                         for (; j + 4 <= cur; j += 4) {   // four loads in flight
                             double y0 = lst[j], y1 = lst[j + 1], y2 = lst[j + 2], y3 = lst[j + 3];
@@ -1373,7 +1387,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                         }
                         n = cur * (e + 1);
                     }
-                    if (sL == nfa) s_ar[loff + nL] = h0;   // empty H part: the list {1}
+                    if (!inH) s_ar[loff + nL] = h0;   // empty H part: the list {1}
                 }
                 __syncthreads();
                 // lane l takes candidates l, l+64, l+128, ... of the pass, two per
@@ -1535,19 +1549,18 @@ extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *it
 }
 
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
-                                     const u64 *fpair, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
-                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq, int lds_fac)
+                                     const uint32_t *oftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr,
+                                     u64 *ctr, u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq,
+                                     int lds_fac)
 {
     ExArgs a;
     a.n_rids = n_rids;
-    a.quads = quads; a.quad_cap = quad_cap; a.ctl = ctl; a.fpair = (const ExFac *)fpair; a.tab_n = tab_n; a.ndiv = ndiv; a.qr = qr;
+    a.quads = quads; a.quad_cap = quad_cap; a.ctl = ctl; a.oftab = (const ExOdd *)oftab; a.tab_n = tab_n; a.ndiv = ndiv; a.qr = qr;
     a.ctr = ctr;
     a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs; a.exq = (uint4 *)exq;
     a.neg_m1 = -M1;
-    if (lds_fac)
-        hipLaunchKernelGGL(k_ext<true>, dim3(grid), dim3(EX_NT), 0, st, a);
-    else
-        hipLaunchKernelGGL(k_ext<false>, dim3(grid), dim3(EX_NT), 0, st, a);
+    (void)lds_fac;
+    hipLaunchKernelGGL(k_ext, dim3(grid), dim3(EX_NT), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1572,12 +1585,12 @@ extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u
 
 extern "C" int ps4_rq_threads(void) { return RQ_NT; }
 extern "C" int ps4_ext_threads(void) { return EX_NT; }
-extern "C" int ps4_ext_queue_cap(void) { return 2 * EX_WG_SCRATCH; }   // u64 words per workgroup
+extern "C" int ps4_ext_queue_cap(void) { return 2 * EX_Q2; }   // u64 words per workgroup
 extern "C" int ps4_rq_window(void) { return (int)RQ_W; }
 extern "C" int ps4_rq_nb_max(void) { return RQ_NB_MAX; }
 
 extern "C" void *ps4_k_rq_ptr(void) { return (void *)&k_rq; }
-extern "C" void *ps4_k_ext_ptr(int lds_fac) { return lds_fac ? (void *)&k_ext<true> : (void *)&k_ext<false>; }
+extern "C" void *ps4_k_ext_ptr(int lds_fac) { (void)lds_fac; return (void *)&k_ext; }
 
 // ----------------------------------------------------------------------------
 // k_triple: d with d+a, d+b, d+c square for given square triples {a,b,c}

@@ -30,8 +30,9 @@ extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *it
                                     u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u32 n_rids, u64 *scratch, ps4_quad *quads,
                                     u64 quad_cap, u32 cap_reps, u32 verify);
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
-                                     const u64 *fpair, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr, u64 *ctr,
-                                     u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq, int lds_fac);
+                                     const u32 *oftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr,
+                                     u64 *ctr, u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq,
+                                     int lds_fac);
 extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u32 n_ctr);
 extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u64 d_max, const u64 *ftab,
                                         u64 tab_n, u64 *d_out, u32 cap, u32 *count, u32 *err);
@@ -201,7 +202,7 @@ struct ps4_ctx {
     u32 hit_cap = 0, items_cap = 0, ctr_cap = 0;
     // device
     u64 *d_ftab = nullptr;
-    u64 *d_fpair = nullptr;   // {ftab[n], ftab[cofactor of n]}: two prime powers per read
+    u32 *d_oftab = nullptr;   // odd n -> its prime powers (the extension kernel's factor table)
     uint16_t *d_ndiv = nullptr;
     ps4_qrtab *d_qr = nullptr;
     ps4_ctl *d_ctl = nullptr;
@@ -297,19 +298,27 @@ int build_tables(ps4_ctx *ctx)
     for (u32 a = 0; a < 13; ++a) for (u32 r = 0; r < 13; ++r) if (isq((r * r + a) % 13, 13)) qr.t13[a] |= 1u << r;
     for (u32 a = 0; a < 17; ++a) for (u32 r = 0; r < 17; ++r) if (isq((r * r + a) % 17, 17)) qr.t17[a] |= 1u << r;
     for (u32 a = 0; a < 19; ++a) for (u32 r = 0; r < 19; ++r) if (isq((r * r + a) % 19, 19)) qr.t19[a] |= 1u << r;
-    // the extension kernel's factor walk takes two steps per read
-    std::vector<u64> fpair(2 * n, 0);
-    for (u64 i = 0; i < n; ++i) {
-        const u64 c = ftab[i] & 0xFFFFFFFFull;
-        fpair[2 * i] = ftab[i];
-        fpair[2 * i + 1] = (c > 1 && c < n) ? ftab[c] : 0ull;
+    // the extension kernel's odd-factor table: for odd i < n, entry i / 2 holds
+    // the distinct prime powers of i as p << 5 | e, ascending (at most 7 for
+    // i < 2^26), then k | d(i) << 4 with k their number (one gather factors an
+    // odd part completely)
+    const u64 n_odd = (n + 1) / 2;
+    std::vector<u32> oftab(8 * n_odd, 0);
+    for (u64 i = 1; i < n; i += 2) {
+        u32 *w = &oftab[8 * (i >> 1)];
+        u32 k = 0;
+        for (u64 m = i; m > 1; m = ftab[m] & 0xFFFFFFFFull) {
+            if (k == 7) { ctx->last_error = "odd-factor table: more than 7 primes"; return PS4_E_INTERNAL; }
+            w[k++] = (u32)(((ftab[m] >> 32) & 0x7FFFFFFull) << 5) | (u32)(ftab[m] >> 59);
+        }
+        w[7] = k | ((u32)ndiv[i] << 4);
     }
     HIP_OK(hipMalloc(&ctx->d_ftab, n * sizeof(u64)));
-    HIP_OK(hipMalloc(&ctx->d_fpair, 2 * n * sizeof(u64)));
+    HIP_OK(hipMalloc(&ctx->d_oftab, oftab.size() * sizeof(u32)));
     HIP_OK(hipMalloc(&ctx->d_ndiv, n * sizeof(uint16_t)));
     HIP_OK(hipMalloc(&ctx->d_qr, sizeof(ps4_qrtab)));
     HIP_OK(hipMemcpy(ctx->d_ftab, ftab.data(), n * sizeof(u64), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(ctx->d_fpair, fpair.data(), 2 * n * sizeof(u64), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ctx->d_oftab, oftab.data(), oftab.size() * sizeof(u32), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(ctx->d_ndiv, ndiv.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(ctx->d_qr, &qr, sizeof qr, hipMemcpyHostToDevice));
     return 0;
@@ -449,7 +458,7 @@ int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, 
     const bool lds_fac = s_top > ctx->lds_fac_above;
     if (!noext)
         HIP_OK(ps4_launch_ext(ctx->st, lds_fac ? ctx->ext_grid_lds : ctx->ext_grid, ctx->d_quads, ctx->quad_cap,
-                              ctx->d_ctl, ctx->d_fpair, ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, n_rids,
+                              ctx->d_ctl, ctx->d_oftab, ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, n_rids,
                               ctx->d_hits, ctx->hit_cap, ctx->cap_divs, ctx->d_exq, lds_fac ? 1 : 0));
     if (!graph) HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
     if (!noext)
@@ -810,7 +819,7 @@ extern "C" void ps4_close(ps4_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
-    void *dev[] = {ctx->d_ftab, ctx->d_fpair, ctx->d_ndiv, ctx->d_qr, ctx->d_ctl, ctx->d_ctr, ctx->d_items, ctx->d_scratch,
+    void *dev[] = {ctx->d_ftab, ctx->d_oftab, ctx->d_ndiv, ctx->d_qr, ctx->d_ctl, ctx->d_ctr, ctx->d_items, ctx->d_scratch,
                    ctx->d_quads, ctx->d_hits, ctx->d_exq, ctx->d_eout, ctx->d_ecnt, ctx->d_etri, ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
     for (void *p : dev) if (p) (void)hipFree(p);
     void *host[] = {ctx->h_ctl, ctx->h_ctr, ctx->h_items, ctx->h_hits};

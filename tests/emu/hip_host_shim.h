@@ -182,6 +182,10 @@ static inline unsigned __builtin_amdgcn_mbcnt_hi(unsigned m, unsigned c)
     return c + (lane < 32 ? 0u : (unsigned)__builtin_popcount(m & ((1u << (lane - 32)) - 1u)));
 }
 static inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+// the device reciprocal is an approximation (refined by its callers); the
+// emulation perturbs the exact one by a few parts in 2^40 so that the
+// refinement is exercised
+static inline double __builtin_amdgcn_rcp(double x) { return (1.0 / x) * (1.0 + 0x1p-38); }
 static inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((unsigned long long)a * b) >> 32); }
 static inline unsigned __umul24(unsigned a, unsigned b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }
 
@@ -241,6 +245,19 @@ static inline hipError_t hipMemcpy(void *d, const void *s, size_t n, int) { memc
 static inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, int, hipStream_t) { memcpy(d, s, n); return hipSuccess; }
 static inline hipError_t hipMemGetInfo(size_t *f, size_t *t) { *f = (size_t)8 << 30; *t = (size_t)16 << 30; return hipSuccess; }
 static inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, const void *, int, size_t) { *n = 1; return hipSuccess; }
+// graph capture is not emulated: asking for it fails like an unsupported call
+typedef void *hipGraph_t;
+typedef void *hipGraphExec_t;
+typedef void *hipGraphNode_t;
+typedef int hipGraphExecUpdateResult;
+enum { hipStreamCaptureModeThreadLocal = 1, hipGraphExecUpdateSuccess = 0 };
+static inline hipError_t hipStreamBeginCapture(hipStream_t, int) { return 801; }
+static inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) { *g = nullptr; return 801; }
+static inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+static inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+static inline hipError_t hipGraphExecUpdate(hipGraphExec_t, hipGraph_t, hipGraphNode_t *, hipGraphExecUpdateResult *) { return 801; }
+static inline hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, void *, void *, size_t) { return 801; }
+static inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return 801; }
 
 namespace emu {
 template <typename F, typename... Args>
