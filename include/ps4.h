@@ -101,6 +101,18 @@ int ps4_search_batch(ps4_ctx *ctx, size_t n, const uint64_t *lo, const uint64_t 
 int ps4_extend_triple(ps4_ctx *ctx, const uint64_t abc[3], uint64_t d_max,
                       uint64_t *d_out, size_t cap, size_t *n_out);
 
+/* The triple query of ps4_extend_triple without any allocation: the query, its
+ * answer count and its answers are staged in a fixed device scratch of the
+ * library (one per device, 65536 answers, 512 KiB, held until the process
+ * ends), the answers are copied into d_out and sorted there.  No counterpart
+ * in the reference; it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  *n_out receives the
+ * count; a count above 65536 is PS4_E_CAPACITY, and so is a count above cap
+ * when cap > 0 (d_out must hold every answer).  Calls that overlap on one
+ * device share the scratch and race. */
+int ps4_extend_triple_noalloc(ps4_ctx *ctx, const uint64_t abc[3], uint64_t d_max,
+                              uint64_t *d_out, size_t cap, size_t *n_out);
+
 /* Install the durable-alert hook.  With a hook installed, 6-set lines and the
  * PS4_TEST6 drill line go to the hook only; without one they go to the sink. */
 int ps4_set_alert(ps4_ctx *ctx, ps4_alert alert, void *user);

@@ -26,6 +26,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "ps4_internal.h"
 
 typedef uint64_t u64;
@@ -1703,6 +1705,39 @@ extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u
     hipLaunchKernelGGL(k_triple, dim3((u32)n), dim3(TR_NT), 0, st, abc, d_max, ftab, tab_n, d_out, cap, count, err);
     return hipGetLastError();
 }
+
+// Fixed device scratch of the allocation-free triple query
+// (ps4_extend_triple_noalloc): words 0..2 the query's members, word 3 its
+// answer count (low half) and error word (high half), words 8.. the staging of
+// its answers (PS4_NA_CAP of them).  Each device holds its own copy until the
+// process ends; host code reaches it only through the address
+// ps4_noalloc_scratch() returns (the array's name is not a device address in
+// host code).  Calls that overlap on one device share it and race.
+#define PS4_NA_CAP 65536u
+#define PS4_NA_WORDS (8u + PS4_NA_CAP)
+static __device__ __align__(256) u64 scratch[PS4_NA_WORDS];
+
+// The device address of the scratch on `device`, looked up once per device
+// (the first call for a device must have it current: the library's ps4_open
+// makes this call right after hipSetDevice) and kept for later calls.
+#define PS4_NA_DEVICES 64
+static void *g_na_ptr[PS4_NA_DEVICES];
+static std::mutex g_na_mu;
+extern "C" hipError_t ps4_noalloc_scratch(int device, void **ptr)
+{
+    if (device < 0 || device >= PS4_NA_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_na_ptr[device] = p;
+    }
+    *ptr = g_na_ptr[device];
+    return hipSuccess;
+}
+
+extern "C" unsigned ps4_noalloc_cap(void) { return PS4_NA_CAP; }
 
 // Fill n 32-bit words with a value (used instead of memset so launch
 // sequences stay free of memset nodes).

@@ -37,6 +37,8 @@ extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u
 extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u64 d_max, const u64 *ftab,
                                         u64 tab_n, u64 *d_out, u32 cap, u32 *count, u32 *err);
 extern "C" hipError_t ps4_launch_fill_u32(hipStream_t st, u32 *p, u32 n, u32 v);
+extern "C" hipError_t ps4_noalloc_scratch(int device, void **ptr);
+extern "C" unsigned ps4_noalloc_cap(void);
 extern "C" hipError_t ps4_launch_elem(hipStream_t st, const ps4_quad *quads, const ps4_ctl *ctl, u64 quad_cap,
                                       u64 n_max, ps4_quad *out, u32 cap, u32 *count, u32 grid);
 extern "C" hipError_t ps4_launch_sqtri(hipStream_t st, u64 n_max, unsigned long long *total);
@@ -232,6 +234,7 @@ struct ps4_ctx {
     u32 *d_tcnt = nullptr, *d_terr = nullptr;
     u32 tq_cap = 0, tq_out = 0;
     bool debug_sync = false;
+    void *na_scratch = nullptr;   // this device's fixed scratch (ps4_extend_triple_noalloc)
     // element-range mode: largest member allowed (0: off), the device buffer of
     // the 4-sets that qualify in a batch, and those collected so far
     u64 elem_max = 0;
@@ -850,6 +853,7 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     HIP_OK(hipGetDeviceProperties(&prop, ctx->device));
     ctx->n_cu = (u32)prop.multiProcessorCount;
     HIP_OK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
+    HIP_OK(ps4_noalloc_scratch(ctx->device, &ctx->na_scratch));
     for (hipEvent_t &e : ctx->ev) HIP_OK(hipEventCreate(&e));
     if (int rc = build_tables(ctx)) return rc;
     int b_rq = 0, b_ext = 0, b_ext_lds = 0;
@@ -1087,6 +1091,44 @@ extern "C" int ps4_extend_triple(ps4_ctx *ctx, const uint64_t abc[3], uint64_t d
                                  size_t *n_out)
 {
     return ps4_extend_triple_batch(ctx, 1, abc, d_max, d_out, cap, n_out);
+}
+
+// The triple query without any allocation: the query, its count and its
+// answers are staged in the device's fixed scratch (ps4_noalloc_scratch), the
+// answers are copied into the caller's d_out and sorted there in place.
+extern "C" int ps4_extend_triple_noalloc(ps4_ctx *ctx, const uint64_t abc[3], uint64_t d_max, uint64_t *d_out,
+                                         size_t cap, size_t *n_out)
+{
+    if (!ctx || !abc || !n_out || (cap && !d_out)) return PS4_E_ARG;
+    *n_out = 0;
+    u64 v[3] = {abc[0], abc[1], abc[2]};
+    std::sort(v, v + 3);
+    if (v[0] == 0 || v[0] == v[1] || v[1] == v[2] || !is_square((u128)v[0] + v[1]) || !is_square((u128)v[0] + v[2]) ||
+        !is_square((u128)v[1] + v[2])) {
+        ctx->last_error = "not a triple of distinct positive integers with square pair sums";
+        return PS4_E_ARG;
+    }
+    if (2 * isqrt64(v[2] + v[0]) + 2 >= ctx->tab_n) { ctx->last_error = "triple beyond the tables of this context"; return PS4_E_ARG; }
+    if (!ctx->na_scratch) { ctx->last_error = "no fixed scratch on this device"; return PS4_E_INTERNAL; }
+    HIP_OK(hipSetDevice(ctx->device));
+    u64 *s_abc = (u64 *)ctx->na_scratch;
+    u32 *s_ce = (u32 *)(s_abc + 3);          // count, error word
+    u64 *s_out = s_abc + 8;
+    const u32 na_cap = ps4_noalloc_cap();
+    HIP_OK(hipMemcpyAsync(s_abc, abc, 3 * sizeof(u64), hipMemcpyHostToDevice, ctx->st));
+    HIP_OK(ps4_launch_fill_u32(ctx->st, s_ce, 2u, 0u));   // the last call's count and error word
+    HIP_OK(ps4_launch_triple(ctx->st, s_abc, 1, d_max, ctx->d_ftab, ctx->tab_n, s_out, na_cap, s_ce, s_ce + 1));
+    u32 ce[2] = {0u, 0u};
+    HIP_OK(hipMemcpyAsync(ce, s_ce, sizeof ce, hipMemcpyDeviceToHost, ctx->st));
+    HIP_OK(hipStreamSynchronize(ctx->st));
+    if (ce[1]) { ctx->last_error = "divisor lists of a triple exceed the kernel's arena"; return PS4_E_INTERNAL; }
+    *n_out = ce[0];
+    if (ce[0] > na_cap) { ctx->last_error = "more answers than the fixed scratch stages (65536)"; return PS4_E_CAPACITY; }
+    if (!cap || !ce[0]) return 0;
+    if (ce[0] > cap) { ctx->last_error = "d_out holds fewer values than the query has answers"; return PS4_E_CAPACITY; }
+    HIP_OK(hipMemcpy(d_out, s_out, (size_t)ce[0] * sizeof(u64), hipMemcpyDeviceToHost));
+    std::sort(d_out, d_out + ce[0]);
+    return 0;
 }
 
 

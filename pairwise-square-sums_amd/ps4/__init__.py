@@ -114,6 +114,7 @@ def load():
     lib.ps4_search_batch.argtypes = [P, ctypes.c_size_t, u64p, u64p, SINK, ctypes.POINTER(P),
                                      ctypes.POINTER(Counters)]
     lib.ps4_extend_triple.argtypes = [P, u64p, u64, u64p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    lib.ps4_extend_triple_noalloc.argtypes = [P, u64p, u64, u64p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     lib.ps4_extend_triple_batch.argtypes = [P, ctypes.c_size_t, u64p, u64, u64p, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)]
     lib.ps4_search_elements.argtypes = [P, u64, SINK, P, u64p, u64p]
@@ -237,6 +238,15 @@ class Engine:
         n = ctypes.c_size_t()
         self._check(self.lib.ps4_extend_triple(self.ctx, arr, int(d_max), out, cap, ctypes.byref(n)))
         return [int(out[i]) for i in range(min(n.value, cap))]
+
+    def extend_triple_noalloc(self, abc, d_max, cap=1 << 16):
+        """The same query through ps4_extend_triple_noalloc (no device
+        allocation: the library's fixed per-device scratch)."""
+        arr = (ctypes.c_uint64 * 3)(*[int(x) for x in abc])
+        out = (ctypes.c_uint64 * cap)()
+        n = ctypes.c_size_t()
+        self._check(self.lib.ps4_extend_triple_noalloc(self.ctx, arr, int(d_max), out, cap, ctypes.byref(n)))
+        return [int(out[i]) for i in range(n.value)]
 
     def extend_triple_batch(self, triples, d_max, cap=4096):
         """Triple queries in one launch: one ascending list of d per triple (a

@@ -245,6 +245,10 @@ static inline hipError_t hipMemcpy(void *d, const void *s, size_t n, int) { memc
 static inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, int, hipStream_t) { memcpy(d, s, n); return hipSuccess; }
 static inline hipError_t hipMemGetInfo(size_t *f, size_t *t) { *f = (size_t)8 << 30; *t = (size_t)16 << 30; return hipSuccess; }
 static inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, const void *, int, size_t) { *n = 1; return hipSuccess; }
+// device symbols are host objects here
+#define HIP_SYMBOL(x) (&(x))
+static inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
+enum { hipErrorInvalidDevice = 101 };
 // graph capture is not emulated: asking for it fails like an unsupported call
 typedef void *hipGraph_t;
 typedef void *hipGraphExec_t;
