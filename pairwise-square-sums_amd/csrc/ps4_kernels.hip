@@ -759,8 +759,8 @@ struct ExArgs {
 //         the mod-65 mask, then the masks mod 11 (bit 1), 13 (12), 17 (25), 19 (42);
 //         bits 61..63 hold the pair choice
 // Read only for queued candidates: s_mem[k] = {isqrt(A), F1, F2, F3} (the roots
-// of the pair sums containing A) and s_aux[k] = isqrt(2A) when 2A is a square,
-// else 0, | lane << 26 (the 4-set's lane in the tile).
+// of the pair sums containing A), s_aux[k] = isqrt(2A) when 2A is a square,
+// else 0, and s_qix[k] = the 4-set's queue index.
 #define EX_PAR_WORDS 3
 
 
@@ -1110,7 +1110,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
     __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
     __shared__ uint4 s_par[EX_NT * EX_PAR_WORDS];   // parameters of the 4-sets of the current pass
     __shared__ uint4 s_mem[EX_NT];             // their e > 0 and member bounds
-    __shared__ u32 s_aux[EX_NT];               // isqrt(2A) or 0, | lane << 26
+    __shared__ u32 s_aux[EX_NT];               // isqrt(2A) or 0
+    __shared__ u32 s_qix[EX_NT];               // their queue indices
     __shared__ uint4 s_bits4[EX_CMAX / 128];   // bit g: a 4-set's candidates start at g + 1
     __shared__ u64 s_q1[EX_Q1];                // candidates past the residue filters: u | record << 52
     u32 *s_bits = (u32 *)s_bits4;
@@ -1144,39 +1145,77 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
         qn2 = 0;
     };
 
-    u64 pf_tile = ~0ull;   // the tile whose entries qpf holds (loaded during the previous tile)
-    ps4_quad qpf;
-    u32 chunk = 0;
-    if (tid == 0) chunk = atomicAdd(&a.ctl->tile_next, 1u);
-    chunk = uni(chunk);   // lane 0's value (every lane is active here)
-    while (chunk < nchunks) {
-        // claim the next chunk now; its latency overlaps this chunk's work
-        u32 next = 0;
-        if (tid == 0) next = atomicAdd(&a.ctl->tile_next, 1u);
-        for (u32 sub = 0; sub < EX_CHUNK; ++sub) {
-            const u64 tile = (u64)chunk * EX_CHUNK + sub;
-            if (tile >= ntiles) break;
-            const u64 qidx = tile * EX_NT + tid;
-            bool valid = qidx < nquads;
-            const ps4_quad qd = tile == pf_tile ? qpf : a.quads[valid ? qidx : tile * EX_NT];   // lane 0's entry exists
-            valid = valid && qd.rid != PS4_RID_NONE;   // padding of a window's block
-            const u32 rid0 = uni(qd.rid);
-            const bool uniform = __ballot(valid && qd.rid != rid0) == 0ull && rid0 < a.n_rids;
-            if (!uniform || rid0 != acc_rid) {
-                ex_flush(a.ctr, a.n_rids, acc_rid, acc_cand, acc_esq, acc_md);
-                acc_cand = acc_esq = acc_md = 0;
-                acc_rid = uniform ? rid0 : 0xFFFFFFFFu;
+    // ---- The wave's stream of queued 4-sets: chunks of EX_CHUNK tiles, claimed
+    // one ahead.  Each lane holds one set-up 4-set until a pass takes it, and
+    // then the next 4-set of the stream, so every pass fills the arena with the
+    // 4-sets the lanes hold (a whole tile need not be used up first).
+    u32 chunk = 0, nxt = 0;
+    if (tid == 0) {
+        chunk = atomicAdd(&a.ctl->tile_next, 1u);
+        nxt = atomicAdd(&a.ctl->tile_next, 1u);
+    }
+    chunk = uni(chunk);   // lane 0's values (every lane is active here)
+    nxt = uni(nxt);
+    const u64 CH = (u64)EX_CHUNK * EX_NT;   // 4-sets per chunk
+    u64 s_pos = (u64)chunk * CH;            // next stream index to hand out
+    u64 s_end = chunk < nchunks ? min(s_pos + CH, nquads) : s_pos;   // end of the current chunk (s_pos: none left)
+    // The next stream indices for the lanes of mask, in lane order (~0: none
+    // left); every lane must be active.
+    auto take = [&](u64 mask) -> u64 {
+        const u32 k = (u32)__popcll(mask), r = lanes_below(mask);
+        const u64 avail = s_end - s_pos;
+        u64 qi = ~0ull;
+        if ((mask >> tid) & 1ull) {
+            if (r < avail) {
+                qi = s_pos + r;
+            } else if (nxt < nchunks) {
+                const u64 b = (u64)nxt * CH + (r - avail);
+                if (b < nquads) qi = b;
             }
-            u64 my_cand = 0, my_maxdiv = 0;
-            u32 my_esq = 0, my_unc = 0;   // ext_sqrt: residue-filter passes, minus the queued ones that fail the rest
-            u32 e2 = 0, e2p = 0, sL = 0, half = 0, nL = 0, nH = 0, lsize = 0, ncand = 0, ndp = 0, best = 0;
-            u32 o1 = 1, o2 = 1, n1 = 0, n2 = 0;   // odd parts of ra - rb, ra + rb and their numbers of primes
-            u32 st[EX_NPOS];                      // stage words of D' (ex_stages)
+        }
+        if ((u64)k < avail) {
+            s_pos += k;
+        } else if (nxt < nchunks) {   // the current chunk is used up: move to the claimed one
+            s_end = min((u64)nxt * CH + CH, nquads);
+            s_pos = min((u64)nxt * CH + (k - avail), s_end);
+            u32 n = 0;
+            if (tid == 0) n = atomicAdd(&a.ctl->tile_next, 1u);
+            nxt = uni(n);
+        } else {
+            s_pos = s_end;
+        }
+        return qi;
+    };
+
+    // the lane's set-up 4-set (has: one waits for a pass) and its parameters
+    bool has = false;
+    u32 qix = 0, lrid = 0;                  // its queue index and interval
+    u32 ncand = 0, lsize = 0, nL = 0, nH = 0, sL = 0, e2p = 0, half = 0, best = 0;
+    u32 o1n = 0, o2n = 0;                   // odd parts of ra - rb, ra + rb | their numbers of primes << 28
+    u64 pm64 = 0, pm63 = 0, pm65 = 0, pmsm = 0;
+    u32 psqA = 0, pf1 = 0, pf2 = 0, pf3 = 0, pf4 = 0;
+    // the entry each free lane sets up next, loaded ahead (index pqi, ~0: none)
+    u64 pqi = take(~0ull);
+    ps4_quad qpf;
+    if (pqi != ~0ull) qpf = a.quads[pqi];
+    for (;;) {
+        // ---- lanes without a 4-set set up their next one
+        const bool refill = !has;
+        bool valid = refill && pqi != ~0ull;
+        const ps4_quad qd = qpf;
+        valid = valid && qd.rid != PS4_RID_NONE;   // padding of a window's block
+        u64 my_cand = 0, my_maxdiv = 0;
+        u32 e2 = 0, ndp = 0, n1 = 0, n2 = 0;
+        u32 st[EX_NPOS];                        // stage words of D' (ex_stages)
 #pragma unroll
-            for (int q = 0; q < EX_NPOS; ++q) st[q] = 0u;
-            u64 pm64 = 0, pm63 = 0, pm65 = 0, pmsm = 0;
-            u32 psqA = 0, pf1 = 0, pf2 = 0, pf3 = 0, pf4 = 0;
-            if (valid) {
+        for (int q = 0; q < EX_NPOS; ++q) st[q] = 0u;
+        if (refill) {
+            ncand = 0;
+            qix = (u32)pqi;
+            lrid = qd.rid;
+        }
+        if (valid) {
+            u32 o1 = 1, o2 = 1;
                 const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
                 const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
                 const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
@@ -1220,6 +1259,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     e2 = z1 + z2;
                     o1 = m1 >> z1;
                     o2 = m2 >> z2;
+                    o1n = o1;
+                    o2n = o2;
                     const ExOdd f1 = ex_odd_at(a.oftab, o1), f2 = ex_odd_at(a.oftab, o2);
                     // quadratic-residue masks of this 4-set (read together with the
                     // odd-factor entries, so that the reads overlap): bit r of M_m set iff
@@ -1268,6 +1309,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     ex_stages(f1, f2, o1, e2p, st, &ndodd);
                     n1 = f1.b.w & 15u;
                     n2 = f2.b.w & 15u;
+                    o1n |= n1 << 28;
+                    o2n |= n2 << 28;
                     const u64 ndD = (u64)(e2 + 1) * ndodd;
                     my_maxdiv = ndD;
                     if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
@@ -1278,12 +1321,13 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     }
                 }
             }
-            // Split the stage sequence into a prefix L (positions < sL) and the
-            // rest H, minimising |L| + |H| = nL + nH over the prefixes; ndp / nl
-            // is exact (nl divides ndp < 2^24) by a rounded single-precision
-            // quotient.  Positions no lane of the wave uses are skipped (uniform).
+        // Split the stage sequence into a prefix L (positions < sL) and the
+        // rest H, minimising |L| + |H| = nL + nH over the prefixes; ndp / nl is
+        // exact (nl divides ndp < 2^24) by a rounded single-precision quotient.
+        // Positions no lane set up now uses are skipped (uniform).
+        {
             const u32 n1max = wave_max7(n1), n2max = wave_max7(n2);
-            if (ncand > 0) {
+            if (ncand > 0 && refill) {
                 const float fndp = (float)ndp;
                 u32 nl = 1, bestc = 1 + ndp, bestnl = 1;
                 sL = 0;
@@ -1300,41 +1344,69 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 lsize = bestc;
                 if (nL * nH != ndp || nl != ndp || ncand > EX_CMAX) { err |= ERR_FACTORS; ncand = 0; }   // never
             }
-
-            // the next tile's entries: loaded now, used after this tile's passes
-            {
-                const u64 nt = sub + 1 < EX_CHUNK ? tile + 1 : (u64)uni(next) * EX_CHUNK;
-                if (nt < ntiles) {
-                    const u64 ni = nt * EX_NT + tid;
-                    qpf = a.quads[ni < nquads ? ni : nt * EX_NT];
-                    pf_tile = nt;
+        }
+        // ext_cand and the divisor maximum of the 4-sets set up now: into the
+        // wave's counters when they all belong to its interval, else (an
+        // interval boundary) each to its own
+        {
+            const u64 gm = __ballot(valid);
+            if (gm != 0ull) {
+                const u32 r0 = (u32)__builtin_amdgcn_readlane((int)lrid, (int)__builtin_ctzll(gm));
+                const bool uni_r = __ballot(valid && lrid != r0) == 0ull && r0 < a.n_rids;
+                if (uni_r) {
+                    if (r0 != acc_rid) {
+                        ex_flush(a.ctr, a.n_rids, acc_rid, acc_cand, acc_esq, acc_md);
+                        acc_cand = acc_esq = acc_md = 0;
+                        acc_rid = r0;
+                    }
+                    acc_cand += my_cand;
+                    acc_md = my_maxdiv > acc_md ? my_maxdiv : acc_md;
+                } else if (valid && lrid < a.n_rids) {
+                    unsigned long long *c = (unsigned long long *)(a.ctr + (u64)lrid * CT_N);
+                    atomicAdd(c + CT_EXT_CAND, (unsigned long long)my_cand);
+                    atomicMax(c + CT_MAX_DIVS, (unsigned long long)my_maxdiv);
                 }
             }
+        }
+        if (refill) has = ncand > 0;
+        if (__ballot(has) == 0ull) {      // nothing to extend: take more, or stop
+            if (s_pos >= s_end) break;
+            pqi = take(~0ull);
+            if (pqi != ~0ull) qpf = a.quads[pqi];
+            continue;
+        }
 
-            // ---- passes: divisor lists of as many 4-sets as the arena holds,
-            //      then all their candidates spread over the lanes
-            bool pending = ncand > 0;
-            while (__ballot(pending) != 0ull) {
-                // the odd-factor entries again (cheaper than keeping the stage words
-                // in registers across the candidate loop): issued before the scans
-                // and the barrier, used by the list building after them
-                ExOdd g1, g2;
-                if (pending) {
-                    g1 = ex_odd_at(a.oftab, o1);
-                    g2 = ex_odd_at(a.oftab, o2);
-                }
-                u32 ltot, ptot, ctot;
-                const u32 loff = wave_excl_scan(pending ? lsize : 0u, &ltot);
-                const u32 coff = wave_excl_scan(pending ? ncand : 0u, &ptot);
-                bool fits = pending && loff + lsize <= EX_ARENA && coff + ncand <= EX_CMAX;
-                if (pending && loff == 0 && !fits) { err |= ERR_ARENA; pending = false; }   // never (bounds above)
-                const u64 fmask = __ballot(fits);
-                const u32 slot = (u32)__popcll(fmask & ((1ull << tid) - 1ull));
-                (void)wave_excl_scan(fits ? ncand : 0u, &ctot);
-                // fitting 4-sets are a prefix of the pending ones: coff is each one's
-                // first candidate in the pass
-                for (u32 w = tid; w < (ctot + 127) / 128 * 4; w += EX_NT) s_bits[w] = 0u;   // whole 128-bit words
-                __syncthreads();
+        // ---- one pass: the divisor lists of as many held 4-sets as the arena
+        //      holds, then all their candidates spread over the lanes
+        {
+            const bool pending = has;
+            // the odd-factor entries again (cheaper than keeping the stage words
+            // in registers across the candidate loop): issued before the scans
+            // and the barrier, used by the list building after them
+            ExOdd g1, g2;
+            if (pending) {
+                g1 = ex_odd_at(a.oftab, o1n & 0x0FFFFFFFu);
+                g2 = ex_odd_at(a.oftab, o2n & 0x0FFFFFFFu);
+            }
+            u32 ltot, ptot, ctot;
+            const u32 loff = wave_excl_scan(pending ? lsize : 0u, &ltot);
+            const u32 coff = wave_excl_scan(pending ? ncand : 0u, &ptot);
+            bool fits = pending && loff + lsize <= EX_ARENA && coff + ncand <= EX_CMAX;
+            bool drop = false;
+            if (pending && loff == 0 && !fits) { err |= ERR_ARENA; drop = true; }   // never (bounds above)
+            const u64 fmask = __ballot(fits);
+            const u32 slot = (u32)__popcll(fmask & ((1ull << tid) - 1ull));
+            (void)wave_excl_scan(fits ? ncand : 0u, &ctot);
+            // the stage positions the fitting 4-sets use (uniform)
+            const u32 p1max = wave_max7(fits ? (o1n >> 28) : 0u), p2max = wave_max7(fits ? (o2n >> 28) : 0u);
+            // ext_sqrt of this pass into the wave's counters when every fitting
+            // 4-set belongs to its interval (else per candidate, rare)
+            const bool puni = __ballot(fits && lrid != acc_rid) == 0ull && acc_rid < a.n_rids;
+            u32 my_esq = 0, my_unc = 0;   // ext_sqrt: residue-filter passes, minus the queued ones that fail the rest
+            // fitting 4-sets are a prefix of the pending ones: coff is each one's
+            // first candidate in the pass
+            for (u32 w = tid; w < (ctot + 127) / 128 * 4; w += EX_NT) s_bits[w] = 0u;   // whole 128-bit words
+            __syncthreads();
                 if (fits) {
                     uint4 *pw = s_par + slot * EX_PAR_WORDS;
                     const u32 aL = 8u * loff, aH = 8u * (loff + nL);
@@ -1343,7 +1415,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     pw[1] = make_uint4((u32)pm64, (u32)(pm64 >> 32), (u32)pm63, (u32)(pm63 >> 32));
                     pw[2] = make_uint4((u32)pm65, (u32)(pm65 >> 32), (u32)pmsm, (u32)(pmsm >> 32) | (best << 29));
                     s_mem[slot] = make_uint4(psqA, pf1, pf2, pf3);
-                    s_aux[slot] = pf4 | (tid << 26);
+                    s_aux[slot] = pf4;
+                    s_qix[slot] = qix;
                     if (coff > 0u) atomicOr(&s_bits[(coff - 1u) >> 5], 1u << ((coff - 1u) & 31u));
                     // mixed-radix lists over the stages: stage s multiplies the list
                     // so far by p_s^1 .. p_s^e (earlier stages are the less
@@ -1352,7 +1425,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // the H list holds halves (exact doubles), so that
                     // u = L[i] * H[j] + L[i'] * H[j'] in both cases.
                     u32 sw[EX_NPOS], nd2;
-                    ex_stages(g1, g2, o1, e2p, sw, &nd2);
+                    ex_stages(g1, g2, o1n & 0x0FFFFFFFu, e2p, sw, &nd2);
                     const double h0 = half ? 0.5 : 1.0;
                     double *lst = s_ar + loff;
                     u32 n = 1;
@@ -1360,7 +1433,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     bool inH = false;
 #pragma unroll
                     for (int q = 0; q < EX_NPOS; ++q) {
-                        if (!ex_pos_live(q, n1max, n2max)) continue;
+                        if (!ex_pos_live(q, p1max, p2max)) continue;
                         if (!inH && (u32)q >= sL) {   // start the H list
                             lst = s_ar + loff + nL;
                             n = 1;
@@ -1392,6 +1465,11 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     if (!inH) s_ar[loff + nL] = h0;   // empty H part: the list {1}
                 }
                 __syncthreads();
+            __syncthreads();
+            // the entries the lanes free after this pass set up next: loaded now,
+            // used after the candidate loop
+            pqi = take(__ballot(!has || fits || drop));
+            if (pqi != ~0ull) qpf = a.quads[pqi];
                 // lane l takes candidates l, l+64, l+128, ... of the pass, two per
                 // iteration (g and g + 64: two independent load/compute chains).
                 // Pipelined: while this iteration's list entries load, the first
@@ -1446,15 +1524,15 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     const ExCand B = ex_cand(B1, B2, LB, a.neg_m1);
                     fetch_masks();
                     const bool oka = va & A.ok, okb = vb & B.ok;
-                    if (uniform) {
+                    if (puni) {
                         my_esq += (oka ? 1u : 0u) + (okb ? 1u : 0u);
-                    } else {   // a tile with several intervals (rare): count where each belongs
+                    } else {   // a pass with several intervals (rare): count where each belongs
                         if (oka) {
-                            const u32 r = a.quads[tile * EX_NT + (s_aux[ka0] >> 26)].rid;
+                            const u32 r = a.quads[s_qix[ka0]].rid;
                             atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
                         }
                         if (okb) {
-                            const u32 r = a.quads[tile * EX_NT + (s_aux[kb0] >> 26)].rid;
+                            const u32 r = a.quads[s_qix[kb0]].rid;
                             atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
                         }
                     }
@@ -1479,17 +1557,17 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                             if (ex_counted(u, s_mem[k], aux)) {
                                 const uint4 q2 = s_par[k * EX_PAR_WORDS + 2];
                                 go = ex_late(u, q2);
-                                aux = (aux >> 26) | ((q2.w >> 29) << 6);   // lane | pair choice << 6
-                            } else if (uniform) {
+                                aux = k | ((q2.w >> 29) << 6);   // record | pair choice << 6
+                            } else if (puni) {
                                 ++my_unc;
                             } else {   // take it back from its interval's count
-                                const u32 r = a.quads[tile * EX_NT + (aux >> 26)].rid;
+                                const u32 r = a.quads[s_qix[k]].rid;
                                 atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), ~0ull);
                             }
                         }
                         const u64 bg = __ballot(go);
                         if (go) {
-                            const u64 qi = tile * EX_NT + (aux & 63u);
+                            const u64 qi = s_qix[aux & 63u];
                             exq2[qn2 + lanes_below(bg)] =
                                 make_uint4((u32)ent, ((u32)(ent >> 32) & 0xFFFFFu) | ((aux >> 6) << 20), (u32)qi,
                                            (u32)(qi >> 32));
@@ -1509,24 +1587,13 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     while (qn1 >= 64u || (base + 128u >= ctot && qn1 > 0u))   // (the pass's records
                         round1();                                              //  are about to be replaced)
                 }
-                __syncthreads();
-                if (fits) pending = false;
-            }
-
-            // ---- counters of this tile
-            if (uniform) {
-                acc_cand += my_cand;
+            __syncthreads();
+            if (fits || drop) has = false;
+            if (puni) {
                 acc_esq += my_esq;
                 acc_esq -= my_unc;   // lane sums may wrap; the wave's sum is exact
-                acc_md = my_maxdiv > acc_md ? my_maxdiv : acc_md;
-            } else if (valid) {
-                unsigned long long *c = (unsigned long long *)(a.ctr + (u64)qd.rid * CT_N);
-                atomicAdd(c + CT_EXT_CAND, (unsigned long long)my_cand);
-                atomicMax(c + CT_MAX_DIVS, (unsigned long long)my_maxdiv);
             }
-            __syncthreads();
         }
-        chunk = uni(next);
     }
     if (qn2 > 0u) drain2();
     ex_flush(a.ctr, a.n_rids, acc_rid, acc_cand, acc_esq, acc_md);
