@@ -722,7 +722,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                           // odd part of ra - rb and EX_OSLOT for that of ra + rb
 
 // The odd-factor table: for odd n < tab_n, entry (n - 1) / 2 is two 16-byte
-// words {w0..w7}: w0..w6 the distinct prime powers of n as p << 5 | e in
+// words {w0..w7}: w0..w6 the distinct prime powers of n as p << 6 | e in
 // ascending order of p (0 past the last), w7 = k | d(n) << 4 with k the
 // number of distinct primes.  One gather factors an odd part completely.
 struct __align__(16) ExOdd {
@@ -922,7 +922,8 @@ __device__ __forceinline__ bool ex_divides(u32 p, double o)
     return p > 1u && __builtin_fma(-q, dp, o) == 0.0;
 }
 
-// The prime powers of D' in stage order, as words p << 5 | e (0: no stage):
+// The prime powers of D' in stage order, as words p << 6 | e (0: no stage;
+// six exponent bits: the power of two of D < 2^51 can exceed 31):
 // position 0 the power of two 2^e2p, positions 1..7 those of the odd part o1
 // of ra - rb (entry f1), positions 8..14 those of the odd part of ra + rb
 // (entry f2) whose prime does not divide o1.  A prime dividing both (it then
@@ -932,7 +933,7 @@ __device__ __forceinline__ void ex_stages(const ExOdd &f1, const ExOdd &f2, u32 
 {
     const u32 w1[EX_OSLOT] = {f1.a.x, f1.a.y, f1.a.z, f1.a.w, f1.b.x, f1.b.y, f1.b.z};
     const u32 w2[EX_OSLOT] = {f2.a.x, f2.a.y, f2.a.z, f2.a.w, f2.b.x, f2.b.y, f2.b.z};
-    st[0] = e2p ? (2u << 5) | e2p : 0u;
+    st[0] = e2p ? (2u << 6) | e2p : 0u;
 #pragma unroll
     for (int i = 0; i < EX_OSLOT; ++i) st[1 + i] = w1[i];
     u32 d = (f1.b.w >> 4) * (f2.b.w >> 4);
@@ -940,14 +941,14 @@ __device__ __forceinline__ void ex_stages(const ExOdd &f1, const ExOdd &f2, u32 
 #pragma unroll
     for (int j = 0; j < EX_OSLOT; ++j) {
         u32 w = w2[j];
-        if (ex_divides(w >> 5, o)) {   // a prime of both parts (rare)
-            const u32 p = w >> 5, eb = w & 31u;
+        if (ex_divides(w >> 6, o)) {   // a prime of both parts (rare)
+            const u32 p = w >> 6, eb = w & 63u;
 #pragma unroll
             for (int i = 0; i < EX_OSLOT; ++i) {
                 const u32 v = st[1 + i];
-                if ((v >> 5) == p) {
+                if ((v >> 6) == p) {
                     // d: the factors (ea + 1)(eb + 1) of the two parts become ea + eb + 1
-                    const u32 ea = v & 31u;
+                    const u32 ea = v & 63u;
                     d = d / ((ea + 1u) * (eb + 1u)) * (ea + eb + 1u);
                     st[1 + i] = v + eb;
                 }
@@ -1196,7 +1197,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
     u32 psqA = 0, pf1 = 0, pf2 = 0, pf3 = 0, pf4 = 0;
     // the entry each free lane sets up next, loaded ahead (index pqi, ~0: none)
     u64 pqi = take(~0ull);
-    ps4_quad qpf;
+    ps4_quad qpf = {0u, 0u, 0u, 0u, 0u, 0u, PS4_RID_NONE, 0u};
     if (pqi != ~0ull) qpf = a.quads[pqi];
     for (;;) {
         // ---- lanes without a 4-set set up their next one
@@ -1321,6 +1322,15 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     }
                 }
             }
+        // the odd-factor entries of the 4-sets held over from an earlier pass
+        // (their stage words are not kept in registers across the candidate
+        // loop): issued now, merged before the list building
+        const bool carried = has;
+        ExOdd g1 = {}, g2 = {};
+        if (carried) {
+            g1 = ex_odd_at(a.oftab, o1n & 0x0FFFFFFFu);
+            g2 = ex_odd_at(a.oftab, o2n & 0x0FFFFFFFu);
+        }
         // Split the stage sequence into a prefix L (positions < sL) and the
         // rest H, minimising |L| + |H| = nL + nH over the prefixes; ndp / nl is
         // exact (nl divides ndp < 2^24) by a rounded single-precision quotient.
@@ -1334,7 +1344,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
 #pragma unroll
                 for (int q = 0; q < EX_NPOS; ++q) {
                     if (!ex_pos_live(q, n1max, n2max)) continue;
-                    nl *= (st[q] & 31u) + 1u;
+                    nl *= (st[q] & 63u) + 1u;
                     const u32 nh = (u32)__builtin_rintf(fndp * __builtin_amdgcn_rcpf((float)nl));
                     const u32 c = nl + nh;
                     if (c < bestc) { bestc = c; sL = (u32)q + 1u; bestnl = nl; }
@@ -1380,14 +1390,6 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
         //      holds, then all their candidates spread over the lanes
         {
             const bool pending = has;
-            // the odd-factor entries again (cheaper than keeping the stage words
-            // in registers across the candidate loop): issued before the scans
-            // and the barrier, used by the list building after them
-            ExOdd g1, g2;
-            if (pending) {
-                g1 = ex_odd_at(a.oftab, o1n & 0x0FFFFFFFu);
-                g2 = ex_odd_at(a.oftab, o2n & 0x0FFFFFFFu);
-            }
             u32 ltot, ptot, ctot;
             const u32 loff = wave_excl_scan(pending ? lsize : 0u, &ltot);
             const u32 coff = wave_excl_scan(pending ? ncand : 0u, &ptot);
@@ -1424,8 +1426,11 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // positions before sL build L, the rest H.  When u = (f + g) / 2
                     // the H list holds halves (exact doubles), so that
                     // u = L[i] * H[j] + L[i'] * H[j'] in both cases.
-                    u32 sw[EX_NPOS], nd2;
-                    ex_stages(g1, g2, o1n & 0x0FFFFFFFu, e2p, sw, &nd2);
+                    if (carried) {   // (a 4-set set up in this round has its words from the set-up)
+                        u32 nd2;
+                        ex_stages(g1, g2, o1n & 0x0FFFFFFFu, e2p, st, &nd2);
+                    }
+                    const u32 *sw = st;
                     const double h0 = half ? 0.5 : 1.0;
                     double *lst = s_ar + loff;
                     u32 n = 1;
@@ -1440,13 +1445,13 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                             lst[0] = h0;
                             inH = true;
                         }
-                        const u32 e = sw[q] & 31u;
+                        const u32 e = sw[q] & 63u;
                         if (e == 0u) continue;
                         // entry (ee, j) = lst[j] * p^ee at ee * cur + j: each entry of the
                         // list so far is read once and its e multiples are written, so
                         // the only read-after-write dependence is between stages
                         const u32 cur = n;
-                        const double dp = (double)(sw[q] >> 5);
+                        const double dp = (double)(sw[q] >> 6);
                         u32 j = 0;
                         for (; j + 4 <= cur; j += 4) {   // four loads in flight
                             double y0 = lst[j], y1 = lst[j + 1], y2 = lst[j + 2], y3 = lst[j + 3];

@@ -302,7 +302,7 @@ int build_tables(ps4_ctx *ctx)
     for (u32 a = 0; a < 17; ++a) for (u32 r = 0; r < 17; ++r) if (isq((r * r + a) % 17, 17)) qr.t17[a] |= 1u << r;
     for (u32 a = 0; a < 19; ++a) for (u32 r = 0; r < 19; ++r) if (isq((r * r + a) % 19, 19)) qr.t19[a] |= 1u << r;
     // the extension kernel's odd-factor table: for odd i < n, entry i / 2 holds
-    // the distinct prime powers of i as p << 5 | e, ascending (at most 7 for
+    // the distinct prime powers of i as p << 6 | e, ascending (at most 7 for
     // i < 2^26), then k | d(i) << 4 with k their number (one gather factors an
     // odd part completely)
     const u64 n_odd = (n + 1) / 2;
@@ -312,7 +312,7 @@ int build_tables(ps4_ctx *ctx)
         u32 k = 0;
         for (u64 m = i; m > 1; m = ftab[m] & 0xFFFFFFFFull) {
             if (k == 7) { ctx->last_error = "odd-factor table: more than 7 primes"; return PS4_E_INTERNAL; }
-            w[k++] = (u32)(((ftab[m] >> 32) & 0x7FFFFFFull) << 5) | (u32)(ftab[m] >> 59);
+            w[k++] = (u32)(((ftab[m] >> 32) & 0x7FFFFFFull) << 6) | (u32)(ftab[m] >> 59);
         }
         w[7] = k | ((u32)ndiv[i] << 4);
     }
