@@ -51,14 +51,20 @@ __device__ __forceinline__ u64 d_isqrt64(u64 n)
     return r;
 }
 
-// floor(sqrt(n)) for n < 2^52: n converts exactly, so the root of the
-// correctly rounded square root is within one of it; one step each way.
-__device__ __forceinline__ u64 d_isqrt52(u64 n)
+// floor(sqrt(n)) for 1 <= n < 2^52 from the hardware reciprocal square root
+// refined by one Newton step (a few operations, where the correctly rounded
+// square root is a long sequence); every candidate root t < 2^26 has t^2 and
+// (t+1)^2 exact in doubles, so the corrections are exact whatever the estimate.
+__device__ __forceinline__ u32 d_isqrt52r(u64 n)
 {
-    u64 r = (u64)__builtin_sqrt((double)n);
-    r -= (r * r > n) ? 1u : 0u;
-    r += ((r + 1) * (r + 1) <= n) ? 1u : 0u;
-    return r;
+    const double x = (double)n;
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y;
+    s = __builtin_fma(s, __builtin_fma(-s, 0.5 * y, 0.5), s);
+    double t = __builtin_trunc(s);
+    while (t * t > x) t -= 1.0;
+    while ((t + 1.0) * (t + 1.0) <= x) t += 1.0;
+    return (u32)t;
 }
 
 // n is a perfect square, n < 2^106 (exhaustive/s4.c:101-108).
@@ -1290,7 +1296,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m,
                     // i.e. u is a root of a pair sum containing A, or u^2 = 2A
                     // (every root is below 2^26 for S < 2^50)
-                    const u64 sa = d_isqrt52(P.A), s2a = d_isqrt52(2 * P.A);
+                    const u64 sa = d_isqrt52r(P.A), s2a = d_isqrt52r(2 * P.A);
                     psqA = (u32)sa;
                     pf1 = P.F1;
                     pf2 = P.F2;

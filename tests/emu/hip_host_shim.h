@@ -186,6 +186,8 @@ static inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 // emulation perturbs the exact one by a few parts in 2^40 so that the
 // refinement is exercised
 static inline double __builtin_amdgcn_rcp(double x) { return (1.0 / x) * (1.0 + 0x1p-38); }
+// the reciprocal square root likewise, a few parts in 2^30 off
+static inline double __builtin_amdgcn_rsq(double x) { return (1.0 / sqrt(x)) * (1.0 - 0x1p-30); }
 static inline unsigned __umulhi(unsigned a, unsigned b) { return (unsigned)(((unsigned long long)a * b) >> 32); }
 static inline unsigned __umul24(unsigned a, unsigned b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }
 
