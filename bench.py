@@ -326,6 +326,7 @@ def bench_serving(args):
                        "triple": {"p50": pct(lat_t, 0.5), "p99": pct(lat_t, 0.99)}},
         "throughput": {"requests_per_s": n / (t_end - t_start), "range_S_per_s": s_served / (t_end - t_start)},
         "steps": len(srv.steps), "mean_requests_per_step": n / max(1, len(srv.steps)),
+        "step_ms": {"p50": pct([t for _, t in srv.steps], 0.5), "p99": pct([t for _, t in srv.steps], 0.99)},
         "same_alone_and_in_mix": same, "sample_checked": len(sample), "errors": errors[:5],
     }
 

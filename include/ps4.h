@@ -113,6 +113,44 @@ int ps4_extend_triple(ps4_ctx *ctx, const uint64_t abc[3], uint64_t d_max,
 int ps4_extend_triple_noalloc(ps4_ctx *ctx, const uint64_t abc[3], uint64_t d_max,
                               uint64_t *d_out, size_t cap, size_t *n_out);
 
+/* ---- Serving: continuous batching over one context.  No reference
+ * counterpart (s4 answers one interval per process, exhaustive/s4.c:416-426);
+ * the census of a range request is that of ps4_search, a triple request that
+ * of ps4_extend_triple.  Requests are queued by ps4_submit_*; each ps4_step
+ * runs one short batch: the range requests in order of their remaining work
+ * (shortest first) each take the next part of their interval while the
+ * step's expected 4-sets fit its budget and the free part of the 4-set queue,
+ * all in shared persistent launches, and the pending triple queries run
+ * beside them on a second stream.  A request's result is the same as when it
+ * runs alone. */
+typedef struct {
+    uint64_t tag;              /* the caller's id of the request                          */
+    int rc;                    /* 0, or the PS4_E_* this request failed with              */
+    int kind;                  /* 0 range, 1 triple                                       */
+    ps4_counters counters;     /* range: as ps4_search gives them                         */
+    const char *lines;         /* range: its result lines (valid during the callback)     */
+    size_t nbytes;
+    const uint64_t *d;         /* triple: every answer, ascending (valid during the call)  */
+    size_t n_d;
+} ps4_done;
+typedef void (*ps4_done_fn)(void *user, const ps4_done *done);
+
+typedef struct {
+    uint64_t pending_ranges, pending_triples;   /* requests still queued after the step        */
+    uint64_t quad_slots;       /* 4-set queue entries one step may fill                          */
+    uint64_t hit_slots;        /* extension entries one step may hold                            */
+    uint64_t triple_slots;     /* triple queries one step may hold                              */
+} ps4_stream_state;
+
+/* Queue a range request [S_lo, S_hi) (checked as by ps4_search). */
+int ps4_submit_range(ps4_ctx *ctx, uint64_t tag, uint64_t S_lo, uint64_t S_hi);
+/* Queue a triple request (checked as by ps4_extend_triple). */
+int ps4_submit_triple(ps4_ctx *ctx, uint64_t tag, const uint64_t abc[3], uint64_t d_max);
+/* Run one step; requests it finishes go to done(user, ...).  max_quads: the
+ * step's budget of expected 4-sets (0: 2^23, about a millisecond of work);
+ * *state (may be NULL) gets what remains queued and the step's capacities. */
+int ps4_step(ps4_ctx *ctx, uint64_t max_quads, ps4_done_fn done, void *user, ps4_stream_state *state);
+
 /* Install the durable-alert hook.  With a hook installed, 6-set lines and the
  * PS4_TEST6 drill line go to the hook only; without one they go to the sink. */
 int ps4_set_alert(ps4_ctx *ctx, ps4_alert alert, void *user);

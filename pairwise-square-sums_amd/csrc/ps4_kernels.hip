@@ -1197,7 +1197,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
     // the lane's set-up 4-set (has: one waits for a pass) and its parameters
     bool has = false;
     u32 qix = 0, lrid = 0;                  // its queue index and interval
-    u32 ncand = 0, lsize = 0, nL = 0, nH = 0, sL = 0, e2p = 0, half = 0, best = 0;
+    u32 ncand = 0, lsize = 0, nL = 0;       // candidates, nL + nH and nL of its lists
+    u32 lpk = 0;                            // sL | e2p << 4 | half << 10 | pair choice << 11 | d(D) << 14
     u32 o1n = 0, o2n = 0;                   // odd parts of ra - rb, ra + rb | their numbers of primes << 28
     u64 pm64 = 0, pm63 = 0, pm65 = 0, pmsm = 0;
     u32 psqA = 0, pf1 = 0, pf2 = 0, pf3 = 0, pf4 = 0;
@@ -1211,7 +1212,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
         bool valid = refill && pqi != ~0ull;
         const ps4_quad qd = qpf;
         valid = valid && qd.rid != PS4_RID_NONE;   // padding of a window's block
-        u64 my_cand = 0, my_maxdiv = 0;
+        u64 my_maxdiv = 0;
+        u32 sL = 0, e2p = 0, half = 0, best = 0, nH = 0;   // this round's values (lpk keeps them)
         u32 e2 = 0, ndp = 0, n1 = 0, n2 = 0;
         u32 st[EX_NPOS];                        // stage words of D' (ex_stages)
 #pragma unroll
@@ -1324,7 +1326,6 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     if (e2 != 1 && ndD <= a.cap_divs) {
                         ndp = (e2p + 1) * ndodd;
                         ncand = ndp / 2;
-                        my_cand = ncand;
                     }
                 }
             }
@@ -1361,29 +1362,17 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 if (nL * nH != ndp || nl != ndp || ncand > EX_CMAX) { err |= ERR_FACTORS; ncand = 0; }   // never
             }
         }
-        // ext_cand and the divisor maximum of the 4-sets set up now: into the
-        // wave's counters when they all belong to its interval, else (an
-        // interval boundary) each to its own
-        {
-            const u64 gm = __ballot(valid);
-            if (gm != 0ull) {
-                const u32 r0 = (u32)__builtin_amdgcn_readlane((int)lrid, (int)__builtin_ctzll(gm));
-                const bool uni_r = __ballot(valid && lrid != r0) == 0ull && r0 < a.n_rids;
-                if (uni_r) {
-                    if (r0 != acc_rid) {
-                        ex_flush(a.ctr, a.n_rids, acc_rid, acc_cand, acc_esq, acc_md);
-                        acc_cand = acc_esq = acc_md = 0;
-                        acc_rid = r0;
-                    }
-                    acc_cand += my_cand;
-                    acc_md = my_maxdiv > acc_md ? my_maxdiv : acc_md;
-                } else if (valid && lrid < a.n_rids) {
-                    unsigned long long *c = (unsigned long long *)(a.ctr + (u64)lrid * CT_N);
-                    atomicAdd(c + CT_EXT_CAND, (unsigned long long)my_cand);
-                    atomicMax(c + CT_MAX_DIVS, (unsigned long long)my_maxdiv);
-                }
-            }
-        }
+        // A 4-set set up now that yields no candidates is counted at once (into
+        // its own interval: rare); the others are counted by the pass that
+        // takes them (ext_cand, the divisor maximum and ext_sqrt together).
+        if (valid && ncand == 0 && lrid < a.n_rids)
+            atomicMax((unsigned long long *)(a.ctr + (u64)lrid * CT_N + CT_MAX_DIVS), (unsigned long long)my_maxdiv);
+        if (refill && ncand > 0) lpk = sL | (e2p << 4) | (half << 10) | (best << 11) | ((u32)my_maxdiv << 14);
+        sL = lpk & 15u;
+        e2p = (lpk >> 4) & 63u;
+        half = (lpk >> 10) & 1u;
+        best = (lpk >> 11) & 7u;
+        const u32 lmd = lpk >> 14;
         if (refill) has = ncand > 0;
         if (__ballot(has) == 0ull) {      // nothing to extend: take more, or stop
             if (s_pos >= s_end) break;
@@ -1395,21 +1384,31 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
         // ---- one pass: the divisor lists of as many held 4-sets as the arena
         //      holds, then all their candidates spread over the lanes
         {
-            const bool pending = has;
+            // a pass takes 4-sets of one interval: that of the first lane holding
+            // one (so its counters go to the wave's counters of that interval)
+            const u64 hm = __ballot(has);
+            const u32 prid = (u32)__builtin_amdgcn_readlane((int)lrid, (int)__builtin_ctzll(hm));
+            const bool pending = has && lrid == prid;
+            if (prid != acc_rid) {
+                ex_flush(a.ctr, a.n_rids, acc_rid, acc_cand, acc_esq, acc_md);
+                acc_cand = acc_esq = acc_md = 0;
+                acc_rid = prid;
+            }
             u32 ltot, ptot, ctot;
             const u32 loff = wave_excl_scan(pending ? lsize : 0u, &ltot);
             const u32 coff = wave_excl_scan(pending ? ncand : 0u, &ptot);
             bool fits = pending && loff + lsize <= EX_ARENA && coff + ncand <= EX_CMAX;
             bool drop = false;
             if (pending && loff == 0 && !fits) { err |= ERR_ARENA; drop = true; }   // never (bounds above)
+            if (fits) {
+                acc_cand += ncand;
+                acc_md = lmd > acc_md ? (u64)lmd : acc_md;
+            }
             const u64 fmask = __ballot(fits);
             const u32 slot = (u32)__popcll(fmask & ((1ull << tid) - 1ull));
             (void)wave_excl_scan(fits ? ncand : 0u, &ctot);
             // the stage positions the fitting 4-sets use (uniform)
             const u32 p1max = wave_max7(fits ? (o1n >> 28) : 0u), p2max = wave_max7(fits ? (o2n >> 28) : 0u);
-            // ext_sqrt of this pass into the wave's counters when every fitting
-            // 4-set belongs to its interval (else per candidate, rare)
-            const bool puni = __ballot(fits && lrid != acc_rid) == 0ull && acc_rid < a.n_rids;
             u32 my_esq = 0, my_unc = 0;   // ext_sqrt: residue-filter passes, minus the queued ones that fail the rest
             // fitting 4-sets are a prefix of the pending ones: coff is each one's
             // first candidate in the pass
@@ -1418,7 +1417,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 if (fits) {
                     uint4 *pw = s_par + slot * EX_PAR_WORDS;
                     const u32 aL = 8u * loff, aH = 8u * (loff + nL);
-                    pw[0] = make_uint4(aL | ((aL + 8u * (nL - 1u)) << 16), aH | ((aH + 8u * (nH - 1u)) << 16),
+                    pw[0] = make_uint4(aL | ((aL + 8u * (nL - 1u)) << 16), aH | ((aH + 8u * (lsize - nL - 1u)) << 16),
                                        (0x800000u + nL - 1u) / nL, ((0u - 8u * nL) & 0xFFFFu) | (coff << 16));
                     pw[1] = make_uint4((u32)pm64, (u32)(pm64 >> 32), (u32)pm63, (u32)(pm63 >> 32));
                     pw[2] = make_uint4((u32)pm65, (u32)(pm65 >> 32), (u32)pmsm, (u32)(pmsm >> 32) | (best << 29));
@@ -1535,18 +1534,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     const ExCand B = ex_cand(B1, B2, LB, a.neg_m1);
                     fetch_masks();
                     const bool oka = va & A.ok, okb = vb & B.ok;
-                    if (puni) {
-                        my_esq += (oka ? 1u : 0u) + (okb ? 1u : 0u);
-                    } else {   // a pass with several intervals (rare): count where each belongs
-                        if (oka) {
-                            const u32 r = a.quads[s_qix[ka0]].rid;
-                            atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
-                        }
-                        if (okb) {
-                            const u32 r = a.quads[s_qix[kb0]].rid;
-                            atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), 1ull);
-                        }
-                    }
+                    my_esq += (oka ? 1u : 0u) + (okb ? 1u : 0u);
                     // candidates past the residue filters wait on an LDS stack (their
                     // order does not matter) for the rest of the ext_sqrt tests and the
                     // later filters, which run 64 at a time with every lane busy; the
@@ -1569,11 +1557,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                                 const uint4 q2 = s_par[k * EX_PAR_WORDS + 2];
                                 go = ex_late(u, q2);
                                 aux = k | ((q2.w >> 29) << 6);   // record | pair choice << 6
-                            } else if (puni) {
+                            } else {
                                 ++my_unc;
-                            } else {   // take it back from its interval's count
-                                const u32 r = a.quads[s_qix[k]].rid;
-                                atomicAdd((unsigned long long *)(a.ctr + (u64)r * CT_N + CT_EXT_SQRT), ~0ull);
                             }
                         }
                         const u64 bg = __ballot(go);
@@ -1600,10 +1585,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 }
             __syncthreads();
             if (fits || drop) has = false;
-            if (puni) {
-                acc_esq += my_esq;
-                acc_esq -= my_unc;   // lane sums may wrap; the wave's sum is exact
-            }
+            acc_esq += my_esq;
+            acc_esq -= my_unc;   // lane sums may wrap; the wave's sum is exact
         }
     }
     if (qn2 > 0u) drain2();

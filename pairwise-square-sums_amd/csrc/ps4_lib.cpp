@@ -186,6 +186,7 @@ Quad6 members(const u32 *r)
 struct ps4_ctx {
     int device = 0;
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;   // triple queries of a serving step, beside the census batch on st
     uint32_t flags = 0;
     int64_t dump4_limit = 0;
     u32 cap_reps = 512, cap_divs = 16384, cap_ext = 64;
@@ -242,6 +243,16 @@ struct ps4_ctx {
     u32 *d_ecnt = nullptr;
     unsigned long long *d_etri = nullptr;
     std::vector<ps4_quad> elem_sets;
+    // serving stream (ps4_submit_*, ps4_step): requests in arrival order
+    struct StreamReq {
+        u64 tag = 0;
+        bool triple = false;
+        u64 lo = 0, cur = 0, hi = 0;   // a range request: [lo, hi), searched up to cur
+        u64 abc[3] = {0, 0, 0}, d_max = 0;
+        ps4_counters c;                // counters so far
+        std::string lines;             // result lines so far
+    };
+    std::vector<StreamReq> sq;
 };
 
 #define HIP_OK(call)                                                                     \
@@ -426,7 +437,7 @@ struct Pending {
 // counter reset, k_rq, (k_elem), k_ext and the read-backs of the control block
 // and counter rows into pinned memory.  Kernel time is taken by events around
 // the two census kernels.  Returns once everything is enqueued.
-int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, bool graph)
+int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, bool graph, double quads_expected)
 {
     const u32 n_ctr = n_rids * CT_N;
     if (ctx->rq_grid > ctx->scratch_wgs) { ctx->last_error = "k_rq grid above its scratch"; return PS4_E_INTERNAL; }
@@ -446,7 +457,17 @@ int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, 
     HIP_OK(ps4_launch_reset(ctx->st, ctx->d_ctl, ctx->d_ctr, n_ctr));
     if (int rc = dbg("k_reset")) return rc;
     if (!graph) HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
-    HIP_OK(ps4_launch_rq(ctx->st, ctx->rq_grid, ctx->d_items, n_items, ctx->bin_cap, ctx->nb_alloc, ctx->d_ctl,
+    // A small batch launches only the workgroups it can keep busy: every
+    // workgroup of the persistent kernels claims work by an atomic on one
+    // counter, so idle ones only add to the contention (about 0.2 ms each for
+    // the two kernels at full grids).  k_rq takes one item per claim; k_ext
+    // one chunk of 4 tiles of 64 queued 4-sets (its queue is filled in whole
+    // tiles, so the upper bound on 4-sets plus one tile per item bounds the chunks).
+    const u32 g_rq = std::max<u32>(1u, std::min<u32>(ctx->rq_grid, n_items));
+    const double chunks = (quads_expected + 64.0 * (double)n_items) / 256.0 + 1.0;
+    const u32 g_ext_full = (s_top > ctx->lds_fac_above) ? ctx->ext_grid_lds : ctx->ext_grid;
+    const u32 g_ext = (u32)std::max<double>(1.0, std::min<double>((double)g_ext_full, chunks));
+    HIP_OK(ps4_launch_rq(ctx->st, g_rq, ctx->d_items, n_items, ctx->bin_cap, ctx->nb_alloc, ctx->d_ctl,
                          ctx->d_ctr, n_rids, ctx->d_scratch, ctx->d_quads, ctx->quad_cap, ctx->cap_reps,
                          (ctx->flags & PS4_VERIFY) ? 1u : 0u));
     if (!graph) HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
@@ -460,7 +481,7 @@ int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, 
     // factor tables), in global scratch with two more waves per CU below
     const bool lds_fac = s_top > ctx->lds_fac_above;
     if (!noext)
-        HIP_OK(ps4_launch_ext(ctx->st, lds_fac ? ctx->ext_grid_lds : ctx->ext_grid, ctx->d_quads, ctx->quad_cap,
+        HIP_OK(ps4_launch_ext(ctx->st, g_ext, ctx->d_quads, ctx->quad_cap,
                               ctx->d_ctl, ctx->d_oftab, ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, n_rids,
                               ctx->d_hits, ctx->hit_cap, ctx->cap_divs, ctx->d_exq, lds_fac ? 1 : 0));
     if (!graph) HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
@@ -474,11 +495,11 @@ int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, 
 // One batch launched: directly, or (graph) captured and launched as one
 // hipGraph.  A failure inside the capture ends the capture and drops the
 // partial graph, so the stream is usable again.
-int launch_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, bool graph)
+int launch_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, bool graph, double quads_expected)
 {
-    if (!graph) return enqueue_batch(ctx, n_items, n_rids, s_top, noext, false);
+    if (!graph) return enqueue_batch(ctx, n_items, n_rids, s_top, noext, false, quads_expected);
     HIP_OK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_batch(ctx, n_items, n_rids, s_top, noext, true);
+    const int rc = enqueue_batch(ctx, n_items, n_rids, s_top, noext, true, quads_expected);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(ctx->st, &g);
     if (rc || ec != hipSuccess) {
@@ -510,6 +531,130 @@ int launch_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, b
     return 0;
 }
 
+// The result lines of a finished batch (exhaustive/s4.c:237-277): "4:" lines
+// in the reference's single-thread order (dump_left: per interval, the 4-set
+// lines still to print), and the extensions grouped per 4-set into M:, 5: and
+// 6-set lines, with the 5-set histogram, 5-set and multi counts and the
+// extension maximum added to out[rid].  text[r - rid_lo] gets interval r's lines.
+int make_lines(ps4_ctx *ctx, const Pending &pb, std::vector<int64_t> &dump_left, ps4_counters *out,
+               std::vector<std::string> &text)
+{
+    const u32 rid_lo = pb.rid_lo, rid_hi = pb.rid_hi;
+    text.assign(rid_hi - rid_lo + 1, std::string());
+    // ---- "4:" lines (exhaustive/s4.c:237-241), in the reference's single-thread order
+    if (!pb.qs.empty()) {
+        const u64 nq = pb.qs.size();
+        const std::vector<ps4_quad> &qs = pb.qs;
+        struct K { u64 S, px, py, pz; u32 cls; size_t i; };
+        std::vector<K> keys;
+        keys.reserve(nq);
+        for (u64 i = 0; i < nq; ++i) {
+            const ps4_quad &x = qs[i];
+            if (x.rid == PS4_RID_NONE) continue;   // padding of a window's block
+            if (x.rid < rid_lo || x.rid > rid_hi) { ctx->last_error = "4-set queue entry of another batch"; return PS4_E_INTERNAL; }
+            const u64 S = (u64)x.r_ab * x.r_ab + (u64)x.r_cd * x.r_cd;
+            const u32 cls = x.r_ad < x.r_bc ? 1u : 0u;
+            keys.push_back({S, x.r_cd, x.r_bd, cls ? x.r_ad : x.r_bc, cls, (size_t)i});
+        }
+        std::sort(keys.begin(), keys.end(), [&](const K &x, const K &y) {
+            if (qs[x.i].rid != qs[y.i].rid) return qs[x.i].rid < qs[y.i].rid;
+            if (x.S != y.S) return x.S < y.S;
+            if (x.px != y.px) return x.px < y.px;
+            if (x.py != y.py) return x.py < y.py;
+            if (x.pz != y.pz) return x.pz < y.pz;
+            return x.cls < y.cls;
+        });
+        char line[160];
+        for (const K &k : keys) {
+            const ps4_quad &x = qs[k.i];
+            if (dump_left[x.rid] <= 0) continue;
+            --dump_left[x.rid];
+            const u32 r6[6] = {x.r_ab, x.r_ac, x.r_ad, x.r_bc, x.r_bd, x.r_cd};
+            const Quad6 q = members(r6);
+            snprintf(line, sizeof line, "4: [%lld,%lld,%lld,%lld] S=%llu\n", (long long)q.a, (long long)q.b,
+                     (long long)q.c, (long long)q.d, (unsigned long long)q.S);
+            text[x.rid - rid_lo] += line;
+        }
+    }
+    // ---- extensions -> M:, 5: and 6-set lines (exhaustive/s4.c:245-277)
+    const u32 nh = (u32)pb.hits.size();
+    if (nh) {
+        const ps4_hit *hits = pb.hits.data();
+        // order: interval, S, then the 4-set's roots (one group per 4-set)
+        struct HK { u64 S, k1, k2, k3; u32 rid, i; };   // roots < 2^26: two per key word
+        std::vector<HK> hk(nh);
+        for (u32 i = 0; i < nh; ++i) {
+            const ps4_hit &h = hits[i];
+            const u64 S = (u64)h.r[0] * h.r[0] + (u64)h.r[5] * h.r[5];
+            hk[i] = {S, ((u64)h.r[0] << 26) | h.r[1], ((u64)h.r[2] << 26) | h.r[3], ((u64)h.r[4] << 26) | h.r[5],
+                     h.rid, i};
+        }
+        std::sort(hk.begin(), hk.end(), [](const HK &x, const HK &y) {
+            if (x.rid != y.rid) return x.rid < y.rid;
+            if (x.S != y.S) return x.S < y.S;
+            if (x.k1 != y.k1) return x.k1 < y.k1;
+            if (x.k2 != y.k2) return x.k2 < y.k2;
+            return x.k3 < y.k3;
+        });
+        std::vector<u32> idx(nh);
+        for (u32 i = 0; i < nh; ++i) idx[i] = hk[i].i;
+        auto key = [&](const ps4_hit &h) { return std::make_tuple(h.rid, h.r[0], h.r[1], h.r[2], h.r[3], h.r[4], h.r[5]); };
+        for (u32 i = 0; i < nh;) {
+            const ps4_hit &h0 = hits[idx[i]];
+            Group g;
+            g.rid = h0.rid;
+            g.q = members(h0.r);
+            g.best = h0.pad;
+            u32 j = i;
+            while (j < nh && key(hits[idx[j]]) == key(h0)) {
+                const ps4_hit &h = hits[idx[j]];
+                g.ext.push_back(((u128)h.e_hi << 64) | h.e_lo);
+                ++j;
+            }
+            i = j;
+            ps4_counters &o = out[g.rid];
+            const u64 ne = g.ext.size();
+            if (ne > ctx->cap_ext) { ctx->last_error = "extension capacity exceeded"; return PS4_E_CAPACITY; }
+            o.max_extensions = std::max(o.max_extensions, ne);
+            order_extensions(g);
+            std::string &t = text[g.rid - rid_lo];
+            const Quad6 &q = g.q;
+            if (ne >= 2) {
+                ++o.multi;
+                std::string m = "M: [" + std::to_string((unsigned long long)q.a) + "," + std::to_string((unsigned long long)q.b) +
+                                "," + std::to_string((unsigned long long)q.c) + "," + std::to_string((unsigned long long)q.d) +
+                                "] ext=[";
+                for (size_t k = 0; k < ne; ++k) { if (k) m += ","; m += dec(g.ext[k]); }
+                m += "] S=" + std::to_string((unsigned long long)q.S) + "\n";
+                t += m;
+                if ((ctx->flags & PS4_TEST6) && !ctx->test6_fired) {
+                    ctx->test6_fired = true;
+                    const std::string l = format_set("TEST-6SET (NOT A SOLUTION, --test-sixset drill):",
+                                                     {q.a, q.b, q.c, q.d, g.ext[0], g.ext[1]}, q.S);
+                    if (ctx->alert) ctx->alert(ctx->alert_user, l.c_str()); else t += l;
+                }
+            }
+            for (size_t k = 0; k < ne; ++k) {
+                if (g.ext[k] <= q.a) continue;
+                ++o.sets5;
+                int lg = 0;
+                for (u128 x = g.ext[k]; x > 1; x >>= 1) ++lg;
+                ++o.hist5[lg > 99 ? 99 : lg];
+                const u128 v5[5] = {q.a, q.b, q.c, q.d, g.ext[k]};
+                append_set(t, "5:", v5, 5, q.S);
+            }
+            for (size_t k = 0; k < ne; ++k)
+                for (size_t s = k + 1; s < ne; ++s)
+                    if (is_square(g.ext[k] + g.ext[s])) {
+                        ++o.sets6;
+                        const std::string l = format_set("!!!!!!!! FOUND 6-SET:", {q.a, q.b, q.c, q.d, g.ext[k], g.ext[s]}, q.S);
+                        if (ctx->alert) ctx->alert(ctx->alert_user, l.c_str()); else t += l;
+                    }
+        }
+    }
+    return 0;
+}
+
 int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink sink, void *const *users,
             void *single_user, ps4_counters *out)
 {
@@ -533,118 +678,8 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
     bool have_pending = false;
     auto emit = [&](const Pending &pb) -> int {
         const u32 rid_lo = pb.rid_lo, rid_hi = pb.rid_hi;
-        std::vector<std::string> text(rid_hi - rid_lo + 1);
-        // ---- "4:" lines (exhaustive/s4.c:237-241), in the reference's single-thread order
-        if (!pb.qs.empty()) {
-            const u64 nq = pb.qs.size();
-            const std::vector<ps4_quad> &qs = pb.qs;
-            struct K { u64 S, px, py, pz; u32 cls; size_t i; };
-            std::vector<K> keys;
-            keys.reserve(nq);
-            for (u64 i = 0; i < nq; ++i) {
-                const ps4_quad &x = qs[i];
-                if (x.rid == PS4_RID_NONE) continue;   // padding of a window's block
-                if (x.rid < rid_lo || x.rid > rid_hi) { ctx->last_error = "4-set queue entry of another batch"; return PS4_E_INTERNAL; }
-                const u64 S = (u64)x.r_ab * x.r_ab + (u64)x.r_cd * x.r_cd;
-                const u32 cls = x.r_ad < x.r_bc ? 1u : 0u;
-                keys.push_back({S, x.r_cd, x.r_bd, cls ? x.r_ad : x.r_bc, cls, (size_t)i});
-            }
-            std::sort(keys.begin(), keys.end(), [&](const K &x, const K &y) {
-                if (qs[x.i].rid != qs[y.i].rid) return qs[x.i].rid < qs[y.i].rid;
-                if (x.S != y.S) return x.S < y.S;
-                if (x.px != y.px) return x.px < y.px;
-                if (x.py != y.py) return x.py < y.py;
-                if (x.pz != y.pz) return x.pz < y.pz;
-                return x.cls < y.cls;
-            });
-            char line[160];
-            for (const K &k : keys) {
-                const ps4_quad &x = qs[k.i];
-                if (dump_left[x.rid] <= 0) continue;
-                --dump_left[x.rid];
-                const u32 r6[6] = {x.r_ab, x.r_ac, x.r_ad, x.r_bc, x.r_bd, x.r_cd};
-                const Quad6 q = members(r6);
-                snprintf(line, sizeof line, "4: [%lld,%lld,%lld,%lld] S=%llu\n", (long long)q.a, (long long)q.b,
-                         (long long)q.c, (long long)q.d, (unsigned long long)q.S);
-                text[x.rid - rid_lo] += line;
-            }
-        }
-        // ---- extensions -> M:, 5: and 6-set lines (exhaustive/s4.c:245-277)
-        const u32 nh = (u32)pb.hits.size();
-        if (nh) {
-            const ps4_hit *hits = pb.hits.data();
-            // order: interval, S, then the 4-set's roots (one group per 4-set)
-            struct HK { u64 S, k1, k2, k3; u32 rid, i; };   // roots < 2^26: two per key word
-            std::vector<HK> hk(nh);
-            for (u32 i = 0; i < nh; ++i) {
-                const ps4_hit &h = hits[i];
-                const u64 S = (u64)h.r[0] * h.r[0] + (u64)h.r[5] * h.r[5];
-                hk[i] = {S, ((u64)h.r[0] << 26) | h.r[1], ((u64)h.r[2] << 26) | h.r[3], ((u64)h.r[4] << 26) | h.r[5],
-                         h.rid, i};
-            }
-            std::sort(hk.begin(), hk.end(), [](const HK &x, const HK &y) {
-                if (x.rid != y.rid) return x.rid < y.rid;
-                if (x.S != y.S) return x.S < y.S;
-                if (x.k1 != y.k1) return x.k1 < y.k1;
-                if (x.k2 != y.k2) return x.k2 < y.k2;
-                return x.k3 < y.k3;
-            });
-            std::vector<u32> idx(nh);
-            for (u32 i = 0; i < nh; ++i) idx[i] = hk[i].i;
-            auto key = [&](const ps4_hit &h) { return std::make_tuple(h.rid, h.r[0], h.r[1], h.r[2], h.r[3], h.r[4], h.r[5]); };
-            for (u32 i = 0; i < nh;) {
-                const ps4_hit &h0 = hits[idx[i]];
-                Group g;
-                g.rid = h0.rid;
-                g.q = members(h0.r);
-                g.best = h0.pad;
-                u32 j = i;
-                while (j < nh && key(hits[idx[j]]) == key(h0)) {
-                    const ps4_hit &h = hits[idx[j]];
-                    g.ext.push_back(((u128)h.e_hi << 64) | h.e_lo);
-                    ++j;
-                }
-                i = j;
-                ps4_counters &o = out[g.rid];
-                const u64 ne = g.ext.size();
-                if (ne > ctx->cap_ext) { ctx->last_error = "extension capacity exceeded"; return PS4_E_CAPACITY; }
-                o.max_extensions = std::max(o.max_extensions, ne);
-                order_extensions(g);
-                std::string &t = text[g.rid - rid_lo];
-                const Quad6 &q = g.q;
-                if (ne >= 2) {
-                    ++o.multi;
-                    std::string m = "M: [" + std::to_string((unsigned long long)q.a) + "," + std::to_string((unsigned long long)q.b) +
-                                    "," + std::to_string((unsigned long long)q.c) + "," + std::to_string((unsigned long long)q.d) +
-                                    "] ext=[";
-                    for (size_t k = 0; k < ne; ++k) { if (k) m += ","; m += dec(g.ext[k]); }
-                    m += "] S=" + std::to_string((unsigned long long)q.S) + "\n";
-                    t += m;
-                    if ((ctx->flags & PS4_TEST6) && !ctx->test6_fired) {
-                        ctx->test6_fired = true;
-                        const std::string l = format_set("TEST-6SET (NOT A SOLUTION, --test-sixset drill):",
-                                                         {q.a, q.b, q.c, q.d, g.ext[0], g.ext[1]}, q.S);
-                        if (ctx->alert) ctx->alert(ctx->alert_user, l.c_str()); else t += l;
-                    }
-                }
-                for (size_t k = 0; k < ne; ++k) {
-                    if (g.ext[k] <= q.a) continue;
-                    ++o.sets5;
-                    int lg = 0;
-                    for (u128 x = g.ext[k]; x > 1; x >>= 1) ++lg;
-                    ++o.hist5[lg > 99 ? 99 : lg];
-                    const u128 v5[5] = {q.a, q.b, q.c, q.d, g.ext[k]};
-                    append_set(t, "5:", v5, 5, q.S);
-                }
-                for (size_t k = 0; k < ne; ++k)
-                    for (size_t s = k + 1; s < ne; ++s)
-                        if (is_square(g.ext[k] + g.ext[s])) {
-                            ++o.sets6;
-                            const std::string l = format_set("!!!!!!!! FOUND 6-SET:", {q.a, q.b, q.c, q.d, g.ext[k], g.ext[s]}, q.S);
-                            if (ctx->alert) ctx->alert(ctx->alert_user, l.c_str()); else t += l;
-                        }
-            }
-        }
+        std::vector<std::string> text;
+        if (int rc = make_lines(ctx, pb, dump_left, out, text)) return rc;
         if (sink)
             for (u32 r = rid_lo; r <= rid_hi; ++r) {
                 const std::string &t = text[r - rid_lo];
@@ -710,7 +745,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         }
         // PS4_GRAPH: the sequence is captured and launched as one graph
         const bool graph = (ctx->flags & PS4_GRAPH) && !ctx->debug_sync;
-        if (int rc = launch_batch(ctx, n_items, rid_hi + 1, s_top, noext, graph)) return fail(rc);
+        if (int rc = launch_batch(ctx, n_items, rid_hi + 1, s_top, noext, graph, quads_expected)) return fail(rc);
         if (have_pending) {
             have_pending = false;
             if (int rc = emit(pending)) return fail(rc);
@@ -822,6 +857,7 @@ extern "C" void ps4_close(ps4_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    if (ctx->st2) (void)hipStreamSynchronize(ctx->st2);
     void *dev[] = {ctx->d_ftab, ctx->d_oftab, ctx->d_ndiv, ctx->d_qr, ctx->d_ctl, ctx->d_ctr, ctx->d_items, ctx->d_scratch,
                    ctx->d_quads, ctx->d_hits, ctx->d_exq, ctx->d_eout, ctx->d_ecnt, ctx->d_etri, ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
     for (void *p : dev) if (p) (void)hipFree(p);
@@ -830,6 +866,7 @@ extern "C" void ps4_close(ps4_ctx *ctx)
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     for (hipEvent_t e : ctx->ev) if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
+    if (ctx->st2) (void)hipStreamDestroy(ctx->st2);
     delete ctx;
 }
 
@@ -853,6 +890,7 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     HIP_OK(hipGetDeviceProperties(&prop, ctx->device));
     ctx->n_cu = (u32)prop.multiProcessorCount;
     HIP_OK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&ctx->st2, hipStreamNonBlocking));
     HIP_OK(ps4_noalloc_scratch(ctx->device, &ctx->na_scratch));
     for (hipEvent_t &e : ctx->ev) HIP_OK(hipEventCreate(&e));
     if (int rc = build_tables(ctx)) return rc;
@@ -1036,53 +1074,89 @@ extern "C" int ps4_kernel_time(ps4_ctx *ctx, double *ms_rq, double *ms_ext, uint
 // Triple queries, n at once (one workgroup each on the device).  abc holds 3n
 // values; query i writes its count to n_out[i] and up to cap values to
 // d_out[i*cap ...], ascending.
+namespace {
+
+// A triple of distinct positive integers with square pair sums, inside the
+// tables of the context?  (PS4_E_ARG with the reason otherwise.)
+int check_triple(ps4_ctx *ctx, const uint64_t *abc)
+{
+    u64 v[3] = {abc[0], abc[1], abc[2]};
+    std::sort(v, v + 3);
+    if (v[0] == 0 || v[0] == v[1] || v[1] == v[2] || !is_square((u128)v[0] + v[1]) || !is_square((u128)v[0] + v[2]) ||
+        !is_square((u128)v[1] + v[2])) {
+        ctx->last_error = "not a triple of distinct positive integers with square pair sums";
+        return PS4_E_ARG;
+    }
+    if (2 * isqrt64(v[2] + v[0]) + 2 >= ctx->tab_n) { ctx->last_error = "triple beyond the tables of this context"; return PS4_E_ARG; }
+    return 0;
+}
+
+const u32 TQ_PER = 1u << 16;   // answers one triple query may stage
+
+// Enqueue n checked triple queries on stream st (their counts and answers land
+// in the context's triple buffers, grown here when n exceeds them).
+int triple_enqueue(ps4_ctx *ctx, hipStream_t st, size_t n, const uint64_t *abc, uint64_t d_max)
+{
+    if (n > ctx->tq_cap) {
+        HIP_OK(hipStreamSynchronize(ctx->st));
+        HIP_OK(hipStreamSynchronize(ctx->st2));
+        void *old[] = {ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
+        for (void *p : old) if (p) (void)hipFree(p);
+        const u32 c = (u32)std::max<size_t>(n, 64);
+        HIP_OK(hipMalloc(&ctx->d_tabc, (size_t)c * 3 * sizeof(u64)));
+        HIP_OK(hipMalloc(&ctx->d_tout, (size_t)c * TQ_PER * sizeof(u64)));
+        HIP_OK(hipMalloc(&ctx->d_tcnt, (size_t)c * sizeof(u32)));
+        HIP_OK(hipMalloc(&ctx->d_terr, sizeof(u32)));
+        ctx->tq_cap = c;
+    }
+    HIP_OK(hipMemcpyAsync(ctx->d_tabc, abc, n * 3 * sizeof(u64), hipMemcpyHostToDevice, st));
+    HIP_OK(ps4_launch_fill_u32(st, ctx->d_tcnt, (u32)n, 0u));
+    HIP_OK(ps4_launch_fill_u32(st, ctx->d_terr, 1u, 0u));
+    HIP_OK(ps4_launch_triple(st, ctx->d_tabc, n, d_max, ctx->d_ftab, ctx->tab_n, ctx->d_tout, TQ_PER, ctx->d_tcnt,
+                             ctx->d_terr));
+    return 0;
+}
+
+// After triple_enqueue on st: wait, then query i's answers ascending into
+// ans[i] (all of them) and its count into n_out[i].
+int triple_collect(ps4_ctx *ctx, hipStream_t st, size_t n, std::vector<std::vector<u64>> &ans)
+{
+    std::vector<u32> cnt(n);
+    u32 terr = 0;
+    HIP_OK(hipMemcpyAsync(cnt.data(), ctx->d_tcnt, n * sizeof(u32), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(&terr, ctx->d_terr, sizeof(u32), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (terr) { ctx->last_error = "divisor lists of a triple exceed the kernel's arena"; return PS4_E_INTERNAL; }
+    ans.assign(n, std::vector<u64>());
+    for (size_t i = 0; i < n; ++i) {
+        if (cnt[i] > TQ_PER) { ctx->last_error = "too many solutions for one triple"; return PS4_E_CAPACITY; }
+        if (!cnt[i]) continue;
+        ans[i].resize(cnt[i]);
+        HIP_OK(hipMemcpy(ans[i].data(), ctx->d_tout + i * TQ_PER, cnt[i] * sizeof(u64), hipMemcpyDeviceToHost));
+        std::sort(ans[i].begin(), ans[i].end());
+    }
+    return 0;
+}
+
+}  // namespace
+
+// Triple queries, n at once (one workgroup each on the device).  abc holds 3n
+// values; query i writes its count to n_out[i] and up to cap values to
+// d_out[i*cap ...], ascending.
 extern "C" int ps4_extend_triple_batch(ps4_ctx *ctx, size_t n, const uint64_t *abc, uint64_t d_max,
                                        uint64_t *d_out, size_t cap, size_t *n_out)
 {
     if (!ctx || !abc || !n_out || (cap && !d_out)) return PS4_E_ARG;
     if (n == 0) return 0;
     HIP_OK(hipSetDevice(ctx->device));
+    for (size_t i = 0; i < n; ++i)
+        if (int rc = check_triple(ctx, abc + 3 * i)) return rc;
+    if (int rc = triple_enqueue(ctx, ctx->st, n, abc, d_max)) return rc;
+    std::vector<std::vector<u64>> ans;
+    if (int rc = triple_collect(ctx, ctx->st, n, ans)) return rc;
     for (size_t i = 0; i < n; ++i) {
-        u64 v[3] = {abc[3 * i], abc[3 * i + 1], abc[3 * i + 2]};
-        std::sort(v, v + 3);
-        if (v[0] == 0 || v[0] == v[1] || v[1] == v[2] || !is_square((u128)v[0] + v[1]) || !is_square((u128)v[0] + v[2]) ||
-            !is_square((u128)v[1] + v[2])) {
-            ctx->last_error = "not a triple of distinct positive integers with square pair sums";
-            return PS4_E_ARG;
-        }
-        if (2 * isqrt64(v[2] + v[0]) + 2 >= ctx->tab_n) { ctx->last_error = "triple beyond the tables of this context"; return PS4_E_ARG; }
-    }
-    const u32 per = 1u << 16;
-    if (n > ctx->tq_cap) {
-        void *old[] = {ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
-        for (void *p : old) if (p) (void)hipFree(p);
-        const u32 c = (u32)std::max<size_t>(n, 64);
-        HIP_OK(hipMalloc(&ctx->d_tabc, (size_t)c * 3 * sizeof(u64)));
-        HIP_OK(hipMalloc(&ctx->d_tout, (size_t)c * per * sizeof(u64)));
-        HIP_OK(hipMalloc(&ctx->d_tcnt, (size_t)c * sizeof(u32)));
-        HIP_OK(hipMalloc(&ctx->d_terr, sizeof(u32)));
-        ctx->tq_cap = c;
-    }
-    HIP_OK(hipMemcpyAsync(ctx->d_tabc, abc, n * 3 * sizeof(u64), hipMemcpyHostToDevice, ctx->st));
-    HIP_OK(ps4_launch_fill_u32(ctx->st, ctx->d_tcnt, (u32)n, 0u));
-    HIP_OK(ps4_launch_fill_u32(ctx->st, ctx->d_terr, 1u, 0u));
-    HIP_OK(ps4_launch_triple(ctx->st, ctx->d_tabc, n, d_max, ctx->d_ftab, ctx->tab_n, ctx->d_tout, per, ctx->d_tcnt,
-                             ctx->d_terr));
-    std::vector<u32> cnt(n);
-    u32 terr = 0;
-    HIP_OK(hipMemcpyAsync(cnt.data(), ctx->d_tcnt, n * sizeof(u32), hipMemcpyDeviceToHost, ctx->st));
-    HIP_OK(hipMemcpyAsync(&terr, ctx->d_terr, sizeof(u32), hipMemcpyDeviceToHost, ctx->st));
-    HIP_OK(hipStreamSynchronize(ctx->st));
-    if (terr) { ctx->last_error = "divisor lists of a triple exceed the kernel's arena"; return PS4_E_INTERNAL; }
-    std::vector<u64> buf;
-    for (size_t i = 0; i < n; ++i) {
-        if (cnt[i] > per) { ctx->last_error = "too many solutions for one triple"; return PS4_E_CAPACITY; }
-        n_out[i] = cnt[i];
-        if (!cap || !cnt[i]) continue;
-        buf.resize(cnt[i]);
-        HIP_OK(hipMemcpy(buf.data(), ctx->d_tout + i * per, cnt[i] * sizeof(u64), hipMemcpyDeviceToHost));
-        std::sort(buf.begin(), buf.end());
-        for (size_t k = 0; k < buf.size() && k < cap; ++k) d_out[i * cap + k] = buf[k];
+        n_out[i] = ans[i].size();
+        for (size_t k = 0; k < ans[i].size() && k < cap; ++k) d_out[i * cap + k] = ans[i][k];
     }
     return 0;
 }
@@ -1093,6 +1167,191 @@ extern "C" int ps4_extend_triple(ps4_ctx *ctx, const uint64_t abc[3], uint64_t d
     return ps4_extend_triple_batch(ctx, 1, abc, d_max, d_out, cap, n_out);
 }
 
+// ---------------------------------------------------------------------------
+// serving stream: continuous batching over one context
+// ---------------------------------------------------------------------------
+
+namespace {
+
+void add_counters(ps4_counters &a, const ps4_counters &b)
+{
+    a.S_done += b.S_done; a.pairs += b.pairs; a.S_ge3 += b.S_ge3; a.triples += b.triples; a.sets4 += b.sets4;
+    a.sets5 += b.sets5; a.sets6 += b.sets6; a.multi += b.multi; a.ext_cand += b.ext_cand; a.ext_sqrt += b.ext_sqrt;
+    for (int i = 0; i < 100; ++i) a.hist5[i] += b.hist5[i];
+    a.max_reps = std::max(a.max_reps, b.max_reps);
+    a.max_divisors = std::max(a.max_divisors, b.max_divisors);
+    a.max_extensions = std::max(a.max_extensions, b.max_extensions);
+}
+
+int append_sink(void *user, const char *lines, size_t n)
+{
+    ((std::string *)user)->append(lines, n);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int ps4_submit_range(ps4_ctx *ctx, uint64_t tag, uint64_t S_lo, uint64_t S_hi)
+{
+    if (!ctx) return PS4_E_ARG;
+    const u64 lo = S_lo < 1 ? 1 : S_lo;
+    if (S_hi < lo) { ctx->last_error = "S_hi < S_lo"; return PS4_E_ARG; }
+    if (S_hi > ctx->S_hi_max) { ctx->last_error = "S_hi beyond the tables of this context (S_hi_max)"; return PS4_E_ARG; }
+    ps4_ctx::StreamReq r;
+    r.tag = tag;
+    r.lo = r.cur = lo;
+    r.hi = S_hi;
+    memset(&r.c, 0, sizeof r.c);
+    ctx->sq.push_back(std::move(r));
+    return 0;
+}
+
+extern "C" int ps4_submit_triple(ps4_ctx *ctx, uint64_t tag, const uint64_t abc[3], uint64_t d_max)
+{
+    if (!ctx || !abc) return PS4_E_ARG;
+    if (int rc = check_triple(ctx, abc)) return rc;
+    ps4_ctx::StreamReq r;
+    r.tag = tag;
+    r.triple = true;
+    for (int i = 0; i < 3; ++i) r.abc[i] = abc[i];
+    r.d_max = d_max;
+    memset(&r.c, 0, sizeof r.c);
+    ctx->sq.push_back(std::move(r));
+    return 0;
+}
+
+// One step of the stream: the range requests in order of their remaining
+// work (shortest first) take the next part of their interval as long as the
+// step's expected 4-sets fit max_quads and the free part of the 4-set queue
+// (a request larger than what is left is cut, so every step is short), all
+// run as one batch of shared persistent launches; the pending triple queries
+// of one d_max (up to the triple buffers, at most 256) run beside them on a
+// second stream.  Requests finished by this step are handed to done.
+extern "C" int ps4_step(ps4_ctx *ctx, uint64_t max_quads, ps4_done_fn done, void *user, ps4_stream_state *state)
+{
+    if (!ctx) return PS4_E_ARG;
+    HIP_OK(hipSetDevice(ctx->device));
+    std::vector<ps4_ctx::StreamReq> &q = ctx->sq;
+    const double free_quads = ctx->batch_frac * (double)ctx->quad_cap;
+    double budget = max_quads ? std::min((double)max_quads, free_quads) : std::min(free_quads, 8388608.0);
+    // ---- admission: triples of the oldest pending triple's d_max
+    std::vector<size_t> tri;
+    for (size_t i = 0; i < q.size() && tri.size() < 256; ++i)
+        if (q[i].triple && (tri.empty() || q[i].d_max == q[tri[0]].d_max)) tri.push_back(i);
+    // ---- admission: range parts, shortest remaining first
+    std::vector<size_t> order;
+    for (size_t i = 0; i < q.size(); ++i)
+        if (!q[i].triple) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return q[x].hi - q[x].cur < q[y].hi - q[y].cur; });
+    std::vector<size_t> rng;
+    std::vector<u64> los, his;
+    for (size_t i : order) {
+        ps4_ctx::StreamReq &r = q[i];
+        u64 len = r.hi - r.cur;
+        const double dens = quad_density(r.hi);
+        if (len > 0 && dens * (double)len > budget) {
+            if (!rng.empty() && budget < dens * (double)ctx->W) break;
+            len = std::max<u64>((u64)(budget / dens) / ctx->W * ctx->W, ctx->W);
+            len = std::min(len, r.hi - r.cur);
+        }
+        rng.push_back(i);
+        los.push_back(r.cur);
+        his.push_back(r.cur + len);
+        budget -= dens * (double)len;
+        if (budget <= 0.0) break;
+    }
+    // ---- device work: the triples on the second stream, the range parts as one batched search
+    std::vector<std::vector<u64>> tans;
+    std::vector<u64> tabc(3 * tri.size());
+    for (size_t k = 0; k < tri.size(); ++k)
+        for (int j = 0; j < 3; ++j) tabc[3 * k + j] = q[tri[k]].abc[j];
+    if (!tri.empty())
+        if (int rc = triple_enqueue(ctx, ctx->st2, tri.size(), tabc.data(), q[tri[0]].d_max)) return rc;
+    std::vector<ps4_counters> cs(rng.size());
+    std::vector<std::string> txt(rng.size());
+    std::vector<void *> users(rng.size());
+    for (size_t k = 0; k < rng.size(); ++k) users[k] = &txt[k];
+    int rc_r = 0;
+    if (!rng.empty()) {
+        const int64_t dump4 = ctx->dump4_limit;
+        ctx->dump4_limit = 0;
+        rc_r = run_job(ctx, rng.size(), los.data(), his.data(), append_sink, users.data(), nullptr, cs.data());
+        ctx->dump4_limit = dump4;
+    }
+    int rc_t = tri.empty() ? 0 : triple_collect(ctx, ctx->st2, tri.size(), tans);
+    // a failed part is run again alone, so that only the request it belongs to fails
+    std::vector<int> rc_each(rng.size(), 0);
+    if (rc_r) {
+        for (size_t k = 0; k < rng.size(); ++k) {
+            txt[k].clear();
+            void *u = &txt[k];
+            const int64_t dump4 = ctx->dump4_limit;
+            ctx->dump4_limit = 0;
+            rc_each[k] = run_job(ctx, 1, &los[k], &his[k], append_sink, &u, nullptr, &cs[k]);
+            ctx->dump4_limit = dump4;
+        }
+    }
+    std::vector<int> rc_tri(tri.size(), rc_t);
+    if (rc_t) {   // one query at a time, likewise
+        for (size_t k = 0; k < tri.size(); ++k) {
+            std::vector<std::vector<u64>> one;
+            rc_tri[k] = triple_enqueue(ctx, ctx->st2, 1, &tabc[3 * k], q[tri[k]].d_max);
+            if (!rc_tri[k]) rc_tri[k] = triple_collect(ctx, ctx->st2, 1, one);
+            if (!rc_tri[k]) { tans.resize(tri.size()); tans[k] = one[0]; }
+        }
+    }
+    // ---- completions
+    std::vector<char> gone(q.size(), 0);
+    for (size_t k = 0; k < rng.size(); ++k) {
+        ps4_ctx::StreamReq &r = q[rng[k]];
+        if (!rc_each[k]) {
+            add_counters(r.c, cs[k]);
+            r.lines += txt[k];
+            r.cur = his[k];
+        }
+        if (rc_each[k] || r.cur >= r.hi) {
+            ps4_done d;
+            memset(&d, 0, sizeof d);
+            d.tag = r.tag;
+            d.rc = rc_each[k];
+            d.kind = 0;
+            d.counters = r.c;
+            d.lines = r.lines.c_str();
+            d.nbytes = r.lines.size();
+            if (done) done(user, &d);
+            gone[rng[k]] = 1;
+        }
+    }
+    for (size_t k = 0; k < tri.size(); ++k) {
+        ps4_done d;
+        memset(&d, 0, sizeof d);
+        d.tag = q[tri[k]].tag;
+        d.rc = rc_tri[k];
+        d.kind = 1;
+        if (!rc_tri[k]) {
+            d.d = tans[k].data();
+            d.n_d = tans[k].size();
+        }
+        if (done) done(user, &d);
+        gone[tri[k]] = 1;
+    }
+    size_t w = 0;
+    for (size_t i = 0; i < q.size(); ++i)
+        if (!gone[i]) {
+            if (w != i) q[w] = std::move(q[i]);
+            ++w;
+        }
+    q.resize(w);
+    if (state) {
+        memset(state, 0, sizeof *state);
+        for (const auto &r : q) (r.triple ? state->pending_triples : state->pending_ranges) += 1;
+        state->quad_slots = (uint64_t)free_quads;
+        state->hit_slots = ctx->hit_cap;
+        state->triple_slots = std::max<u64>(ctx->tq_cap, 256);
+    }
+    return 0;
+}
+
 // The triple query without any allocation: the query, its count and its
 // answers are staged in the device's fixed scratch (ps4_noalloc_scratch), the
 // answers are copied into the caller's d_out and sorted there in place.
@@ -1101,14 +1360,7 @@ extern "C" int ps4_extend_triple_noalloc(ps4_ctx *ctx, const uint64_t abc[3], ui
 {
     if (!ctx || !abc || !n_out || (cap && !d_out)) return PS4_E_ARG;
     *n_out = 0;
-    u64 v[3] = {abc[0], abc[1], abc[2]};
-    std::sort(v, v + 3);
-    if (v[0] == 0 || v[0] == v[1] || v[1] == v[2] || !is_square((u128)v[0] + v[1]) || !is_square((u128)v[0] + v[2]) ||
-        !is_square((u128)v[1] + v[2])) {
-        ctx->last_error = "not a triple of distinct positive integers with square pair sums";
-        return PS4_E_ARG;
-    }
-    if (2 * isqrt64(v[2] + v[0]) + 2 >= ctx->tab_n) { ctx->last_error = "triple beyond the tables of this context"; return PS4_E_ARG; }
+    if (int rc = check_triple(ctx, abc)) return rc;
     if (!ctx->na_scratch) { ctx->last_error = "no fixed scratch on this device"; return PS4_E_INTERNAL; }
     HIP_OK(hipSetDevice(ctx->device));
     u64 *s_abc = (u64 *)ctx->na_scratch;

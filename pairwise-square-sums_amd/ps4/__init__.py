@@ -86,7 +86,20 @@ def format_tuning(d, comment=""):
     return head + "".join(f"{k} = {int(d[k])}\n" for k in Tuning.KEYS if k in d)
 
 
+class Done(ctypes.Structure):
+    """One finished request of the serving stream (ps4_done)."""
+    _fields_ = [("tag", ctypes.c_uint64), ("rc", ctypes.c_int), ("kind", ctypes.c_int), ("counters", Counters),
+                ("lines", ctypes.c_void_p), ("nbytes", ctypes.c_size_t), ("d", ctypes.POINTER(ctypes.c_uint64)),
+                ("n_d", ctypes.c_size_t)]
+
+
+class StreamState(ctypes.Structure):
+    _fields_ = [("pending_ranges", ctypes.c_uint64), ("pending_triples", ctypes.c_uint64),
+                ("quad_slots", ctypes.c_uint64), ("hit_slots", ctypes.c_uint64), ("triple_slots", ctypes.c_uint64)]
+
+
 SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t)
+DONE = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(Done))
 ALERT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p)
 
 _lib = None
@@ -118,6 +131,9 @@ def load():
     lib.ps4_extend_triple_batch.argtypes = [P, ctypes.c_size_t, u64p, u64, u64p, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)]
     lib.ps4_search_elements.argtypes = [P, u64, SINK, P, u64p, u64p]
+    lib.ps4_submit_range.argtypes = [P, u64, u64, u64]
+    lib.ps4_submit_triple.argtypes = [P, u64, u64p, u64]
+    lib.ps4_step.argtypes = [P, u64, DONE, P, ctypes.POINTER(StreamState)]
     lib.ps4_set_alert.argtypes = [P, ALERT, P]
     lib.ps4_set_dump4.argtypes = [P, ctypes.c_int64]
     lib.ps4_get_tuning.argtypes = [P, ctypes.POINTER(Tuning), ctypes.POINTER(Tuning)]
@@ -258,6 +274,34 @@ class Engine:
         self._check(self.lib.ps4_extend_triple_batch(self.ctx, n, arr, int(d_max), out, cap, cnt))
         return [[int(out[i * cap + k]) for k in range(cnt[i])] if cnt[i] <= cap
                 else self.extend_triple(triples[i], d_max) for i in range(n)]
+
+    # ---- serving stream (ps4_submit_*, ps4_step): the caller drives the steps
+    def submit_range(self, tag, lo, hi):
+        self._check(self.lib.ps4_submit_range(self.ctx, int(tag), int(lo), int(hi)))
+
+    def submit_triple(self, tag, abc, d_max):
+        arr = (ctypes.c_uint64 * 3)(*[int(x) for x in abc])
+        self._check(self.lib.ps4_submit_triple(self.ctx, int(tag), arr, int(d_max)))
+
+    def step(self, max_quads=0):
+        """One step of the stream: a list of (tag, error or None, result) for
+        the requests it finished (result: (counters, lines) or the list of d),
+        and the state after it (dict)."""
+        out = []
+
+        def cb(_user, dp):
+            d = dp.contents
+            if d.rc:
+                out.append((int(d.tag), PS4Error(d.rc, self.lib.ps4_last_error(self.ctx).decode()), None))
+            elif d.kind == 0:
+                text = ctypes.string_at(d.lines, d.nbytes).decode() if d.nbytes else ""
+                out.append((int(d.tag), None, (d.counters.as_dict(), text.splitlines())))
+            else:
+                out.append((int(d.tag), None, [int(d.d[i]) for i in range(d.n_d)]))
+        f = DONE(cb)
+        st = StreamState()
+        self._check(self.lib.ps4_step(self.ctx, int(max_quads), f, None, ctypes.byref(st)))
+        return out, {k: int(getattr(st, k)) for k, _ in StreamState._fields_}
 
     def kernel_time(self, reset=False):
         """(ms in k_rq, ms in k_ext, batches) accumulated on the device stream."""

@@ -43,7 +43,7 @@ int main(int argc, char **argv)
     long long dump4 = 0;
     unsigned long long smax = 0;
     std::vector<unsigned long long> nums;
-    bool triple = false, batch = false, elements = false;
+    bool triple = false, batch = false, elements = false, stream = false;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--noext")) flags |= PS4_NOEXT;
         else if (!strcmp(argv[i], "--dump4")) dump4 = atoll(argv[++i]);
@@ -51,6 +51,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "--triple")) triple = true;
         else if (!strcmp(argv[i], "--batch")) batch = true;
         else if (!strcmp(argv[i], "--elements")) elements = true;
+        else if (!strcmp(argv[i], "--stream")) stream = true;
         else nums.push_back(strtoull(argv[i], 0, 10));
     }
     ps4_config cfg;
@@ -87,6 +88,48 @@ int main(int argc, char **argv)
         printf("triple %s n=%zu want=%lld rc=%d\n", ok ? "OK" : "MISMATCH", n, (long long)m, rc);
         ps4_close(ctx);
         return ok ? 0 : 1;
+    }
+    if (stream) {   // every range through the serving stream, small steps, plus one triple query
+        struct Got { ps4_counters c; std::string text; int rc; bool seen; std::vector<uint64_t> d; };
+        std::vector<Got> got(nums.size() / 2 + 1);
+        for (size_t r = 0; r + 1 < nums.size(); r += 2) ps4_submit_range(ctx, r / 2, nums[r], nums[r + 1]);
+        const uint64_t abc[3] = {482, 359, 2};
+        ps4_submit_triple(ctx, nums.size() / 2, abc, 1000000);
+        auto cb = [](void *u, const ps4_done *d) {
+            Got &g = (*(std::vector<Got> *)u)[d->tag];
+            g.c = d->counters;
+            g.text.assign(d->lines ? d->lines : "", d->nbytes);
+            g.rc = d->rc;
+            g.seen = true;
+            g.d.assign(d->d, d->d + d->n_d);
+        };
+        ps4_stream_state stt;
+        int steps = 0;
+        do {
+            rc = ps4_step(ctx, 20000, cb, &got, &stt);
+            if (rc) { printf("step rc=%d %s\n", rc, ps4_last_error(ctx)); return 1; }
+            ++steps;
+        } while (stt.pending_ranges + stt.pending_triples > 0 && steps < 100000);
+        int bad_s = 0;
+        for (size_t r = 0; r + 1 < nums.size(); r += 2) {
+            Got &g = got[r / 2];
+            std::string want;
+            ora_counters w;
+            ora_census(nums[r], nums[r + 1], 0u, 0, ora_line, &want, &w);
+            const bool ok = g.seen && g.rc == 0 && g.c.sets4 == w.sets4 && g.c.ext_cand == w.ext_cand &&
+                            g.c.ext_sqrt == w.ext_sqrt && g.c.sets5 == w.sets5 && g.c.pairs == w.pairs &&
+                            g.c.max_divisors == w.max_divisors && split_sorted(g.text) == split_sorted(want);
+            printf("stream [%llu,%llu) %s 4-sets=%llu\n", (unsigned long long)nums[r], (unsigned long long)nums[r + 1],
+                   ok ? "OK" : "MISMATCH", (unsigned long long)g.c.sets4);
+            bad_s |= !ok;
+        }
+        uint64_t want_d[64];
+        const int64_t m = ora_extend_triple(abc, 1000000, want_d, 64);
+        const Got &gt = got.back();
+        const bool tok = gt.seen && gt.rc == 0 && (int64_t)gt.d.size() == m && std::equal(gt.d.begin(), gt.d.end(), want_d);
+        printf("stream triple %s n=%zu steps=%d\n", tok ? "OK" : "MISMATCH", gt.d.size(), steps);
+        ps4_close(ctx);
+        return (bad_s || !tok) ? 1 : 0;
     }
     const size_t nr = nums.size() / 2;
     std::vector<std::string> btext(nr);

@@ -552,7 +552,7 @@ def test_serving_results_same_alone_and_in_the_mix(eng):
     stream = workloads.serving_stream(80, rate=2000.0, seed=7, s_top=10**10, len_e0=3, len_e1=6, c_max=10**7,
                                       d_max=10**10)
     say(f"serving {len(stream)} requests")
-    srv = serve.Server(eng, out_budget_bytes=1 << 20)
+    srv = serve.Server(eng, step_quads=1 << 14)   # small steps: requests are cut into several parts
     t0 = time.perf_counter()
     reqs = []
     for t_arr, kind, a in stream:
@@ -563,7 +563,7 @@ def test_serving_results_same_alone_and_in_the_mix(eng):
     for r in reqs:
         assert r.wait(120)
     srv.close()
-    assert len(srv.steps) < len(reqs)           # some steps served several requests
+    assert any(n > 1 for n, _ in srv.steps)     # some steps finished several requests
     for r in reqs:
         assert r.error is None
         if r.kind == "range":

@@ -212,3 +212,43 @@ def test_cli_kill_after_checkpoint_then_resume(tmp_path, golden):
     lines = set(out.read_text().splitlines())
     assert len(lines) == rec["n_lines"]
     assert sha_sorted(lines) == rec["sorted_sha256"]
+
+
+def test_serving_stream_at_stated_size():
+    """The serving workload at its stated size: an open-loop Poisson stream at
+    200 requests/s (seed 7), 70 % range queries with lo uniform in [1, 1e12]
+    and lengths log-uniform in [1e3, 1e8], 30 % triple queries with c <= 1e9
+    asking for every d <= 1e12, through the continuous-batching loop.  Every
+    result equals the same request run alone; a refused request (not a square
+    triple, a range beyond the tables) fails alone without touching the others."""
+    from ps4 import serve
+    stream = workloads.serving_stream(400, rate=200.0, seed=7)
+    e = ps4.Engine(S_hi_max=10**12 + 10**8 + 1)
+    try:
+        e.search(10**12 - 10**6, 10**12)
+        srv = serve.Server(e)
+        say(f"serving {len(stream)} requests at 200/s")
+        t0 = time.perf_counter()
+        reqs = []
+        for k, (t_arr, kind, a) in enumerate(stream):
+            dt = t0 + t_arr - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            reqs.append(srv.submit(kind, a))
+            if k == 100:
+                bad = [srv.submit("triple", ((1, 2, 3), 10**12)), srv.submit("range", (10**13, 10**13 + 5))]
+        for r in reqs + bad:
+            assert r.wait(300)
+        srv.close()
+        assert all(isinstance(r.error, ps4.PS4Error) for r in bad)
+        assert max(n for n, _ in srv.steps) > 1          # requests share steps
+        say("serving: every request alone")
+        for r in reqs:
+            assert r.error is None, (r.args, r.error)
+            if r.kind == "range":
+                alone = e.search(*r.args)
+                assert alone[0] == r.result[0] and sorted(alone[1]) == sorted(r.result[1]), r.args
+            else:
+                assert e.extend_triple(*r.args) == r.result, r.args
+    finally:
+        e.close()
