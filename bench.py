@@ -7,7 +7,7 @@ reference (panjanek/pairwise-square-sums, exhaustive/s4.c).  The whole range
 is the input; nothing is cached between steps.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
-                    [--lo 1] [--hi 10000000000] [--no-cpu] [--no-latency]
+                    [--lo 1] [--hi 10000000000] [--no-cpu] [--no-latency] [--no-workloads]
     python bench.py --workload batched|band|serving|elements
     (any of them) [--tune FILE]  tuning file from ps4/autotune.py
                   [--graph]      every batch launched as one captured hipGraph
@@ -36,11 +36,16 @@ Also reported:
                 ops = 256*4sets + 48*ext_cand + 96*ext_sqrt for that kernel;
   roofline_job  the whole census: 4*S + 12*pairs + 40*triples + 256*4sets +
                 48*ext_cand + 96*ext_sqrt over the wall time;
-  cpu_baseline  the reference s4 binary (oracle/_ref/s4, built from the
-                reference's source) on this host's cores, over a bounded sample
-                of the same range;
+  cpu_baseline  the reference s4 (built from the reference's source into
+                oracle/_ref/) on every CPU the process may use (nproc), over
+                three subranges spread over the range, median of three runs of
+                about 30 s, with the CPU model, nproc and the -march used;
   latency_ms    p50/p99 of range queries (lengths log-uniform 1e3..1e8, seed 7)
-                answered one at a time.
+                answered one at a time;
+  batched       the batched workload's S/s and its ratio to one call per
+                subrange (per-subrange results compared);
+  serving       the serving workload (1000 requests): p50/p99 latency,
+                throughput, and results alone against in the mix.
 """
 import argparse
 import hashlib
@@ -77,37 +82,77 @@ def op_model(c):
     return rq, ext
 
 
-def cpu_baseline(lo, hi, threads):
-    """Time the reference s4 (built from exhaustive/s4.c) on a bounded, spread sample."""
-    s4 = os.path.join(REPO, "oracle", "_ref", "s4")
-    kind = "reference"
-    if not os.path.exists(s4):
-        s4 = os.path.join(REPO, "oracle", "_bin", "ps4_oracle")
-        kind = "port"
-        threads = 1
+def host_cpu():
+    """(model name, nproc = CPUs this process may run on, os.cpu_count(), flags)."""
+    model, flags = None, set()
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, timeout=30).stdout.decode()
+        m = re.search(r"^Model name:\s*(.+)$", out, re.M)
+        model = m.group(1).strip() if m else None
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if model is None and line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0)), os.cpu_count(), flags
+
+
+def cpu_baseline(lo, hi, target_s=30.0, runs=3, nsub=3):
+    """The reference s4 (built from exhaustive/s4.c into oracle/_ref/) on every
+    CPU this process may use (nproc, s4's own default), over nsub subranges
+    spread evenly over [lo, hi), each a separate s4 process timed from outside:
+    one calibration run, then `runs` runs sized to about target_s seconds each,
+    reported as their median S/s.  The -march=x86-64-v3 build is used where the
+    host has AVX2, BMI2 and FMA."""
+    model, nproc, ncpu, flags = host_cpu()
+    ref = os.path.join(REPO, "oracle", "_ref")
+    s4, march = os.path.join(ref, "s4_v3"), "x86-64-v3"
+    if not ({"avx2", "bmi2", "fma"} <= flags and os.path.exists(s4)):
+        s4, march = os.path.join(ref, "s4"), "x86-64"
     if not os.path.exists(s4):
         return None
+    threads = nproc
     span = hi - lo
-    nsub, width = 10, min(10**8, max(1, span // 100))
-    if kind == "port":
-        width = min(width, 2 * 10**6)
-    done = 0
-    t_total = 0.0
-    with tempfile.TemporaryDirectory() as tmp:
-        for k in range(nsub):
-            a = lo + (span * k) // nsub + (span // nsub - width) // 2
-            args = [s4, str(a), str(a + width)] + ([str(threads), "20"] if kind == "reference" else [])
-            t0 = time.perf_counter()
-            r = subprocess.run(args, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, timeout=600)
-            t_total += time.perf_counter() - t0
-            if r.returncode != 0:
-                return None
-            m = re.search(rb"S_done=(\d+)", r.stderr)
-            done += int(m.group(1)) if m else width
-    return {"value": done / t_total, "unit": "S/s", "cores": threads, "kind": kind,
-            "sample": f"{nsub} subranges of {width} S spread evenly over [{lo},{hi}), "
-                      f"{'s4 with ' + str(threads) + ' threads' if kind == 'reference' else 'single-thread port'}, "
-                      f"{t_total:.1f} s"}
+
+    def run(width):
+        done, wall, busy = 0, 0.0, 0.0
+        with tempfile.TemporaryDirectory() as tmp:
+            for k in range(nsub):
+                a = lo + (span * k) // nsub + (span // nsub - width) // 2
+                t0 = time.perf_counter()
+                r = subprocess.run([s4, str(a), str(a + width), str(threads), "20"], cwd=tmp,
+                                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, timeout=600)
+                wall += time.perf_counter() - t0
+                if r.returncode != 0:
+                    return None
+                err = r.stderr.decode()
+                done += int(re.search(r"S_done=(\d+)", err).group(1))
+                # the per-thread compute seconds (exhaustive/s4.c:496): s4's own wall
+                # is rounded up by its one-second monitor poll (:473)
+                busy += max(float(x) for x in re.findall(r"thread\s+\d+:.*?([\d.]+)s\s*$", err, re.M) or ["0"])
+        return done, wall, busy
+    cal = run(min(10**8, span // nsub))
+    if cal is None:
+        return None
+    width = int(min(span // nsub, max(10**7, cal[0] / cal[1] * target_s / nsub)))
+    log(f"cpu baseline: s4 ({march}) on {threads} threads, {runs} runs of {nsub} x {width} S")
+    res = [run(width) for _ in range(runs)]
+    if any(x is None for x in res):
+        return None
+    rates = sorted(d / w for d, w, _ in res)
+    return {"value": rates[len(rates) // 2], "unit": "S/s", "cores": threads, "kind": "reference",
+            "sample": f"{nsub} subranges of {width} S spread evenly over [{lo},{hi}), one s4 process each, "
+                      f"median of {runs} runs ({', '.join(f'{w:.1f}' for _, w, _ in res)} s)",
+            "runs_S_per_s": rates, "compute_S_per_s": sorted(d / b for d, _, b in res if b > 0),
+            "binary": f"oracle/_ref/{os.path.basename(s4)}: gcc -O3 -march={march} exhaustive/s4.c",
+            "cpu_model": model, "nproc": nproc, "cpu_count": ncpu}
 
 
 def latency(eng, hi, n=100, seed=7):
@@ -381,10 +426,12 @@ def main():
     ap.add_argument("--hi", type=int, default=10**10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="plain census only (skip the batched and serving summaries)")
     ap.add_argument("--workload", choices=["plain", "batched", "band", "serving", "elements"], default="plain")
     ap.add_argument("--tune", default=None, help="tuning file from ps4/autotune.py (launch shape only)")
     ap.add_argument("--graph", action="store_true", help="launch every batch as one captured hipGraph")
-    ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
+    ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once (a plain copy of known size that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters in a profiled run)")
@@ -429,7 +476,8 @@ def main():
         gold = json.load(open(GOLDEN))
         rec = next((r for r in gold["runs"] if (r["lo"], r["hi"]) == (lo, hi)), None)
         if rec:
-            exact = (rec["counters"] == c and rec["hist5"] == res["hist5"] and len(lines) == rec["n_lines"])
+            exact = (rec["counters"] == c and rec["hist5"] == res["hist5"] and len(lines) == rec["n_lines"]
+                     and lines_sha(lines) == rec["sorted_sha256"])
     out = {
         "metric": "elements/sec exhausted to 1e10 (bit-exact vs s4.c); p50/p99 range-query latency",
         "value": value,
@@ -460,10 +508,20 @@ def main():
         log("range-query latency")
         out["latency_ms"] = latency(eng, hi)
     eng.close()
+    if not args.no_workloads:
+        # the batched and serving workloads, summarised (each also has its own
+        # --workload line with every detail)
+        b = bench_batched(args)
+        out["batched"] = {k: b[k] for k in ("value", "unit", "plain_value", "batched_over_plain",
+                                             "per_subrange_identical", "seconds_batched", "seconds_plain")}
+        out["batched"]["workload"] = b["config"]["workload"]
+        sv = bench_serving(args)
+        out["serving"] = {"latency_ms": sv["latency_ms"], "throughput": sv["throughput"],
+                          "same_alone_and_in_mix": sv["same_alone_and_in_mix"], "requests": sv["config"]["requests"],
+                          "workload": sv["config"]["workload"], "errors": sv["errors"]}
     if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        log(f"cpu baseline on {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(lo, hi, threads)
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(lo, hi)
     print(json.dumps(out), flush=True)
 
 

@@ -188,15 +188,16 @@ int ps4_kernel_time(ps4_ctx *ctx, double *ms_rq, double *ms_ext, uint64_t *batch
  * are its thread count and log2 chunk (exhaustive/s4.c:424-426, 451-456). */
 typedef struct {
     uint32_t rq_grid;        /* workgroups of the persistent representation kernel (k_rq)          */
-    uint32_t ext_grid;       /* single-wave workgroups of the extension kernel (k_ext), factor
-                                lists in global scratch                                             */
-    uint32_t ext_grid_lds;   /* the same for k_ext's form with the factor lists in LDS              */
+    uint32_t ext_grid;       /* single-wave workgroups of the extension kernel (k_ext)              */
+    uint32_t rq_block;       /* threads per k_rq workgroup: 128, 256 or 512 (a compiled shape)      */
     uint32_t sw_shift;       /* subrange split: superwindows 2^-sw_shift of the default length (a
                                 power of two near 2 sqrt(S)), never shorter than one 4096-S window  */
     uint32_t batch_pct;      /* a batch is planned to fill this percentage of the 4-set queue       */
     uint32_t n_cu;           /* the device's compute units (reported by ps4_get_tuning; ignored by
                                 ps4_set_tuning)                                                     */
-    uint64_t lds_fac_above;  /* batches reaching above this S run k_ext's LDS-list form             */
+    uint32_t rq_tile;        /* window points a k_rq thread loads at once in its bucket passes:
+                                2, 4 or 8                                                           */
+    uint32_t ext_chunk;      /* tiles of 64 queued 4-sets a k_ext workgroup claims at once: 1, 4, 16 */
 } ps4_tuning;
 
 /* The values in force (*cur) and the largest each field may take (*max; grids:
@@ -204,9 +205,10 @@ typedef struct {
  * Either pointer may be NULL. */
 int ps4_get_tuning(ps4_ctx *ctx, ps4_tuning *cur, ps4_tuning *max);
 
-/* Set every field; a field of 0 selects its default (resident maximum for the
- * grids, 0 for sw_shift, 80 for batch_pct, 2^35 for lds_fac_above).  A value
- * above its maximum is PS4_E_ARG and changes nothing. */
+/* Set every field; a field of 0 selects its default (the resident maximum of
+ * the shape for the grids, 256 / 4 / 4 for rq_block / rq_tile / ext_chunk, 0 for
+ * sw_shift, 80 for batch_pct).  A value above its maximum, or a shape that is
+ * not compiled, is PS4_E_ARG and changes nothing. */
 int ps4_set_tuning(ps4_ctx *ctx, const ps4_tuning *t);
 
 /* Parse a tuning file: "key = value" lines, keys named as the ps4_tuning

@@ -238,7 +238,11 @@ __device__ __forceinline__ u32 tri3(u32 k) { return __umulhi((k * (k - 1) >> 1) 
 // k_rq: representations and 4-sets
 // ----------------------------------------------------------------------------
 
-#define RQ_NT 256
+// threads per workgroup (NT) and window points a thread loads at once in the
+// bucket passes (UN) are template parameters of k_rq (the launch shape the
+// autotuner chooses among); 256 and 4 by default
+#define RQ_NT_DEFAULT 256
+#define RQ_UN_DEFAULT 4
 #define RQ_LOGW 12          // windows of 4096 S: 29 KB of LDS, 5 workgroups (20 waves) per CU
 #define RQ_W (1u << RQ_LOGW)
 #define RQ_PV_CAP 1536      // representations of the counted S of a window (about 800 at
@@ -246,7 +250,6 @@ __device__ __forceinline__ u32 tri3(u32 k) { return __umulhi((k * (k - 1) >> 1) 
 #define RQ_QS_CAP 512       // counted S of a window (about 140, at most about 160 in samples)
 #define RQ_NB_MAX 1024
 #define RQ_K_MAX 2000u      // largest bucket whose triples a window indexes (C(2000,3) < 2^31)
-#define RQ_UNROLL 4         // window points a thread loads at once in the bucket passes
 
 struct RqArgs {
     const ps4_item *items;
@@ -453,7 +456,8 @@ __device__ u32 rq_walk(u32 c0, u32 c1, u32 nit, const u32 *s_sk, const u32 *s_st
     return n;
 }
 
-__global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
+template <int NT, int UN>
+__global__ __launch_bounds__(NT) void k_rq(RqArgs a)
 {
     __shared__ u32 s_cnt[RQ_W / 2];        // u16 bucket counts, two per word; later offsets
     __shared__ u32 s_pv[RQ_PV_CAP];        // p of every representation of a counted S
@@ -462,8 +466,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
     __shared__ u32 s_st[RQ_QS_CAP];        // first slot in s_pv
     __shared__ u32 s_tp[RQ_QS_CAP + 1];    // triple prefix
     __shared__ u32 s_bin[RQ_NB_MAX];       // points per window of the superwindow
-    __shared__ u32 s_w[2 * RQ_NT / 64];
-    __shared__ u64 s_w64[RQ_NT / 64];
+    __shared__ u32 s_w[2 * NT / 64];
+    __shared__ u64 s_w64[NT / 64];
     __shared__ u32 s_item;
 
     const u32 tid = threadIdx.x;
@@ -494,7 +498,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             if (tid == 0) atomicOr(&a.ctl->err, (u32)ERR_BOUNDS);
             continue;
         }
-        for (u32 i = tid; i < nb; i += RQ_NT) s_bin[i] = 0;
+        for (u32 i = tid; i < nb; i += NT) s_bin[i] = 0;
         __syncthreads();
 
         // ---- every lattice point 1 <= p <= q, X0 <= p^2+q^2 < X1, binned by window.
@@ -502,7 +506,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
         // circle incrementally: off = p^2 + q^2 - X0 with q the least admissible root.
         {
             const u64 pmax = d_isqrt64((X1 - 1) >> 1);
-            const u64 per = (pmax + RQ_NT - 1) / RQ_NT;
+            const u64 per = (pmax + NT - 1) / NT;
             u64 p = 1 + (u64)tid * per;
             if (per > 0 && p <= pmax) {
                 const u64 pend = (p + per - 1 < pmax) ? p + per - 1 : pmax;
@@ -549,21 +553,22 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             u32 npts = uni(s_bin[b]);
             if (npts > a.bin_cap) { err |= ERR_BIN_OVF; npts = a.bin_cap; }
             const u64 *pts = scr + (u64)b * a.bin_cap;
-            for (u32 i = tid; i < RQ_W / 8; i += RQ_NT) ((uint4 *)s_cnt)[i] = make_uint4(0u, 0u, 0u, 0u);
+            for (u32 i = tid; i < RQ_W / 8; i += NT) ((uint4 *)s_cnt)[i] = make_uint4(0u, 0u, 0u, 0u);
             __syncthreads();
-            // pass 1: count (exhaustive/s4.c:285-298); RQ_UNROLL points per thread
+            // pass 1: count (exhaustive/s4.c:285-298); UN points per thread
             // loaded together (independent loads in flight), then counted
-            for (u32 i0 = tid; i0 < npts; i0 += RQ_UNROLL * RQ_NT) {
-                u32 sv[RQ_UNROLL];
+            for (u32 i0 = tid; i0 < npts; i0 += UN * NT) {
+                u32 sv[UN];
 #pragma unroll
-                for (u32 u = 0; u < RQ_UNROLL; ++u) {
-                    const u32 i = i0 + u * RQ_NT;
+                for (u32 u = 0; u < UN; ++u) {
+                    const u32 i = i0 + u * NT;
                     sv[u] = (u32)pts[i < npts ? i : i0] & (RQ_W - 1u);
                 }
-                PS4_KEEP("v"(sv[0]), "v"(sv[1]), "v"(sv[2]), "v"(sv[3]));   // all loads issued before the branches
 #pragma unroll
-                for (u32 u = 0; u < RQ_UNROLL; ++u)
-                    if (i0 + u * RQ_NT < npts) atomicAdd(&s_cnt[sv[u] >> 1], 1u << ((sv[u] & 1u) << 4));
+                for (u32 u = 0; u < UN; ++u) PS4_KEEP("v"(sv[u]));   // all loads issued before the branches
+#pragma unroll
+                for (u32 u = 0; u < UN; ++u)
+                    if (i0 + u * NT < npts) atomicAdd(&s_cnt[sv[u] >> 1], 1u << ((sv[u] & 1u) << 4));
             }
             __syncthreads();
             // pass 2: which S count, offsets (exhaustive/s4.c:300-305, 314-321).
@@ -571,7 +576,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             // words, read as 16-byte words.  Counted S (k >= 3, S != 2 mod 4) are a few per cent,
             // so one sweep sums and maxes the counts two at a time and marks the
             // counted S in a 32-bit mask; only the marked S are visited after.
-            constexpr u32 SPT = RQ_W / RQ_NT;
+            constexpr u32 SPT = RQ_W / NT;
             static_assert(SPT % 8 == 0 && SPT <= 32, "whole 16-byte words, one 32-bit mask of counted S per thread");
             const u32 s0 = tid * SPT;
             const u32 b0 = (u32)B0;   // S mod 4 = (b0 + s) mod 4, and s0 = 0 mod 4
@@ -627,7 +632,7 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             // large for the 32-bit triple index (without one, the sum of C(k,3)
             // over <= 2560 representations stays below 2^31)
             u32 tot_p, tot_t, b_p, b_tr;
-            block_excl_scan2<RQ_NT>(nq | (qr << 16), big ? 0x80000000u : tr, s_w, &b_p, &b_tr, &tot_p, &tot_t);
+            block_excl_scan2<NT>(nq | (qr << 16), big ? 0x80000000u : tr, s_w, &b_p, &b_tr, &tot_p, &tot_t);
             tot_p = uni(tot_p);
             tot_t = uni(tot_t);
             u32 b_nq = b_p & 0xFFFFu, b_qr = b_p >> 16;
@@ -657,17 +662,17 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             if (tid == 0) s_tp[tot_nq] = tot_tr;
             __syncthreads();
             // pass 3: fill the buckets of counted S only (exhaustive/s4.c:307-310)
-            for (u32 i0 = tid; i0 < npts; i0 += RQ_UNROLL * RQ_NT) {
-                u64 pv[RQ_UNROLL];
+            for (u32 i0 = tid; i0 < npts; i0 += UN * NT) {
+                u64 pv[UN];
 #pragma unroll
-                for (u32 u = 0; u < RQ_UNROLL; ++u) {
-                    const u32 i = i0 + u * RQ_NT;
+                for (u32 u = 0; u < UN; ++u) {
+                    const u32 i = i0 + u * NT;
                     pv[u] = pts[i < npts ? i : i0];
                 }
 #pragma unroll
-                for (u32 u = 0; u < RQ_UNROLL; ++u) {
+                for (u32 u = 0; u < UN; ++u) {
                     const u32 s = (u32)pv[u] & (RQ_W - 1u), sh = (s & 1u) << 4;
-                    if (i0 + u * RQ_NT < npts && ((s_cnt[s >> 1] >> sh) & 0xFFFFu) != 0xFFFFu) {
+                    if (i0 + u * NT < npts && ((s_cnt[s >> 1] >> sh) & 0xFFFFu) != 0xFFFFu) {
                         const u32 old = atomicAdd(&s_cnt[s >> 1], 1u << sh);
                         const u32 slot = (old >> sh) & 0xFFFFu;
                         if (slot < tot_qr) {
@@ -681,18 +686,18 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
             // pass 4: triples -> 4-sets, appended to the wave's tiles of the queue
             const u32 T = tot_tr;   // uniform
             if (T > 0) {
-                const u32 c0 = (u32)(((u64)tid * T) / RQ_NT), c1 = (u32)(((u64)(tid + 1) * T) / RQ_NT);
-                c_sets4 += rq_walk(c0, c1, (T + RQ_NT - 1) / RQ_NT, s_sk, s_st, s_tp, s_pv, s_qv, tot_nq, B0, tl,
+                const u32 c0 = (u32)(((u64)tid * T) / NT), c1 = (u32)(((u64)(tid + 1) * T) / NT);
+                c_sets4 += rq_walk(c0, c1, (T + NT - 1) / NT, s_sk, s_st, s_tp, s_pv, s_qv, tot_nq, B0, tl,
                                    a.quads, a.quad_cap, a.ctl, item.rid, a.verify, &err);
             }
             __syncthreads();
         }
         // ---- counters of this superwindow
-        const u64 t_pairs = block_sum_u64<RQ_NT>(c_pairs, s_w64);
-        const u64 t_sge3 = block_sum_u64<RQ_NT>(c_sge3, s_w64);
-        const u64 t_trip = block_sum_u64<RQ_NT>(c_trip, s_w64);
-        const u64 t_sets4 = block_sum_u64<RQ_NT>(c_sets4, s_w64);
-        const u64 t_maxk = block_max_u64<RQ_NT>(c_maxk, s_w64);
+        const u64 t_pairs = block_sum_u64<NT>(c_pairs, s_w64);
+        const u64 t_sge3 = block_sum_u64<NT>(c_sge3, s_w64);
+        const u64 t_trip = block_sum_u64<NT>(c_trip, s_w64);
+        const u64 t_sets4 = block_sum_u64<NT>(c_sets4, s_w64);
+        const u64 t_maxk = block_max_u64<NT>(c_maxk, s_w64);
         if (err) atomicOr(&a.ctl->err, err);
         if (tid == 0) {
             unsigned long long *c = (unsigned long long *)(a.ctr + (u64)item.rid * CT_N);
@@ -717,7 +722,8 @@ __global__ __launch_bounds__(RQ_NT) void k_rq(RqArgs a)
                           // 16 KB); one 4-set with d(D) <= 16384 needs at most 748 (nL + nH
                           // over the best prefix split, reached by the radices 2, 11, 33, 2, 11)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
-#define EX_CHUNK 4        // tiles claimed per work-queue atomic
+#define EX_CHUNK_DEFAULT 4   // tiles of 64 4-sets claimed per work-queue atomic (a template
+                             // parameter of k_ext: 1, 4 or 16)
 #define EX_Q1 128         // LDS stack of candidates past the residue filters (a round of 64
                           // runs once it holds >= 64; each push adds at most 64)
 #define EX_Q2 128         // queue of candidates for the exact test, per workgroup in global
@@ -1112,6 +1118,7 @@ __device__ __forceinline__ void ex_flush(u64 *ctr, u32 n_rids, u32 rid, u64 cand
 // unordered pair f != g exactly once, and f + g needs no ordering.
 // Counters are summed per wave over consecutive tiles of one interval and
 // added to the device counters when the interval changes and at the end.
+template <int EX_CHUNK>
 __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIMD: <= 168 VGPRs
 {
     __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
@@ -1598,23 +1605,33 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
 // launch wrappers (called by the host library)
 // ----------------------------------------------------------------------------
 
-extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *items, u32 n_items, u32 bin_cap,
-                                    u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u32 n_rids, u64 *scratch, ps4_quad *quads,
-                                    u64 quad_cap, u32 cap_reps, u32 verify)
+// The compiled launch shapes of k_rq: threads per workgroup x window points a
+// thread loads at once (the autotuner's block and tile).
+#define RQ_SHAPES(X) X(128, 2) X(128, 4) X(128, 8) X(256, 2) X(256, 4) X(256, 8) X(512, 2) X(512, 4) X(512, 8)
+
+extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, u32 block, u32 tile, const ps4_item *items, u32 n_items,
+                                    u32 bin_cap, u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u32 n_rids, u64 *scratch,
+                                    ps4_quad *quads, u64 quad_cap, u32 cap_reps, u32 verify)
 {
     RqArgs a;
     a.n_rids = n_rids;
     a.items = items; a.n_items = n_items; a.bin_cap = bin_cap; a.nb_alloc = nb_alloc; a.ctl = ctl; a.ctr = ctr;
     a.scratch = scratch;
     a.quads = quads; a.quad_cap = quad_cap; a.cap_reps = cap_reps; a.verify = verify;
-    hipLaunchKernelGGL(k_rq, dim3(grid), dim3(RQ_NT), 0, st, a);
-    return hipGetLastError();
+#define RQ_LAUNCH(nt, un)                                                                 \
+    if (block == nt && tile == un) {                                                      \
+        hipLaunchKernelGGL((k_rq<nt, un>), dim3(grid), dim3(nt), 0, st, a);               \
+        return hipGetLastError();                                                         \
+    }
+    RQ_SHAPES(RQ_LAUNCH)
+#undef RQ_LAUNCH
+    return hipErrorInvalidValue;
 }
 
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
                                      const uint32_t *oftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr,
                                      u64 *ctr, u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq,
-                                     int lds_fac)
+                                     int chunk)
 {
     ExArgs a;
     a.n_rids = n_rids;
@@ -1622,8 +1639,12 @@ extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *q
     a.ctr = ctr;
     a.hits = hits; a.hit_cap = hit_cap; a.cap_divs = cap_divs; a.exq = (uint4 *)exq;
     a.neg_m1 = -M1;
-    (void)lds_fac;
-    hipLaunchKernelGGL(k_ext, dim3(grid), dim3(EX_NT), 0, st, a);
+    switch (chunk) {
+    case 1: hipLaunchKernelGGL(k_ext<1>, dim3(grid), dim3(EX_NT), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_ext<4>, dim3(grid), dim3(EX_NT), 0, st, a); break;
+    case 16: hipLaunchKernelGGL(k_ext<16>, dim3(grid), dim3(EX_NT), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -1646,14 +1667,27 @@ extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u
     return hipGetLastError();
 }
 
-extern "C" int ps4_rq_threads(void) { return RQ_NT; }
+extern "C" int ps4_rq_threads(void) { return RQ_NT_DEFAULT; }
+extern "C" int ps4_rq_tile(void) { return RQ_UN_DEFAULT; }
+extern "C" int ps4_ext_chunk(void) { return EX_CHUNK_DEFAULT; }
 extern "C" int ps4_ext_threads(void) { return EX_NT; }
 extern "C" int ps4_ext_queue_cap(void) { return 2 * EX_Q2; }   // u64 words per workgroup
 extern "C" int ps4_rq_window(void) { return (int)RQ_W; }
 extern "C" int ps4_rq_nb_max(void) { return RQ_NB_MAX; }
 
-extern "C" void *ps4_k_rq_ptr(void) { return (void *)&k_rq; }
-extern "C" void *ps4_k_ext_ptr(int lds_fac) { (void)lds_fac; return (void *)&k_ext; }
+// The kernel of a launch shape (for occupancy queries), or NULL.
+extern "C" void *ps4_k_rq_ptr(u32 block, u32 tile)
+{
+#define RQ_PTR(nt, un) \
+    if (block == nt && tile == un) return (void *)&k_rq<nt, un>;
+    RQ_SHAPES(RQ_PTR)
+#undef RQ_PTR
+    return nullptr;
+}
+extern "C" void *ps4_k_ext_ptr(int chunk)
+{
+    return chunk == 1 ? (void *)&k_ext<1> : chunk == 4 ? (void *)&k_ext<4> : chunk == 16 ? (void *)&k_ext<16> : nullptr;
+}
 
 // ----------------------------------------------------------------------------
 // k_triple: d with d+a, d+b, d+c square for given square triples {a,b,c}

@@ -26,13 +26,13 @@ typedef uint64_t u64;
 typedef uint32_t u32;
 typedef unsigned __int128 u128;
 
-extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, const ps4_item *items, u32 n_items, u32 bin_cap,
-                                    u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u32 n_rids, u64 *scratch, ps4_quad *quads,
-                                    u64 quad_cap, u32 cap_reps, u32 verify);
+extern "C" hipError_t ps4_launch_rq(hipStream_t st, u32 grid, u32 block, u32 tile, const ps4_item *items, u32 n_items,
+                                    u32 bin_cap, u32 nb_alloc, ps4_ctl *ctl, u64 *ctr, u32 n_rids, u64 *scratch,
+                                    ps4_quad *quads, u64 quad_cap, u32 cap_reps, u32 verify);
 extern "C" hipError_t ps4_launch_ext(hipStream_t st, u32 grid, const ps4_quad *quads, u64 quad_cap, ps4_ctl *ctl,
                                      const u32 *oftab, u64 tab_n, const uint16_t *ndiv, const ps4_qrtab *qr,
                                      u64 *ctr, u32 n_rids, ps4_hit *hits, u32 hit_cap, u32 cap_divs, u64 *exq,
-                                     int lds_fac);
+                                     int chunk);
 extern "C" hipError_t ps4_launch_reset(hipStream_t st, ps4_ctl *ctl, u64 *ctr, u32 n_ctr);
 extern "C" hipError_t ps4_launch_triple(hipStream_t st, const u64 *abc, u64 n, u64 d_max, const u64 *ftab,
                                         u64 tab_n, u64 *d_out, u32 cap, u32 *count, u32 *err);
@@ -43,12 +43,14 @@ extern "C" hipError_t ps4_launch_elem(hipStream_t st, const ps4_quad *quads, con
                                       u64 n_max, ps4_quad *out, u32 cap, u32 *count, u32 grid);
 extern "C" hipError_t ps4_launch_sqtri(hipStream_t st, u64 n_max, unsigned long long *total);
 extern "C" int ps4_rq_threads(void);
+extern "C" int ps4_rq_tile(void);
+extern "C" int ps4_ext_chunk(void);
 extern "C" int ps4_ext_threads(void);
 extern "C" int ps4_ext_queue_cap(void);
 extern "C" int ps4_rq_window(void);
 extern "C" int ps4_rq_nb_max(void);
-extern "C" void *ps4_k_rq_ptr(void);
-extern "C" void *ps4_k_ext_ptr(int lds_fac);
+extern "C" void *ps4_k_rq_ptr(u32 block, u32 tile);
+extern "C" void *ps4_k_ext_ptr(int chunk);
 
 namespace {
 
@@ -192,13 +194,16 @@ struct ps4_ctx {
     u32 cap_reps = 512, cap_divs = 16384, cap_ext = 64;
     u64 S_hi_max = 0, tab_n = 0;
     u32 n_cu = 0, rq_grid = 0, ext_grid = 0;
-    u32 ext_grid_lds = 0;      // workgroups of the extension kernel's LDS-factor form
     // tuning: the grids above may be lowered to these resident maxima (which
     // sized the scratch); superwindow shift, batch fill, k_ext form threshold
-    u32 rq_grid_max = 0, ext_grid_max = 0, ext_grid_lds_max = 0;
+    u32 rq_grid_max = 0, ext_grid_max = 0;
+    // launch shapes: k_rq threads per workgroup and points per thread load, k_ext
+    // tiles per work claim; rq_occ[i][j]: resident k_rq workgroups per CU of
+    // block 128 << i, tile 2 << j
+    u32 rq_block = 256, rq_tile = 4, ext_chunk = 4;
+    u32 rq_occ[3][3] = {{0}};
     u32 sw_shift = 0;
     double batch_frac = 0.8;
-    u64 lds_fac_above = 1ull << 35;
     u32 W = 0, nb_alloc = 0, bin_cap = 0;
     u32 scratch_wgs = 0;       // workgroups the window-bin scratch holds
     u64 quad_cap = 0;
@@ -344,6 +349,13 @@ u64 env_u64(const char *name, u64 dflt)
     return (s && *s) ? strtoull(s, nullptr, 10) : dflt;
 }
 
+// resident k_rq workgroups per CU of a launch shape (block 128/256/512, tile 2/4/8)
+u32 rq_occ_of(const ps4_ctx *ctx, u32 block, u32 tile)
+{
+    const u32 i = block == 128 ? 0 : block == 256 ? 1 : 2, j = tile == 2 ? 0 : tile == 4 ? 1 : 2;
+    return ctx->rq_occ[i][j];
+}
+
 int ensure_ctr(ps4_ctx *ctx, u32 n)
 {
     if (n <= ctx->ctr_cap) return 0;
@@ -464,10 +476,10 @@ int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, 
     // one chunk of 4 tiles of 64 queued 4-sets (its queue is filled in whole
     // tiles, so the upper bound on 4-sets plus one tile per item bounds the chunks).
     const u32 g_rq = std::max<u32>(1u, std::min<u32>(ctx->rq_grid, n_items));
-    const double chunks = (quads_expected + 64.0 * (double)n_items) / 256.0 + 1.0;
-    const u32 g_ext_full = (s_top > ctx->lds_fac_above) ? ctx->ext_grid_lds : ctx->ext_grid;
-    const u32 g_ext = (u32)std::max<double>(1.0, std::min<double>((double)g_ext_full, chunks));
-    HIP_OK(ps4_launch_rq(ctx->st, g_rq, ctx->d_items, n_items, ctx->bin_cap, ctx->nb_alloc, ctx->d_ctl,
+    const double chunks = (quads_expected + 64.0 * (double)n_items) / (64.0 * ctx->ext_chunk) + 1.0;
+    const u32 g_ext = (u32)std::max<double>(1.0, std::min<double>((double)ctx->ext_grid, chunks));
+    (void)s_top;
+    HIP_OK(ps4_launch_rq(ctx->st, g_rq, ctx->rq_block, ctx->rq_tile, ctx->d_items, n_items, ctx->bin_cap, ctx->nb_alloc, ctx->d_ctl,
                          ctx->d_ctr, n_rids, ctx->d_scratch, ctx->d_quads, ctx->quad_cap, ctx->cap_reps,
                          (ctx->flags & PS4_VERIFY) ? 1u : 0u));
     if (!graph) HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
@@ -477,13 +489,10 @@ int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, 
         HIP_OK(ps4_launch_elem(ctx->st, ctx->d_quads, ctx->d_ctl, ctx->quad_cap, ctx->elem_max, ctx->d_eout,
                                PS4_ELEM_CAP, ctx->d_ecnt, ctx->n_cu * 8));
     }
-    // the extension kernel's factor lists: in LDS above S ~ 3*10^10 (larger
-    // factor tables), in global scratch with two more waves per CU below
-    const bool lds_fac = s_top > ctx->lds_fac_above;
     if (!noext)
         HIP_OK(ps4_launch_ext(ctx->st, g_ext, ctx->d_quads, ctx->quad_cap,
                               ctx->d_ctl, ctx->d_oftab, ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, n_rids,
-                              ctx->d_hits, ctx->hit_cap, ctx->cap_divs, ctx->d_exq, lds_fac ? 1 : 0));
+                              ctx->d_hits, ctx->hit_cap, ctx->cap_divs, ctx->d_exq, (int)ctx->ext_chunk));
     if (!graph) HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
     if (!noext)
         if (int rc = dbg("k_ext")) return rc;
@@ -710,7 +719,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
         u64 item_max = ctx->W;
         // null entries an item may leave in the 4-set queue: each wave of the
         // representation kernel ends an item with up to two partly used tiles
-        const double item_pad = 128.0 * (double)(ps4_rq_threads() / 64);
+        const double item_pad = 128.0 * (double)(ctx->rq_block / 64);
         while (item_max * ctx->rq_grid < remaining && item_max < ((u64)1 << 40)) item_max <<= 1;
         // at least two superwindows per representation workgroup in a batch: at
         // high S a batch's 4-set budget covers fewer S, and superwindows of
@@ -894,16 +903,25 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     HIP_OK(ps4_noalloc_scratch(ctx->device, &ctx->na_scratch));
     for (hipEvent_t &e : ctx->ev) HIP_OK(hipEventCreate(&e));
     if (int rc = build_tables(ctx)) return rc;
-    int b_rq = 0, b_ext = 0, b_ext_lds = 0;
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b_rq, ps4_k_rq_ptr(), ps4_rq_threads(), 0));
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b_ext, ps4_k_ext_ptr(0), ps4_ext_threads(), 0));
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b_ext_lds, ps4_k_ext_ptr(1), ps4_ext_threads(), 0));
-    ctx->rq_grid = (u32)env_u64("PS4_RQ_GRID", (u64)ctx->n_cu * (u64)std::max(1, b_rq));
+    // resident workgroups per CU of every compiled launch shape; the window-bin
+    // scratch holds the largest resident grid of them all
+    u32 occ_max = 1;
+    for (u32 i = 0; i < 3; ++i)
+        for (u32 j = 0; j < 3; ++j) {
+            int b = 0;
+            HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ps4_k_rq_ptr(128u << i, 2u << j), (int)(128u << i), 0));
+            ctx->rq_occ[i][j] = (u32)std::max(1, b);
+            occ_max = std::max(occ_max, ctx->rq_occ[i][j]);
+        }
+    int b_ext = 0;
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b_ext, ps4_k_ext_ptr(ps4_ext_chunk()), ps4_ext_threads(), 0));
+    ctx->rq_block = (u32)ps4_rq_threads();
+    ctx->rq_tile = (u32)ps4_rq_tile();
+    ctx->ext_chunk = (u32)ps4_ext_chunk();
+    ctx->rq_grid_max = (u32)env_u64("PS4_RQ_GRID", (u64)ctx->n_cu * occ_max);
+    ctx->rq_grid = std::min(ctx->rq_grid_max, ctx->n_cu * rq_occ_of(ctx, ctx->rq_block, ctx->rq_tile));
     ctx->ext_grid = (u32)env_u64("PS4_EXT_GRID", (u64)ctx->n_cu * (u64)std::max(1, b_ext));
-    ctx->ext_grid_lds = std::min(ctx->ext_grid, ctx->n_cu * (u32)std::max(1, b_ext_lds));   // (the scratch holds ext_grid)
-    ctx->rq_grid_max = ctx->rq_grid;
     ctx->ext_grid_max = ctx->ext_grid;
-    ctx->ext_grid_lds_max = ctx->ext_grid_lds;
     ctx->W = (u32)ps4_rq_window();
     ctx->debug_sync = env_u64("PS4_DEBUG_SYNC", 0) != 0;
     // superwindows up to 2*sqrt(S_hi_max), at most 256 windows
@@ -958,21 +976,23 @@ extern "C" int ps4_get_tuning(ps4_ctx *ctx, ps4_tuning *cur, ps4_tuning *max)
         memset(cur, 0, sizeof *cur);
         cur->rq_grid = ctx->rq_grid;
         cur->ext_grid = ctx->ext_grid;
-        cur->ext_grid_lds = ctx->ext_grid_lds;
+        cur->rq_block = ctx->rq_block;
         cur->sw_shift = ctx->sw_shift;
         cur->batch_pct = (u32)std::lround(ctx->batch_frac * 100.0);
-        cur->lds_fac_above = ctx->lds_fac_above;
         cur->n_cu = ctx->n_cu;
+        cur->rq_tile = ctx->rq_tile;
+        cur->ext_chunk = ctx->ext_chunk;
     }
     if (max) {
         memset(max, 0, sizeof *max);
         max->rq_grid = ctx->rq_grid_max;
         max->ext_grid = ctx->ext_grid_max;
-        max->ext_grid_lds = ctx->ext_grid_lds_max;
+        max->rq_block = 512;
         max->sw_shift = PS4_SW_SHIFT_MAX;
         max->batch_pct = PS4_BATCH_PCT_MAX;
-        max->lds_fac_above = ~0ull;
         max->n_cu = ctx->n_cu;
+        max->rq_tile = 8;
+        max->ext_chunk = 16;
     }
     return 0;
 }
@@ -980,17 +1000,21 @@ extern "C" int ps4_get_tuning(ps4_ctx *ctx, ps4_tuning *cur, ps4_tuning *max)
 extern "C" int ps4_set_tuning(ps4_ctx *ctx, const ps4_tuning *t)
 {
     if (!ctx || !t) return PS4_E_ARG;
-    if (t->rq_grid > ctx->rq_grid_max || t->ext_grid > ctx->ext_grid_max || t->ext_grid_lds > ctx->ext_grid_lds_max ||
-        t->sw_shift > PS4_SW_SHIFT_MAX || t->batch_pct > PS4_BATCH_PCT_MAX) {
-        ctx->last_error = "tuning value above its maximum (ps4_get_tuning)";
+    const bool shape_ok = (t->rq_block == 0 || t->rq_block == 128 || t->rq_block == 256 || t->rq_block == 512) &&
+                          (t->rq_tile == 0 || t->rq_tile == 2 || t->rq_tile == 4 || t->rq_tile == 8) &&
+                          (t->ext_chunk == 0 || t->ext_chunk == 1 || t->ext_chunk == 4 || t->ext_chunk == 16);
+    if (t->rq_grid > ctx->rq_grid_max || t->ext_grid > ctx->ext_grid_max || t->sw_shift > PS4_SW_SHIFT_MAX ||
+        t->batch_pct > PS4_BATCH_PCT_MAX || !shape_ok) {
+        ctx->last_error = "tuning value above its maximum or not a compiled launch shape (ps4_get_tuning)";
         return PS4_E_ARG;
     }
-    ctx->rq_grid = t->rq_grid ? t->rq_grid : ctx->rq_grid_max;
+    ctx->rq_block = t->rq_block ? t->rq_block : (u32)ps4_rq_threads();
+    ctx->rq_tile = t->rq_tile ? t->rq_tile : (u32)ps4_rq_tile();
+    ctx->ext_chunk = t->ext_chunk ? t->ext_chunk : (u32)ps4_ext_chunk();
+    ctx->rq_grid = t->rq_grid ? t->rq_grid : std::min(ctx->rq_grid_max, ctx->n_cu * rq_occ_of(ctx, ctx->rq_block, ctx->rq_tile));
     ctx->ext_grid = t->ext_grid ? t->ext_grid : ctx->ext_grid_max;
-    ctx->ext_grid_lds = t->ext_grid_lds ? t->ext_grid_lds : ctx->ext_grid_lds_max;
     ctx->sw_shift = t->sw_shift;
     ctx->batch_frac = (t->batch_pct ? t->batch_pct : 80u) / 100.0;
-    ctx->lds_fac_above = t->lds_fac_above ? t->lds_fac_above : (1ull << 35);
     return 0;
 }
 
@@ -1014,15 +1038,16 @@ extern "C" int ps4_parse_tuning(const char *text, ps4_tuning *t)
         const size_t eq = line.find('=');
         if (eq == std::string::npos) return PS4_E_ARG;
         const std::string key = trim(line.substr(0, eq)), val = trim(line.substr(eq + 1));
-        if (val.empty() || val.find_first_not_of("0123456789") != std::string::npos || val.size() > 19) return PS4_E_ARG;
+        if (val.empty() || val.find_first_not_of("0123456789") != std::string::npos || val.size() > 10) return PS4_E_ARG;
         const u64 v = strtoull(val.c_str(), nullptr, 10);
-        if (key == "lds_fac_above") { t->lds_fac_above = v; continue; }
         if (v > 0xFFFFFFFFull) return PS4_E_ARG;
         if (key == "rq_grid") t->rq_grid = (u32)v;
         else if (key == "ext_grid") t->ext_grid = (u32)v;
-        else if (key == "ext_grid_lds") t->ext_grid_lds = (u32)v;
+        else if (key == "rq_block") t->rq_block = (u32)v;
         else if (key == "sw_shift") t->sw_shift = (u32)v;
         else if (key == "batch_pct") t->batch_pct = (u32)v;
+        else if (key == "rq_tile") t->rq_tile = (u32)v;
+        else if (key == "ext_chunk") t->ext_chunk = (u32)v;
         else return PS4_E_ARG;
     }
     return 0;

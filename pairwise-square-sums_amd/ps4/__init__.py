@@ -50,10 +50,10 @@ class Config(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     """Launch shape and batch plan (ps4_tuning): results never depend on it."""
-    _fields_ = [("rq_grid", ctypes.c_uint32), ("ext_grid", ctypes.c_uint32), ("ext_grid_lds", ctypes.c_uint32),
+    _fields_ = [("rq_grid", ctypes.c_uint32), ("ext_grid", ctypes.c_uint32), ("rq_block", ctypes.c_uint32),
                 ("sw_shift", ctypes.c_uint32), ("batch_pct", ctypes.c_uint32), ("n_cu", ctypes.c_uint32),
-                ("lds_fac_above", ctypes.c_uint64)]
-    KEYS = ("rq_grid", "ext_grid", "ext_grid_lds", "sw_shift", "batch_pct", "lds_fac_above")
+                ("rq_tile", ctypes.c_uint32), ("ext_chunk", ctypes.c_uint32)]
+    KEYS = ("rq_grid", "ext_grid", "rq_block", "sw_shift", "batch_pct", "rq_tile", "ext_chunk")
 
     def as_dict(self):
         d = {k: int(getattr(self, k)) for k in self.KEYS}

@@ -1,10 +1,13 @@
 """Autotuner of the census engine's launch shape and batch plan.
 
 It varies the fields of ps4_tuning (include/ps4.h):
-  - the resident workgroups of k_rq and of both forms of k_ext (per CU);
+  - the launch shape: k_rq's block size (threads per workgroup: 128, 256,
+    512) and per-thread tile (window points each thread loads at once: 2, 4,
+    8), and k_ext's work-claim size (1, 4 or 16 tiles of 64 4-sets) -- each a
+    compiled instantiation of the kernel;
+  - the resident workgroups of k_rq and k_ext (per CU);
   - the subrange split (superwindow shift);
-  - the batch fill of the 4-set queue;
-  - the S above which k_ext keeps its factor lists in LDS.
+  - the batch fill of the 4-set queue.
 
 The search is coordinate descent on a sample of the uneven-subrange workload:
 n subranges of [1, top] with log-uniform lengths (ps4.workloads.batched_ranges).
@@ -36,16 +39,24 @@ def _run(eng, los, his):
 def candidates(key, cur, mx):
     """Values of one field tried around the one in force."""
     n_cu = max(1, mx.get("n_cu", 1))
-    if key in ("rq_grid", "ext_grid", "ext_grid_lds"):
+    if key in ("rq_grid", "ext_grid"):
         per_cu = max(1, mx[key] // n_cu)
         return sorted({n_cu * b for b in range(max(1, per_cu - 3), per_cu + 1)} | {cur[key]})
+    if key == "rq_block":
+        return [128, 256, 512]
+    if key == "rq_tile":
+        return [2, 4, 8]
+    if key == "ext_chunk":
+        return [1, 4, 16]
     if key == "sw_shift":
         return list(range(0, min(3, mx[key]) + 1))
     if key == "batch_pct":
         return [40, 60, 80, 95]
-    if key == "lds_fac_above":
-        return [1 << 33, 1 << 34, 1 << 35, 1 << 36, 1 << 37, 1 << 50]
     raise KeyError(key)
+
+
+# the order of the descent: launch shape first (the grids' maxima depend on it)
+KEYS = ("rq_block", "rq_tile", "ext_chunk", "ext_grid", "rq_grid", "sw_shift", "batch_pct")
 
 
 def tune(eng, n=256, seed=42, top=10**11, e0=4, e1=9, reps=2, min_gain=0.01, rounds=1, log=print):
@@ -57,16 +68,18 @@ def tune(eng, n=256, seed=42, top=10**11, e0=4, e1=9, reps=2, min_gain=0.01, rou
     _run(eng, los[:8], his[:8])              # warm the device
     t_ref, ref = min((_run(eng, los, his) for _ in range(reps)), key=lambda x: x[0])
     log(f"autotune: {n} subranges, {sum(b - a for a, b in ranges):.3e} S; default {t_ref:.3f} s {cur}")
-    best = {k: cur[k] for k in ("rq_grid", "ext_grid", "ext_grid_lds", "sw_shift", "batch_pct", "lds_fac_above")}
+    best = {k: cur[k] for k in KEYS}
     t_best = t_ref
     trials = []
     for _ in range(rounds):
         changed = False
-        for key in ("ext_grid", "rq_grid", "ext_grid_lds", "sw_shift", "batch_pct", "lds_fac_above"):
+        for key in KEYS:
             for v in candidates(key, best, mx):
                 if v == best[key]:
                     continue
                 trial = dict(best, **{key: v})
+                if key == "rq_block" or key == "rq_tile":
+                    trial["rq_grid"] = 0          # the resident grid of the new shape
                 eng.set_tuning(trial)
                 t, res = min((_run(eng, los, his) for _ in range(reps)), key=lambda x: x[0])
                 if res != ref:

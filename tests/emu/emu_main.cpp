@@ -64,6 +64,10 @@ int main(int argc, char **argv)
     ps4_ctx *ctx = nullptr;
     int rc = ps4_open(&cfg, &ctx);
     if (rc) { printf("open failed %d\n", rc); return 2; }
+    if (const char *tune = getenv("PS4_EMU_TUNE")) {   // a tuning file's text, e.g. "rq_block = 512"
+        ps4_tuning t;
+        if (ps4_parse_tuning(tune, &t) || ps4_set_tuning(ctx, &t)) { printf("bad tuning\n"); return 2; }
+    }
     int bad = 0;
     if (elements) {
         std::string got, want;

@@ -250,7 +250,7 @@ static inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, co
 // device symbols are host objects here
 #define HIP_SYMBOL(x) (&(x))
 static inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
-enum { hipErrorInvalidDevice = 101 };
+enum { hipErrorInvalidDevice = 101, hipErrorInvalidValue = 1 };
 // graph capture is not emulated: asking for it fails like an unsupported call
 typedef void *hipGraph_t;
 typedef void *hipGraphExec_t;

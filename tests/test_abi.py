@@ -50,25 +50,25 @@ def test_struct_layouts_match_header():
 
 
 def test_tuning_struct_layout():
-    # ps4_tuning: five u32, the CU count, then the u64 threshold
+    # ps4_tuning: eight u32 in the declared order
     assert ctypes.sizeof(ps4.Tuning) == 32
-    assert ps4.Tuning.n_cu.offset == 20 and ps4.Tuning.lds_fac_above.offset == 24
+    assert ps4.Tuning.rq_block.offset == 8 and ps4.Tuning.n_cu.offset == 20 and ps4.Tuning.ext_chunk.offset == 28
 
 
 def test_tuning_file_round_trip_and_errors():
     """The library's parser (ps4_parse_tuning) reads what format_tuning writes,
     ignores comments and blank lines, and rejects unknown keys and bad values."""
-    d = {"rq_grid": 1024, "ext_grid": 2816, "ext_grid_lds": 2304, "sw_shift": 1, "batch_pct": 60,
-         "lds_fac_above": 1 << 36}
+    d = {"rq_grid": 1024, "ext_grid": 2816, "rq_block": 512, "sw_shift": 1, "batch_pct": 60, "rq_tile": 8,
+         "ext_chunk": 16}
     assert ps4.parse_tuning(ps4.format_tuning(d, "autotune\nsecond line")) == d
     got = ps4.parse_tuning("\n  # comment only\n ext_grid=12   # trailing\r\n\n")
     assert got == dict.fromkeys(ps4.Tuning.KEYS, 0) | {"ext_grid": 12}
     assert ps4.parse_tuning("") == dict.fromkeys(ps4.Tuning.KEYS, 0)
     for bad in ("foo = 1", "rq_grid = x", "rq_grid 5", "rq_grid =", "rq_grid = -1", "rq_grid = 4294967296",
-                "lds_fac_above = 99999999999999999999", "= 3"):
+                "rq_tile = 99999999999999999999", "lds_fac_above = 1", "= 3"):
         with pytest.raises(ps4.PS4Error):
             ps4.parse_tuning(bad)
-    assert ps4.parse_tuning("lds_fac_above = 1125899906842624")["lds_fac_above"] == 1 << 50
+    assert ps4.parse_tuning("ext_chunk = 1")["ext_chunk"] == 1
 
 
 @pytest.mark.parametrize("name,m,shift,lim", [("mod11", 11, 19, 46189), ("mod13", 13, 19, 46189),

@@ -401,9 +401,10 @@ def test_search_batch_uneven_subranges(eng):
 
 def test_tuning_never_changes_results(eng):
     """Launch shape and batch plan are timing only: extreme tunings (one
-    workgroup per CU, 8x finer subrange split, tiny batches, k_ext's LDS form
-    from S = 2^20 or never) give the default results on a census and on
-    uneven subranges; values above the maxima are rejected."""
+    workgroup per CU, every compiled k_rq block size and tile, every k_ext
+    claim size, 8x finer subrange split, tiny batches) give the default
+    results on a census and on uneven subranges; values above the maxima and
+    shapes that are not compiled are rejected."""
     from ps4 import workloads
     base, mx = eng.get_tuning()
     n_cu = mx["n_cu"]
@@ -414,10 +415,11 @@ def test_tuning_never_changes_results(eng):
     ref_c = eng.search(2 * 10**9, 2 * 10**9 + 2 * 10**7)
     ref_b = eng.search_batch(los, his)
     tunings = [
-        {"rq_grid": n_cu, "ext_grid": n_cu, "ext_grid_lds": n_cu, "sw_shift": 3, "batch_pct": 5,
-         "lds_fac_above": 1 << 20},
-        {"rq_grid": mx["rq_grid"] - n_cu, "ext_grid": mx["ext_grid"] - n_cu, "sw_shift": 1, "batch_pct": 95,
-         "lds_fac_above": 1 << 50},
+        {"rq_grid": n_cu, "ext_grid": n_cu, "rq_block": 128, "rq_tile": 2, "ext_chunk": 1, "sw_shift": 3,
+         "batch_pct": 5},
+        {"rq_grid": mx["rq_grid"] - n_cu, "ext_grid": mx["ext_grid"] - n_cu, "rq_block": 512, "rq_tile": 8,
+         "ext_chunk": 16, "sw_shift": 1, "batch_pct": 95},
+        {"rq_block": 256, "rq_tile": 8, "ext_chunk": 1},
     ]
     try:
         for t in tunings:
@@ -431,7 +433,7 @@ def test_tuning_never_changes_results(eng):
                 assert c == c0, (r, t)
                 assert sorted(lines) == sorted(l0), (r, t)
         for over in ({"rq_grid": mx["rq_grid"] + 1}, {"ext_grid": mx["ext_grid"] + 1}, {"sw_shift": 9},
-                     {"batch_pct": 96}):
+                     {"batch_pct": 96}, {"rq_block": 64}, {"rq_tile": 3}, {"ext_chunk": 2}):
             with pytest.raises(ps4.PS4Error):
                 eng.set_tuning(over)
     finally:
