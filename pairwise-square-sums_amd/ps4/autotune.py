@@ -81,6 +81,7 @@ def tune(eng, n=256, seed=42, top=10**11, e0=4, e1=9, reps=2, min_gain=0.01, rou
                 if key == "rq_block" or key == "rq_tile":
                     trial["rq_grid"] = 0          # the resident grid of the new shape
                 eng.set_tuning(trial)
+                trial = {k: eng.get_tuning()[0][k] for k in KEYS}   # (0 fields resolved)
                 t, res = min((_run(eng, los, his) for _ in range(reps)), key=lambda x: x[0])
                 if res != ref:
                     bad = next(i for i in range(n) if res[i] != ref[i])
