@@ -66,7 +66,7 @@ PEAK_LANE_OPS = 256 * 64 * 2.4e9      # 256 CUs x 64 lanes x 2.4 GHz, 32-bit int
 # HBM bytes per k_ext launch, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
 # [1,1e10) census (4 launches), each counter scaled by what it reads for a 2 GiB device copy
 # (profiles/census_1e10_counters.md); reported for that workload only
-K_EXT_TRAFFIC = {(1, 10**10): 36.18e9}
+K_EXT_TRAFFIC = {(1, 10**10): 24.63e9}
 GOLDEN = os.path.join(REPO, "tests", "golden", "golden.json")
 TUNE = None   # tuning file (--tune): launch shape and batch plan of every engine opened here
 GRAPH = False  # --graph: every batch launched as one captured hipGraph
@@ -108,7 +108,7 @@ def cpu_baseline(lo, hi, target_s=30.0, runs=3, nsub=3):
     """The reference s4 (built from exhaustive/s4.c into oracle/_ref/) on every
     CPU this process may use (nproc, s4's own default), over nsub subranges
     spread evenly over [lo, hi), each a separate s4 process timed from outside:
-    one calibration run, then `runs` runs sized to about target_s seconds each,
+    two calibration runs, then `runs` runs sized to target_s seconds or more each,
     reported as their median S/s.  The -march=x86-64-v3 build is used where the
     host has AVX2, BMI2 and FMA."""
     model, nproc, ncpu, flags = host_cpu()
@@ -138,10 +138,17 @@ def cpu_baseline(lo, hi, target_s=30.0, runs=3, nsub=3):
                 # is rounded up by its one-second monitor poll (:473)
                 busy += max(float(x) for x in re.findall(r"thread\s+\d+:.*?([\d.]+)s\s*$", err, re.M) or ["0"])
         return done, wall, busy
+    # s4's monitor polls once a second (exhaustive/s4.c:473), so a short run's
+    # wall time is rounded up: calibrate on a run of about 5 s, then size the
+    # timed runs from it, with 20 % to spare so that each lasts target_s or more
     cal = run(min(10**8, span // nsub))
     if cal is None:
         return None
-    width = int(min(span // nsub, max(10**7, cal[0] / cal[1] * target_s / nsub)))
+    width = int(min(span // nsub, max(10**7, cal[0] / cal[1] * 5.0 / nsub)))
+    cal = run(width)
+    if cal is None:
+        return None
+    width = int(min(span // nsub, max(10**7, cal[0] / cal[1] * target_s * 1.2 / nsub)))
     log(f"cpu baseline: s4 ({march}) on {threads} threads, {runs} runs of {nsub} x {width} S")
     res = [run(width) for _ in range(runs)]
     if any(x is None for x in res):
