@@ -456,8 +456,12 @@ __device__ u32 rq_walk(u32 c0, u32 c1, u32 nit, const u32 *s_sk, const u32 *s_st
     return n;
 }
 
+// The 256-thread shape is held to 5 waves per SIMD (<= 96 VGPRs, no spills), so
+// that 5 workgroups (20 waves) share a CU: its LDS allows 5, and with 98 VGPRs
+// only 4 fit (k_rq -6.5 % at S <= 10^10, -9.5 % at 9*10^11).  The other shapes
+// are held by their LDS (128 threads) or workgroup size (512) anyway.
 template <int NT, int UN>
-__global__ __launch_bounds__(NT) void k_rq(RqArgs a)
+__global__ __launch_bounds__(NT, NT == 256 ? 5 : 1) void k_rq(RqArgs a)
 {
     __shared__ u32 s_cnt[RQ_W / 2];        // u16 bucket counts, two per word; later offsets
     __shared__ u32 s_pv[RQ_PV_CAP];        // p of every representation of a counted S
