@@ -722,9 +722,10 @@ __global__ __launch_bounds__(NT, NT == 256 ? 5 : 1) void k_rq(RqArgs a)
 // ----------------------------------------------------------------------------
 
 #define EX_NT 64          // one wave per workgroup: every barrier below is a wave's own
-#define EX_ARENA 864      // doubles of divisor lists per pass (the workgroup's LDS stays at
-                          // 16 KB); one 4-set with d(D) <= 16384 needs at most 748 (nL + nH
-                          // over the best prefix split, reached by the radices 2, 11, 33, 2, 11)
+#define EX_ARENA 861      // doubles of divisor lists per pass, + 3 spare (the workgroup's LDS
+                          // stays at 13568 bytes, so 12 workgroups share a CU's 160 KB); one
+                          // 4-set with d(D) <= 16384 needs at most 748 (nL + nH over the best
+                          // prefix split, reached by the radices 2, 11, 33, 2, 11)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
 #define EX_CHUNK_DEFAULT 4   // tiles of 64 4-sets claimed per work-queue atomic (a template
                              // parameter of k_ext: 1, 4 or 16)
@@ -954,10 +955,11 @@ __device__ __forceinline__ void ex_stages(const ExOdd &f1, const ExOdd &f2, u32 
     for (int i = 0; i < EX_OSLOT; ++i) st[1 + i] = w1[i];
     u32 d = (f1.b.w >> 4) * (f2.b.w >> 4);
     const double o = (double)o1;
+    const u32 n2 = f2.b.w & 15u;   // slots of o2 in use (the others hold 0)
 #pragma unroll
     for (int j = 0; j < EX_OSLOT; ++j) {
         u32 w = w2[j];
-        if (ex_divides(w >> 6, o)) {   // a prime of both parts (rare)
+        if ((u32)j < n2 && ex_divides(w >> 6, o)) {   // a prime of both parts (rare)
             const u32 p = w >> 6, eb = w & 63u;
 #pragma unroll
             for (int i = 0; i < EX_OSLOT; ++i) {
@@ -1125,7 +1127,7 @@ __device__ __forceinline__ void ex_flush(u64 *ctr, u32 n_rids, u32 rid, u64 cand
 template <int EX_CHUNK>
 __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIMD: <= 168 VGPRs
 {
-    __shared__ double s_ar[EX_ARENA];          // L and H lists, exact doubles
+    __shared__ double s_ar[EX_ARENA + 3];      // L and H lists, exact doubles (+ 3 spare: see the list building)
     __shared__ uint4 s_par[EX_NT * EX_PAR_WORDS];   // parameters of the 4-sets of the current pass
     __shared__ uint4 s_mem[EX_NT];             // their e > 0 and member bounds
     __shared__ u32 s_aux[EX_NT];               // isqrt(2A) or 0
@@ -1289,16 +1291,16 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     const double d1 = (double)(i64)(P.C1 - P.A), d2 = (double)(i64)(P.C2 - P.A);
                     // representatives mod 262080 (residues mod 64, 63, 65 follow) and
                     // exact residues mod 46189; |C - A| < 2^51
-                    const u32 r1a = rep_m1(d1), r1b = rep_m1(d2);
-                    const u32 r2a = mod_m2(d1), r2b = mod_m2(d2);
-                    // residues index small tables: each index is checked against
-                    // its table (a wrong residue is an error, never a load outside it)
-                    u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(r1a), mod63(r1b), mod65(r1a), mod65(r1b), mod11(r2a),
-                                  mod11(r2b), mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
-                    const u32 lim[7] = {64, 63, 65, 11, 13, 17, 19};
-#pragma unroll
-                    for (int q = 0; q < 14; ++q)
-                        if (ix[q] >= lim[q >> 1]) { err |= ERR_BOUNDS; ix[q] = 0; }
+                    // The residues index small tables.  The representatives are
+                    // clamped to the domains on which every residue below is exact
+                    // and in range (all s < 2^20 for mod 63 and 65, s < 46189 for
+                    // mod 11..19; checked exhaustively on the host), so no index
+                    // can leave its table, whatever the input; for |C - A| < 2^51
+                    // the clamps change nothing.
+                    const u32 r1a = rep_m1(d1) & 0xFFFFFu, r1b = rep_m1(d2) & 0xFFFFFu;
+                    const u32 r2a = min(mod_m2(d1), 46188u), r2b = min(mod_m2(d2), 46188u);
+                    const u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(r1a), mod63(r1b), mod65(r1a), mod65(r1b), mod11(r2a),
+                                        mod11(r2b), mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
                     // (read here, combined after the walk)
                     const u64 t64a = qr->t64[ix[0]], t64b = qr->t64[ix[1]], t63a = qr->t63[ix[2]],
                               t63b = qr->t63[ix[3]], t65a = qr->t65[ix[4]][0], t65b = qr->t65[ix[5]][0],
@@ -1477,9 +1479,20 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                                 o[0] = y0; o[1] = y1; o[2] = y2; o[3] = y3;
                             }
                         }
-                        for (; j < cur; ++j) {
-                            double y = lst[j];
-                            for (u32 ee = 1; ee <= e; ++ee) { y *= dp; lst[ee * cur + j] = y; }
+                        if (j < cur) {   // 1 to 3 entries left: one group, its loads issued together
+                            // (the two reads past cur stay inside the arena's 3 spare
+                            // doubles; the products of entries past cur go to the
+                            // spare slot s_ar[EX_ARENA + 2], never to a list)
+                            const bool w1 = j + 1u < cur, w2 = j + 2u < cur;
+                            double y0 = lst[j], y1 = lst[j + 1], y2 = lst[j + 2];
+                            double *spare = s_ar + EX_ARENA + 2;
+                            for (u32 ee = 1; ee <= e; ++ee) {
+                                y0 *= dp; y1 *= dp; y2 *= dp;
+                                double *o = lst + ee * cur + j;
+                                o[0] = y0;
+                                *(w1 ? o + 1 : spare) = y1;
+                                *(w2 ? o + 2 : spare) = y2;
+                            }
                         }
                         n = cur * (e + 1);
                     }

@@ -84,6 +84,10 @@ def test_kernel_residue_constants_are_exact(name, m, shift, lim):
     for s in range(lim):
         assert s * magic < 2**32
         assert (s * magic) >> sh == s // m
+    # the kernel clamps its representatives to the domains checked here, so no
+    # residue can index past its table whatever the input
+    assert "min(mod_m2(d1), 46188u)" in src and "min(mod_m2(d2), 46188u)" in src
+    assert "rep_m1(d1) & 0xFFFFFu" in src and "rep_m1(d2) & 0xFFFFFu" in src
 
 
 @pytest.mark.parametrize("name,m", [("mod63", 63), ("mod65", 65)])
