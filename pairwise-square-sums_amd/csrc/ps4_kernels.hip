@@ -1238,6 +1238,21 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
         }
         if (valid) {
             u32 o1 = 1, o2 = 1;
+                // The divisor counts of the pair choice below, gathered first so
+                // that their latency overlaps the checks; each index is clamped
+                // into the table (a no-op for a well-formed 4-set, whose indices
+                // are below rab + rac < tab_n), so no entry can read outside it.
+                // (the roots and the table indices are 32-bit: tab_n < 2^27)
+                const u32 ra_[6] = {qd.r_ac, qd.r_ac, qd.r_ab, qd.r_ab, qd.r_ab, qd.r_ab};
+                const u32 rb_[6] = {qd.r_bc, qd.r_ad, qd.r_bc, qd.r_ad, qd.r_bd, qd.r_ac};
+                const u32 nlast = (u32)a.tab_n - 1u;
+                u32 nd_[12];
+#pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    // (32-bit byte offsets: the loads take the table base from scalar registers)
+                    nd_[2 * t] = *(const uint16_t *)((const char *)a.ndiv + 2u * min(ra_[t] - rb_[t], nlast));
+                    nd_[2 * t + 1] = *(const uint16_t *)((const char *)a.ndiv + 2u * min(ra_[t] + rb_[t], nlast));
+                }
                 const u64 rab = qd.r_ab, rac = qd.r_ac, rad = qd.r_ad, rbc = qd.r_bc, rbd = qd.r_bd, rcd = qd.r_cd;
                 const u64 va = (rab * rab + rac * rac - rbc * rbc) >> 1;
                 const u64 vb = rab * rab - va, vc = rac * rac - va, vd = rad * rad - va;
@@ -1257,16 +1272,6 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // D = v_i - v_j, first minimum in the scan order of
                     // exhaustive/s4.c:163-171 (members are a > b > c > d, so the
                     // reference's swap never fires).
-                    // (the roots and the table indices are 32-bit: rab + rac < tab_n < 2^27)
-                    const u32 ra_[6] = {qd.r_ac, qd.r_ac, qd.r_ab, qd.r_ab, qd.r_ab, qd.r_ab};
-                    const u32 rb_[6] = {qd.r_bc, qd.r_ad, qd.r_bc, qd.r_ad, qd.r_bd, qd.r_ac};
-                    u32 nd_[12];
-#pragma unroll
-                    for (int t = 0; t < 6; ++t) {
-                        // (32-bit byte offsets: the loads take the table base from scalar registers)
-                        nd_[2 * t] = *(const uint16_t *)((const char *)a.ndiv + 2u * (ra_[t] - rb_[t]));
-                        nd_[2 * t + 1] = *(const uint16_t *)((const char *)a.ndiv + 2u * (ra_[t] + rb_[t]));
-                    }
                     u32 bestv = 0xFFFFFFFFu;
 #pragma unroll
                     for (int t = 0; t < 6; ++t) {
