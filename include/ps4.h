@@ -182,6 +182,12 @@ int ps4_search_elements(ps4_ctx *ctx, uint64_t n_max, ps4_sink sink, void *user,
  * exhaustive/s4.c:496). */
 int ps4_kernel_time(ps4_ctx *ctx, double *ms_rq, double *ms_ext, uint64_t *batches, int reset);
 
+/* Batches run again since ps4_open because their 4-sets or extensions
+ * overflowed the device queues (each is redone smaller: results are never
+ * truncated, as s4's FATAL checks never let a count pass silently,
+ * exhaustive/s4.c:45-54).  A diagnostic; no reference counterpart. */
+uint64_t ps4_redo_count(ps4_ctx *ctx);
+
 /* Launch shape and batch plan of later searches: what the autotuner
  * (pairwise-square-sums_amd/ps4/autotune.py) varies.  Results never depend on
  * these, only the time does.  No reference counterpart: the shape knobs of s4

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -218,12 +219,16 @@ struct ps4_ctx {
     ps4_item *d_items = nullptr;
     u64 *d_scratch = nullptr;
     ps4_quad *d_quads = nullptr;
-    ps4_hit *d_hits = nullptr;
     u64 *d_exq = nullptr;      // per-workgroup exact-test queues of the extension kernel
-    // pinned host
-    ps4_ctl *h_ctl = nullptr;
-    u64 *h_ctr = nullptr;
-    ps4_item *h_items = nullptr;
+    // Two of each, one per batch parity: the next batch is planned and
+    // enqueued while the current one runs, so its host items, read-backs,
+    // extensions and events must not share a buffer with the current one's.
+    // ev[p]: before k_rq, after k_rq, after k_ext, after the read-backs.
+    ps4_hit *d_hits[2] = {nullptr, nullptr};
+    ps4_ctl *h_ctl[2] = {nullptr, nullptr};        // pinned host
+    u64 *h_ctr[2] = {nullptr, nullptr};
+    ps4_item *h_items[2] = {nullptr, nullptr};
+    hipEvent_t ev[2][4] = {};
     ps4_hit *h_hits = nullptr;
     // state
     ps4_alert alert = nullptr;
@@ -233,7 +238,8 @@ struct ps4_ctx {
     // kernel time accumulated over the launches since the last reset (ms)
     double ms_rq = 0.0, ms_ext = 0.0;
     u64 n_batches = 0;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    u32 density_pct = 100;     // PS4_TEST_DENSITY_PCT: scales the planner's 4-set estimate (tests of the redo path)
+    u64 n_redo = 0;            // batches redone after an overflow (since open)
     hipGraphExec_t gexec = nullptr;   // PS4_GRAPH: the batch sequence, updated per batch
     // triple queries
     u64 *d_tabc = nullptr, *d_tout = nullptr;
@@ -361,9 +367,10 @@ int ensure_ctr(ps4_ctx *ctx, u32 n)
     if (n <= ctx->ctr_cap) return 0;
     u32 cap = std::max<u32>(n, 64);
     if (ctx->d_ctr) { (void)hipFree(ctx->d_ctr); ctx->d_ctr = nullptr; }
-    if (ctx->h_ctr) { (void)hipHostFree(ctx->h_ctr); ctx->h_ctr = nullptr; }
+    for (u64 *&h : ctx->h_ctr)
+        if (h) { (void)hipHostFree(h); h = nullptr; }
     HIP_OK(hipMalloc(&ctx->d_ctr, (size_t)cap * CT_N * sizeof(u64)));
-    HIP_OK(hipHostMalloc(&ctx->h_ctr, (size_t)cap * CT_N * sizeof(u64)));
+    for (u64 *&h : ctx->h_ctr) HIP_OK(hipHostMalloc(&h, (size_t)cap * CT_N * sizeof(u64)));
     ctx->ctr_cap = cap;
     return 0;
 }
@@ -449,11 +456,11 @@ struct Pending {
 // counter reset, k_rq, (k_elem), k_ext and the read-backs of the control block
 // and counter rows into pinned memory.  Kernel time is taken by events around
 // the two census kernels.  Returns once everything is enqueued.
-int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, bool graph, double quads_expected)
+int enqueue_batch(ps4_ctx *ctx, int bp, u32 n_items, u32 n_rids, u64 s_top, bool noext, bool graph, double quads_expected)
 {
     const u32 n_ctr = n_rids * CT_N;
     if (ctx->rq_grid > ctx->scratch_wgs) { ctx->last_error = "k_rq grid above its scratch"; return PS4_E_INTERNAL; }
-    HIP_OK(hipMemcpyAsync(ctx->d_items, ctx->h_items, n_items * sizeof(ps4_item), hipMemcpyHostToDevice, ctx->st));
+    HIP_OK(hipMemcpyAsync(ctx->d_items, ctx->h_items[bp], n_items * sizeof(ps4_item), hipMemcpyHostToDevice, ctx->st));
     // PS4_DEBUG_SYNC=1: wait after every launch and name the kernel that failed
     auto dbg = [&](const char *what) -> int {
         if (!ctx->debug_sync) return 0;
@@ -468,7 +475,7 @@ int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, 
     };
     HIP_OK(ps4_launch_reset(ctx->st, ctx->d_ctl, ctx->d_ctr, n_ctr));
     if (int rc = dbg("k_reset")) return rc;
-    if (!graph) HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
+    if (!graph) HIP_OK(hipEventRecord(ctx->ev[bp][0], ctx->st));
     // A small batch launches only the workgroups it can keep busy: every
     // workgroup of the persistent kernels claims work by an atomic on one
     // counter, so idle ones only add to the contention (about 0.2 ms each for
@@ -482,7 +489,7 @@ int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, 
     HIP_OK(ps4_launch_rq(ctx->st, g_rq, ctx->rq_block, ctx->rq_tile, ctx->d_items, n_items, ctx->bin_cap, ctx->nb_alloc, ctx->d_ctl,
                          ctx->d_ctr, n_rids, ctx->d_scratch, ctx->d_quads, ctx->quad_cap, ctx->cap_reps,
                          (ctx->flags & PS4_VERIFY) ? 1u : 0u));
-    if (!graph) HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
+    if (!graph) HIP_OK(hipEventRecord(ctx->ev[bp][1], ctx->st));
     if (int rc = dbg("k_rq")) return rc;
     if (ctx->elem_max) {
         HIP_OK(ps4_launch_fill_u32(ctx->st, ctx->d_ecnt, 1, 0));
@@ -492,23 +499,27 @@ int enqueue_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, 
     if (!noext)
         HIP_OK(ps4_launch_ext(ctx->st, g_ext, ctx->d_quads, ctx->quad_cap,
                               ctx->d_ctl, ctx->d_oftab, ctx->tab_n, ctx->d_ndiv, ctx->d_qr, ctx->d_ctr, n_rids,
-                              ctx->d_hits, ctx->hit_cap, ctx->cap_divs, ctx->d_exq, (int)ctx->ext_chunk));
-    if (!graph) HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
+                              ctx->d_hits[bp], ctx->hit_cap, ctx->cap_divs, ctx->d_exq, (int)ctx->ext_chunk));
+    if (!graph) HIP_OK(hipEventRecord(ctx->ev[bp][2], ctx->st));
     if (!noext)
         if (int rc = dbg("k_ext")) return rc;
-    HIP_OK(hipMemcpyAsync(ctx->h_ctl, ctx->d_ctl, sizeof(ps4_ctl), hipMemcpyDeviceToHost, ctx->st));
-    HIP_OK(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, (size_t)n_ctr * sizeof(u64), hipMemcpyDeviceToHost, ctx->st));
+    HIP_OK(hipMemcpyAsync(ctx->h_ctl[bp], ctx->d_ctl, sizeof(ps4_ctl), hipMemcpyDeviceToHost, ctx->st));
+    HIP_OK(hipMemcpyAsync(ctx->h_ctr[bp], ctx->d_ctr, (size_t)n_ctr * sizeof(u64), hipMemcpyDeviceToHost, ctx->st));
     return 0;
 }
 
 // One batch launched: directly, or (graph) captured and launched as one
 // hipGraph.  A failure inside the capture ends the capture and drops the
 // partial graph, so the stream is usable again.
-int launch_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, bool graph, double quads_expected)
+int launch_batch(ps4_ctx *ctx, int bp, u32 n_items, u32 n_rids, u64 s_top, bool noext, bool graph, double quads_expected)
 {
-    if (!graph) return enqueue_batch(ctx, n_items, n_rids, s_top, noext, false, quads_expected);
+    if (!graph) {
+        if (int rc = enqueue_batch(ctx, bp, n_items, n_rids, s_top, noext, false, quads_expected)) return rc;
+        HIP_OK(hipEventRecord(ctx->ev[bp][3], ctx->st));
+        return 0;
+    }
     HIP_OK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_batch(ctx, n_items, n_rids, s_top, noext, true, quads_expected);
+    const int rc = enqueue_batch(ctx, bp, n_items, n_rids, s_top, noext, true, quads_expected);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(ctx->st, &g);
     if (rc || ec != hipSuccess) {
@@ -533,10 +544,11 @@ int launch_batch(ps4_ctx *ctx, u32 n_items, u32 n_rids, u64 s_top, bool noext, b
     if (!ctx->gexec) e = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HIP_OK(e);
-    HIP_OK(hipEventRecord(ctx->ev[0], ctx->st));
+    HIP_OK(hipEventRecord(ctx->ev[bp][0], ctx->st));
     HIP_OK(hipGraphLaunch(ctx->gexec, ctx->st));
-    HIP_OK(hipEventRecord(ctx->ev[1], ctx->st));
-    HIP_OK(hipEventRecord(ctx->ev[2], ctx->st));
+    HIP_OK(hipEventRecord(ctx->ev[bp][1], ctx->st));
+    HIP_OK(hipEventRecord(ctx->ev[bp][2], ctx->st));
+    HIP_OK(hipEventRecord(ctx->ev[bp][3], ctx->st));
     return 0;
 }
 
@@ -679,8 +691,28 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
     if (int rc = ensure_ctr(ctx, (u32)n)) return rc;
     std::vector<int64_t> dump_left(n, ctx->dump4_limit);
     const bool noext = (ctx->flags & PS4_NOEXT) != 0;
+    // PS4_GRAPH: each batch's sequence is captured and launched as one graph
+    const bool graph = (ctx->flags & PS4_GRAPH) && !ctx->debug_sync;
+    bool want_dump_any = false;
+    for (size_t r = 0; r < n; ++r) want_dump_any |= dump_left[r] > 0;
+    // Two batches in flight: the next one is planned and enqueued before the
+    // host waits for the current one, so the device does not idle between
+    // them.  One at a time where a batch's results are read from a buffer the
+    // next batch would overwrite (the 4-set dump, the element mode), and for
+    // graph launches and the debugging mode.
+    const size_t depth = (want_dump_any || ctx->elem_max || graph || ctx->debug_sync) ? 1 : 2;
     double budget_scale = 1.0;
     size_t next_iv = 0;
+    int parity = 0;
+    // a batch in flight: its intervals, buffer parity, and the interval state
+    // before it was planned (restored if it has to be redone)
+    struct Flight {
+        u32 rid_lo = 0, rid_hi = 0;
+        int bp = 0;
+        size_t next_iv = 0;
+        std::vector<Interval> save;
+    };
+    std::deque<Flight> fl;
     // the lines of a finished batch are made on the host while the next batch
     // runs on the device (batches emit in order)
     Pending pending;
@@ -701,85 +733,126 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
     // an error after a launch: let the device finish before returning
     auto fail = [&](int rc) { (void)hipStreamSynchronize(ctx->st); return rc; };
     for (;;) {
-        while (next_iv < n && iv[next_iv].cur >= iv[next_iv].hi) ++next_iv;
-        if (next_iv >= n) break;
-        // ---- assemble one batch of superwindows
-        std::vector<Interval> save(iv.begin() + next_iv, iv.end());
-        double quads_expected = 0.0;
-        // element mode: a batch's 4-sets must also fit the buffer of qualifying ones
-        const double quads_budget =
-            ctx->batch_frac * (double)(ctx->elem_max ? std::min<u64>(ctx->quad_cap, PS4_ELEM_CAP) : ctx->quad_cap) * budget_scale;
-        u32 n_items = 0;
-        u64 s_top = 0;   // largest S of the batch
-        u32 rid_lo = iv[next_iv].rid, rid_hi = rid_lo;
-        // little work left (small queries): shorter superwindows, so that the
-        // representation kernel still has about one per workgroup
-        u64 remaining = 0;
-        for (size_t i = next_iv; i < n; ++i) remaining += iv[i].hi - iv[i].cur;
-        u64 item_max = ctx->W;
-        // null entries an item may leave in the 4-set queue: each wave of the
-        // representation kernel ends an item with up to two partly used tiles
-        const double item_pad = 128.0 * (double)(ctx->rq_block / 64);
-        while (item_max * ctx->rq_grid < remaining && item_max < ((u64)1 << 40)) item_max <<= 1;
-        // at least two superwindows per representation workgroup in a batch: at
-        // high S a batch's 4-set budget covers fewer S, and superwindows of
-        // ~2 sqrt(S) would leave workgroups idle (at S ~ 9*10^11: 820 per batch
-        // for 1024 workgroups)
-        auto sw_at = [&](u64 S) {
-            u64 sw = superwindow(ctx, S);
-            const double span = quads_budget / quad_density(S);
-            while (sw > ctx->W && (double)sw * 2.0 * (double)ctx->rq_grid > span) sw >>= 1;
-            return sw;
-        };
-        for (size_t i = next_iv; i < n && n_items < ctx->items_cap && quads_expected < quads_budget; ++i) {
-            Interval &v = iv[i];
-            while (v.cur < v.hi && n_items < ctx->items_cap && quads_expected < quads_budget) {
-                u64 len = std::min(sw_at(v.cur), item_max);
-                if (len > v.hi - v.cur) len = v.hi - v.cur;
-                const double dq = quad_density(v.cur + len) * (double)len + item_pad;
-                if (n_items > 0 && quads_expected + dq > quads_budget) {
-                    // shrink the last item to the remaining budget, in whole windows
-                    const double left = quads_budget - quads_expected - item_pad;
-                    u64 room = left > 0.0 ? (u64)(left / quad_density(v.cur + len)) : 0;
-                    room = room / ctx->W * ctx->W;
-                    if (room == 0) { quads_expected = quads_budget; break; }
-                    len = room;
+        // ---- plan and enqueue batches of superwindows until depth are in flight
+        while (fl.size() < depth) {
+            while (next_iv < n && iv[next_iv].cur >= iv[next_iv].hi) ++next_iv;
+            if (next_iv >= n) break;
+            Flight f;
+            f.bp = parity;
+            parity ^= 1;
+            f.next_iv = next_iv;
+            f.save.assign(iv.begin() + next_iv, iv.end());
+            ps4_item *items = ctx->h_items[f.bp];
+            double quads_expected = 0.0;
+            // element mode: a batch's 4-sets must also fit the buffer of qualifying ones
+            double quads_budget = ctx->batch_frac *
+                                  (double)(ctx->elem_max ? std::min<u64>(ctx->quad_cap, PS4_ELEM_CAP) : ctx->quad_cap) *
+                                  budget_scale;
+            // the planner's estimate of 4-sets per S (an upper bound; scaled down
+            // only by the test hook that drills the redo path)
+            const double dscale = (double)ctx->density_pct / 100.0;
+            auto density = [&](u64 S) { return quad_density(S) * dscale; };
+            // A job's last batch has no device work to hide its host work (the
+            // lines), so a large job ends with a small batch: when what is left
+            // fits in this batch but is more than a sixth of a full one, this
+            // batch leaves about a sixth of a full one for the last.  Only below
+            // S = 10^11: higher up a batch has few lines (5-sets are 10x rarer
+            // per S at 10^12 than at 10^10), and a small batch there needs short
+            // superwindows, whose column walks cost about what long ones do.
+            {
+                const double tail = quads_budget / 6.0;
+                double left = 0.0;
+                u64 top = 0;
+                for (size_t i = next_iv; i < n; ++i) {
+                    left += density(iv[i].hi) * (double)(iv[i].hi - iv[i].cur);
+                    top = std::max(top, iv[i].hi);
                 }
-                ctx->h_items[n_items++] = {v.cur, v.cur + len, v.rid, 0};
-                s_top = std::max(s_top, v.cur + len);
-                quads_expected += quad_density(v.cur + len) * (double)len + item_pad;
-                v.cur += len;
-                rid_hi = v.rid;
+                if (top <= 100000000000ull && left <= quads_budget && left > 2.0 * tail) quads_budget = left - tail;
             }
+            u32 n_items = 0;
+            u64 s_top = 0;   // largest S of the batch
+            u32 rid_lo = iv[next_iv].rid, rid_hi = rid_lo;
+            // little work left (small queries): shorter superwindows, so that the
+            // representation kernel still has about one per workgroup
+            u64 remaining = 0;
+            for (size_t i = next_iv; i < n; ++i) remaining += iv[i].hi - iv[i].cur;
+            u64 item_max = ctx->W;
+            // null entries an item may leave in the 4-set queue: each wave of the
+            // representation kernel ends an item with up to two partly used tiles
+            const double item_pad = 128.0 * (double)(ctx->rq_block / 64);
+            while (item_max * ctx->rq_grid < remaining && item_max < ((u64)1 << 40)) item_max <<= 1;
+            // at least two superwindows per representation workgroup in a batch: at
+            // high S a batch's 4-set budget covers fewer S, and superwindows of
+            // ~2 sqrt(S) would leave workgroups idle (at S ~ 9*10^11: 820 per batch
+            // for 1024 workgroups)
+            auto sw_at = [&](u64 S) {
+                u64 sw = superwindow(ctx, S);
+                const double span = quads_budget / density(S);
+                while (sw > ctx->W && (double)sw * 2.0 * (double)ctx->rq_grid > span) sw >>= 1;
+                return sw;
+            };
+            for (size_t i = next_iv; i < n && n_items < ctx->items_cap && quads_expected < quads_budget; ++i) {
+                Interval &v = iv[i];
+                while (v.cur < v.hi && n_items < ctx->items_cap && quads_expected < quads_budget) {
+                    u64 len = std::min(sw_at(v.cur), item_max);
+                    if (len > v.hi - v.cur) len = v.hi - v.cur;
+                    const double dq = density(v.cur + len) * (double)len + item_pad;
+                    if (n_items > 0 && quads_expected + dq > quads_budget) {
+                        // shrink the last item to the remaining budget, in whole windows
+                        const double left = quads_budget - quads_expected - item_pad;
+                        u64 room = left > 0.0 ? (u64)(left / density(v.cur + len)) : 0;
+                        room = room / ctx->W * ctx->W;
+                        if (room == 0) { quads_expected = quads_budget; break; }
+                        len = room;
+                    }
+                    items[n_items++] = {v.cur, v.cur + len, v.rid, 0};
+                    s_top = std::max(s_top, v.cur + len);
+                    quads_expected += density(v.cur + len) * (double)len + item_pad;
+                    v.cur += len;
+                    rid_hi = v.rid;
+                }
+            }
+            f.rid_lo = rid_lo;
+            f.rid_hi = rid_hi;
+            if (int rc = launch_batch(ctx, f.bp, n_items, rid_hi + 1, s_top, noext, graph, quads_expected)) return fail(rc);
+            fl.push_back(std::move(f));
         }
-        // PS4_GRAPH: the sequence is captured and launched as one graph
-        const bool graph = (ctx->flags & PS4_GRAPH) && !ctx->debug_sync;
-        if (int rc = launch_batch(ctx, n_items, rid_hi + 1, s_top, noext, graph, quads_expected)) return fail(rc);
+        if (fl.empty()) break;
         if (have_pending) {
             have_pending = false;
             if (int rc = emit(pending)) return fail(rc);
         }
-        HIP_OK(hipStreamSynchronize(ctx->st));
+        // ---- the oldest batch in flight: wait for it and its read-backs
+        Flight f = std::move(fl.front());
+        fl.pop_front();
+        const int bp = f.bp;
+        HIP_OK(hipEventSynchronize(ctx->ev[bp][3]));
         float t_rq = 0.f, t_ext = 0.f;
-        HIP_OK(hipEventElapsedTime(&t_rq, ctx->ev[0], ctx->ev[1]));
-        HIP_OK(hipEventElapsedTime(&t_ext, ctx->ev[1], ctx->ev[2]));
+        HIP_OK(hipEventElapsedTime(&t_rq, ctx->ev[bp][0], ctx->ev[bp][1]));
+        HIP_OK(hipEventElapsedTime(&t_ext, ctx->ev[bp][1], ctx->ev[bp][2]));
         if (graph) { t_rq += t_ext; t_ext = 0.f; }   // one figure for the whole graph
         ctx->ms_rq += t_rq;
         ctx->ms_ext += t_ext;
         ++ctx->n_batches;
-        const u32 err = ctx->h_ctl->err;
-        if (err & ERR_CAP_REPS) { ctx->last_error = "representation capacity exceeded"; return PS4_E_CAPACITY; }
-        if (err & ERR_CAP_DIVS) { ctx->last_error = "divisor capacity exceeded"; return PS4_E_CAPACITY; }
-        if (err & ERR_VERIFY) { ctx->last_error = "quadruple verification failed"; return PS4_E_VERIFY; }
+        const ps4_ctl *hc = ctx->h_ctl[bp];
+        const u32 err = hc->err;
+        if (err & ERR_CAP_REPS) { ctx->last_error = "representation capacity exceeded"; return fail(PS4_E_CAPACITY); }
+        if (err & ERR_CAP_DIVS) { ctx->last_error = "divisor capacity exceeded"; return fail(PS4_E_CAPACITY); }
+        if (err & ERR_VERIFY) { ctx->last_error = "quadruple verification failed"; return fail(PS4_E_VERIFY); }
         if (err & (ERR_ARENA | ERR_FACTORS | ERR_LDS_OVF | ERR_BADQUAD | ERR_BOUNDS)) {
             char buf[96];
             snprintf(buf, sizeof buf, "kernel capacity flag 0x%x", err);
             ctx->last_error = buf;
-            return PS4_E_INTERNAL;
+            return fail(PS4_E_INTERNAL);
         }
         if (err & (ERR_QUAD_OVF | ERR_HIT_OVF | ERR_BIN_OVF)) {
-            // never truncate: redo this batch smaller (or with larger point bins)
-            std::copy(save.begin(), save.end(), iv.begin() + next_iv);
+            // never truncate: redo this batch smaller (or with larger point bins);
+            // the batch enqueued after it ran on the intervals as planned after
+            // this one, so it is dropped and planned again
+            HIP_OK(hipStreamSynchronize(ctx->st));
+            fl.clear();
+            std::copy(f.save.begin(), f.save.end(), iv.begin() + f.next_iv);
+            next_iv = f.next_iv;
             if (err & ERR_BIN_OVF) {
                 if (ctx->bin_cap >= 4 * ctx->W) { ctx->last_error = "window point bins overflow"; return PS4_E_INTERNAL; }
                 ctx->bin_cap *= 2;
@@ -787,11 +860,12 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
             }
             if (err & (ERR_QUAD_OVF | ERR_HIT_OVF)) budget_scale *= 0.5;
             if (budget_scale < 1e-6) { ctx->last_error = "batch cannot shrink further"; return PS4_E_INTERNAL; }
+            ++ctx->n_redo;
             continue;
         }
         // ---- counters
-        for (u32 r = rid_lo; r <= rid_hi; ++r) {
-            const u64 *c = ctx->h_ctr + (size_t)r * CT_N;
+        for (u32 r = f.rid_lo; r <= f.rid_hi; ++r) {
+            const u64 *c = ctx->h_ctr[bp] + (size_t)r * CT_N;
             ps4_counters &o = out[r];
             o.S_done += c[CT_S_DONE];
             o.pairs += c[CT_PAIRS];
@@ -803,7 +877,7 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
             o.max_reps = std::max(o.max_reps, c[CT_MAX_REPS]);
             o.max_divisors = std::max(o.max_divisors, c[CT_MAX_DIVS]);
         }
-        // ---- element mode: the batch's 4-sets with largest member <= elem_max
+        // ---- element mode (one batch in flight): the batch's 4-sets with largest member <= elem_max
         if (ctx->elem_max) {
             u32 ne = 0;
             HIP_OK(hipMemcpy(&ne, ctx->d_ecnt, sizeof ne, hipMemcpyDeviceToHost));
@@ -814,19 +888,20 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
                 HIP_OK(hipMemcpy(ctx->elem_sets.data() + old, ctx->d_eout, (size_t)ne * sizeof(ps4_quad),
                                  hipMemcpyDeviceToHost));
         }
-        // ---- this batch's 4-set dump and extensions, to be turned into lines
-        // while the next batch runs on the device
+        // ---- this batch's 4-set dump (one batch in flight) and extensions, to be
+        // turned into lines while the next batch runs on the device (the copies
+        // read buffers that batch does not write)
         Pending cur;
-        cur.rid_lo = rid_lo;
-        cur.rid_hi = rid_hi;
+        cur.rid_lo = f.rid_lo;
+        cur.rid_hi = f.rid_hi;
         bool want_dump = false;
-        for (u32 r = rid_lo; r <= rid_hi; ++r) want_dump |= dump_left[r] > 0;
-        if (want_dump && ctx->h_ctl->quad_count) {
-            cur.qs.resize(ctx->h_ctl->quad_count);
+        for (u32 r = f.rid_lo; r <= f.rid_hi; ++r) want_dump |= dump_left[r] > 0;
+        if (want_dump && hc->quad_count) {
+            cur.qs.resize(hc->quad_count);
             HIP_OK(hipMemcpy(cur.qs.data(), ctx->d_quads, cur.qs.size() * sizeof(ps4_quad), hipMemcpyDeviceToHost));
         }
-        if (const u32 nh = ctx->h_ctl->hit_count) {
-            HIP_OK(hipMemcpy(ctx->h_hits, ctx->d_hits, (size_t)nh * sizeof(ps4_hit), hipMemcpyDeviceToHost));
+        if (const u32 nh = hc->hit_count) {
+            HIP_OK(hipMemcpy(ctx->h_hits, ctx->d_hits[bp], (size_t)nh * sizeof(ps4_hit), hipMemcpyDeviceToHost));
             cur.hits.assign(ctx->h_hits, ctx->h_hits + nh);
         }
         pending = std::move(cur);
@@ -868,12 +943,14 @@ extern "C" void ps4_close(ps4_ctx *ctx)
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     if (ctx->st2) (void)hipStreamSynchronize(ctx->st2);
     void *dev[] = {ctx->d_ftab, ctx->d_oftab, ctx->d_ndiv, ctx->d_qr, ctx->d_ctl, ctx->d_ctr, ctx->d_items, ctx->d_scratch,
-                   ctx->d_quads, ctx->d_hits, ctx->d_exq, ctx->d_eout, ctx->d_ecnt, ctx->d_etri, ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
+                   ctx->d_quads, ctx->d_hits[0], ctx->d_hits[1], ctx->d_exq, ctx->d_eout, ctx->d_ecnt, ctx->d_etri,
+                   ctx->d_tabc, ctx->d_tout, ctx->d_tcnt, ctx->d_terr};
     for (void *p : dev) if (p) (void)hipFree(p);
-    void *host[] = {ctx->h_ctl, ctx->h_ctr, ctx->h_items, ctx->h_hits};
+    void *host[] = {ctx->h_ctl[0], ctx->h_ctl[1], ctx->h_ctr[0], ctx->h_ctr[1], ctx->h_items[0], ctx->h_items[1], ctx->h_hits};
     for (void *p : host) if (p) (void)hipHostFree(p);
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
-    for (hipEvent_t e : ctx->ev) if (e) (void)hipEventDestroy(e);
+    for (auto &evs : ctx->ev)
+        for (hipEvent_t e : evs) if (e) (void)hipEventDestroy(e);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
     if (ctx->st2) (void)hipStreamDestroy(ctx->st2);
     delete ctx;
@@ -901,7 +978,8 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     HIP_OK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&ctx->st2, hipStreamNonBlocking));
     HIP_OK(ps4_noalloc_scratch(ctx->device, &ctx->na_scratch));
-    for (hipEvent_t &e : ctx->ev) HIP_OK(hipEventCreate(&e));
+    for (auto &evs : ctx->ev)
+        for (hipEvent_t &e : evs) HIP_OK(hipEventCreate(&e));
     if (int rc = build_tables(ctx)) return rc;
     // resident workgroups per CU of every compiled launch shape; the window-bin
     // scratch holds the largest resident grid of them all
@@ -924,6 +1002,7 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     ctx->ext_grid_max = ctx->ext_grid;
     ctx->W = (u32)ps4_rq_window();
     ctx->debug_sync = env_u64("PS4_DEBUG_SYNC", 0) != 0;
+    ctx->density_pct = (u32)std::min<u64>(100, std::max<u64>(1, env_u64("PS4_TEST_DENSITY_PCT", 100)));
     // superwindows up to 2*sqrt(S_hi_max), at most 256 windows
     u32 nb = 1;
     while ((u64)ctx->W * nb < 2 * isqrt64(ctx->S_hi_max) + 1 && nb < 512) nb <<= 1;
@@ -941,11 +1020,13 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     ctx->hit_cap = 1u << 20;
     ctx->items_cap = 1u << 20;
     HIP_OK(hipMalloc(&ctx->d_ctl, sizeof(ps4_ctl)));
-    HIP_OK(hipHostMalloc(&ctx->h_ctl, sizeof(ps4_ctl)));
     HIP_OK(hipMalloc(&ctx->d_items, (size_t)ctx->items_cap * sizeof(ps4_item)));
-    HIP_OK(hipHostMalloc(&ctx->h_items, (size_t)ctx->items_cap * sizeof(ps4_item)));
+    for (int bp = 0; bp < 2; ++bp) {
+        HIP_OK(hipHostMalloc(&ctx->h_ctl[bp], sizeof(ps4_ctl)));
+        HIP_OK(hipHostMalloc(&ctx->h_items[bp], (size_t)ctx->items_cap * sizeof(ps4_item)));
+        HIP_OK(hipMalloc(&ctx->d_hits[bp], (size_t)ctx->hit_cap * sizeof(ps4_hit)));
+    }
     HIP_OK(hipMalloc(&ctx->d_quads, (size_t)ctx->quad_cap * sizeof(ps4_quad)));
-    HIP_OK(hipMalloc(&ctx->d_hits, (size_t)ctx->hit_cap * sizeof(ps4_hit)));
     HIP_OK(hipMalloc(&ctx->d_exq, (size_t)ctx->ext_grid * (size_t)ps4_ext_queue_cap() * sizeof(u64)));
     HIP_OK(hipHostMalloc(&ctx->h_hits, (size_t)ctx->hit_cap * sizeof(ps4_hit)));
     if (int rc = alloc_scratch(ctx)) return rc;
@@ -1086,6 +1167,8 @@ extern "C" int ps4_search_batch(ps4_ctx *ctx, size_t n, const uint64_t *lo, cons
 
 // Kernel time (ms, HIP events on the context's stream) accumulated since the
 // last reset, and the number of batches; reset when reset != 0.
+extern "C" uint64_t ps4_redo_count(ps4_ctx *ctx) { return ctx ? ctx->n_redo : 0; }
+
 extern "C" int ps4_kernel_time(ps4_ctx *ctx, double *ms_rq, double *ms_ext, uint64_t *batches, int reset)
 {
     if (!ctx) return PS4_E_ARG;

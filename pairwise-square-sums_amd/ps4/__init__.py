@@ -145,6 +145,8 @@ def load():
     lib.ps4_strerror.restype = ctypes.c_char_p
     lib.ps4_last_error.argtypes = [P]
     lib.ps4_last_error.restype = ctypes.c_char_p
+    lib.ps4_redo_count.argtypes = [P]
+    lib.ps4_redo_count.restype = ctypes.c_uint64
     lib.ps4_kernel_time.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                     u64p, ctypes.c_int]
     _lib = lib
@@ -302,6 +304,10 @@ class Engine:
         st = StreamState()
         self._check(self.lib.ps4_step(self.ctx, int(max_quads), f, None, ctypes.byref(st)))
         return out, {k: int(getattr(st, k)) for k, _ in StreamState._fields_}
+
+    def redo_count(self):
+        """Batches redone after a device queue overflowed, since the engine opened."""
+        return int(self.lib.ps4_redo_count(self.ctx))
 
     def kernel_time(self, reset=False):
         """(ms in k_rq, ms in k_ext, batches) accumulated on the device stream."""

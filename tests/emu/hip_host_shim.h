@@ -233,6 +233,7 @@ static inline hipError_t hipEventCreate(hipEvent_t *e) { *e = new double(0); ret
 static inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 static inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { *e = emu_now_ms(); return hipSuccess; }
 static inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(*b - *a); return hipSuccess; }
+static inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }   // launches run to completion
 template <typename T> static inline hipError_t hipMalloc(T **p, size_t n)
 {
     *p = (T *)malloc(n ? n : 1);

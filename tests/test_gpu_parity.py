@@ -64,9 +64,10 @@ def test_single_s_4205_regression(eng, golden):
     bring-up the GPU took a memory fault on it: residues mod 11..19 computed as
     __umul24(s, M) >> k were compiled to a 24-bit multiply followed by an
     arithmetic shift, so a product of 2^31 or more gave a negative residue that
-    indexed far past a residue table.  Fixed with plain 32-bit multiplies and a
-    bounds check on every table index (a bad index raises ERR_BOUNDS instead of
-    loading); test_abi.py checks the compiled kernels for arithmetic shifts."""
+    indexed far past a residue table.  Fixed with plain 32-bit multiplies; the
+    residue inputs are clamped to the domains on which every residue is checked
+    exhaustively (test_abi.py), so no index can leave its table, and test_abi.py
+    checks the compiled kernels for arithmetic shifts."""
     rec = next(r for r in golden["runs"] if (r["lo"], r["hi"]) == (4205, 4206))
     say("single S [4205,4206)")
     res, lines = eng.search(4205, 4206)
@@ -333,6 +334,42 @@ def test_fixtures_1e8_1e9(eng, golden):
         say(f"fixture [1,{1 + span})")
         res, lines = eng.search(1, 1 + span)
         check(rec, res, lines)
+
+
+def test_overflowing_batches_are_redone(eng, golden):
+    """A batch whose 4-sets overflow the queue is redone smaller, never
+    truncated, also when the next batch is already in flight: with the
+    planner's 4-set estimate scaled to 5 % (a test hook) the first attempts
+    overflow a 2^20-entry queue and are redone; [1,1e9) still equals s4's run
+    over the same range, and uneven subranges in one batched call equal the
+    main engine's."""
+    rec = next(r for r in golden["runs"] if (r["lo"], r["hi"]) == (1, 10**9))
+    from ps4 import workloads
+    ranges = workloads.batched_ranges(64, seed=5, top=10**9, e0=4, e1=8)
+    los, his = [a for a, _ in ranges], [b for _, b in ranges]
+    ref_b = eng.search_batch(los, his)
+    engines = []
+    for pct in ("100", "5"):
+        os.environ["PS4_QUAD_CAP"] = str(1 << 20)
+        os.environ["PS4_TEST_DENSITY_PCT"] = pct
+        try:
+            engines.append(ps4.Engine(S_hi_max=10**9 + 1))
+        finally:
+            del os.environ["PS4_QUAD_CAP"], os.environ["PS4_TEST_DENSITY_PCT"]
+    try:
+        for e, pct in zip(engines, ("100", "5")):
+            say(f"[1,1e9) with a 2^20-entry queue, estimate at {pct} %")
+            e.kernel_time(reset=True)
+            res, lines = e.search(1, 10**9)
+            check(rec, res, lines)
+            say(f"uneven subranges, estimate at {pct} %")
+            for r, (c, ls), (c0, l0) in zip(ranges, e.search_batch(los, his), ref_b):
+                assert c == c0 and sorted(ls) == sorted(l0), (r, pct)
+        assert engines[0].redo_count() == 0 and engines[1].redo_count() > 0
+        assert engines[0].kernel_time()[2] > 1
+    finally:
+        for e in engines:
+            e.close()
 
 
 def test_high_s_fixtures(eng, golden):
