@@ -754,13 +754,14 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
             auto density = [&](u64 S) { return quad_density(S) * dscale; };
             // A job's last batch has no device work to hide its host work (the
             // lines), so a large job ends with a small batch: when what is left
-            // fits in this batch but is more than a sixth of a full one, this
-            // batch leaves about a sixth of a full one for the last.  Only below
+            // fits in this batch but is more than twice the tail (a sixth of a
+            // full batch, at most 1.4*10^8 4-sets: about 2 ms of host lines at
+            // 10^10), this batch leaves the tail for the last.  Only below
             // S = 10^11: higher up a batch has few lines (5-sets are 10x rarer
             // per S at 10^12 than at 10^10), and a small batch there needs short
             // superwindows, whose column walks cost about what long ones do.
             {
-                const double tail = quads_budget / 6.0;
+                const double tail = std::min(quads_budget / 6.0, 1.4e8);
                 double left = 0.0;
                 u64 top = 0;
                 for (size_t i = next_iv; i < n; ++i) {
@@ -1008,10 +1009,12 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     while ((u64)ctx->W * nb < 2 * isqrt64(ctx->S_hi_max) + 1 && nb < 512) nb <<= 1;
     ctx->nb_alloc = std::min<u32>(nb, (u32)ps4_rq_nb_max());
     ctx->bin_cap = ctx->W / 2 + 1024;
-    // 4-set queue: enough for the whole range when it is small, else 2^30 entries
-    // (32 GiB, a few batches per 10^10 S), at most 40% of the free device memory
+    // 4-set queue: enough for the whole range when it is small, else 2^31 entries
+    // (64 GiB of the 288 GB: [1,10^10) in two batches and a small last one; each
+    // batch boundary costs the two kernels' tails), at most 40% of the free
+    // device memory
     const double whole = quad_density(ctx->S_hi_max) * (double)ctx->S_hi_max * 1.5 + 65536.0;
-    u64 qcap = env_u64("PS4_QUAD_CAP", 1ull << 30);
+    u64 qcap = env_u64("PS4_QUAD_CAP", 1ull << 31);
     if ((double)qcap > whole) qcap = (u64)whole;
     size_t fr = 0, tot = 0;
     HIP_OK(hipMemGetInfo(&fr, &tot));
