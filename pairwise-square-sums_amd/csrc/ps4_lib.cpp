@@ -768,7 +768,9 @@ int run_job(ps4_ctx *ctx, size_t n, const u64 *lo_in, const u64 *hi_in, ps4_sink
                     left += density(iv[i].hi) * (double)(iv[i].hi - iv[i].cur);
                     top = std::max(top, iv[i].hi);
                 }
-                if (top <= 100000000000ull && left <= quads_budget && left > 2.0 * tail) quads_budget = left - tail;
+                // (not in the element mode: its lines are made after the last batch)
+                if (!ctx->elem_max && top <= 100000000000ull && left <= quads_budget && left > 2.0 * tail)
+                    quads_budget = left - tail;
             }
             u32 n_items = 0;
             u64 s_top = 0;   // largest S of the batch
