@@ -1236,8 +1236,20 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
             qix = (u32)pqi;
             lrid = qd.rid;
         }
+        // The odd-factor entries of D's two odd parts: gathered below for the
+        // lanes set up now, and gathered again here for the lanes that hold a
+        // 4-set over from an earlier pass (stage words are not kept in registers
+        // across the candidate loop), so that one merge into stage words serves
+        // both kinds of lane.
+        const bool carried = has;
+        ExOdd f1 = {}, f2 = {};
+        if (carried) {
+            f1 = ex_odd_at(a.oftab, o1n & 0x0FFFFFFFu);
+            f2 = ex_odd_at(a.oftab, o2n & 0x0FFFFFFFu);
+        }
+        u32 o1 = 1;
         if (valid) {
-            u32 o1 = 1, o2 = 1;
+            u32 o2 = 1;
                 // The divisor counts of the pair choice below, gathered first so
                 // that their latency overlaps the checks; each index is clamped
                 // into the table (a no-op for a well-formed 4-set, whose indices
@@ -1288,7 +1300,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     o2 = m2 >> z2;
                     o1n = o1;
                     o2n = o2;
-                    const ExOdd f1 = ex_odd_at(a.oftab, o1), f2 = ex_odd_at(a.oftab, o2);
+                    f1 = ex_odd_at(a.oftab, o1);
+                    f2 = ex_odd_at(a.oftab, o2);
                     // quadratic-residue masks of this 4-set (read together with the
                     // odd-factor entries, so that the reads overlap): bit r of M_m set iff
                     // r^2 + (C1-A) and r^2 + (C2-A) are both squares mod m (the
@@ -1332,29 +1345,25 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // reference's ext_cand.
                     half = (e2 == 0);
                     e2p = e2 >= 2 ? e2 - 2 : 0u;
-                    u32 ndodd;   // d of the odd part (< 2^32 for D < 2^51)
-                    ex_stages(f1, f2, o1, e2p, st, &ndodd);
-                    n1 = f1.b.w & 15u;
-                    n2 = f2.b.w & 15u;
-                    o1n |= n1 << 28;
-                    o2n |= n2 << 28;
-                    const u64 ndD = (u64)(e2 + 1) * ndodd;
-                    my_maxdiv = ndD;
-                    if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
-                    if (e2 != 1 && ndD <= a.cap_divs) {
-                        ndp = (e2p + 1) * ndodd;
-                        ncand = ndp / 2;
-                    }
                 }
             }
-        // the odd-factor entries of the 4-sets held over from an earlier pass
-        // (their stage words are not kept in registers across the candidate
-        // loop): issued now, merged before the list building
-        const bool carried = has;
-        ExOdd g1 = {}, g2 = {};
-        if (carried) {
-            g1 = ex_odd_at(a.oftab, o1n & 0x0FFFFFFFu);
-            g2 = ex_odd_at(a.oftab, o2n & 0x0FFFFFFFu);
+        if (valid || carried) {
+            // stage words of D' (a lane carried over keeps its o1 and e2p)
+            u32 ndodd;   // d of the odd part (< 2^32 for D < 2^51)
+            ex_stages(f1, f2, carried ? (o1n & 0x0FFFFFFFu) : o1, carried ? ((lpk >> 4) & 63u) : e2p, st, &ndodd);
+            if (valid) {
+                n1 = f1.b.w & 15u;
+                n2 = f2.b.w & 15u;
+                o1n |= n1 << 28;
+                o2n |= n2 << 28;
+                const u64 ndD = (u64)(e2 + 1) * ndodd;
+                my_maxdiv = ndD;
+                if (ndD > a.cap_divs) err |= ERR_CAP_DIVS;
+                if (e2 != 1 && ndD <= a.cap_divs) {
+                    ndp = (e2p + 1) * ndodd;
+                    ncand = ndp / 2;
+                }
+            }
         }
         // Split the stage sequence into a prefix L (positions < sL) and the
         // rest H, minimising |L| + |H| = nL + nH over the prefixes; ndp / nl is
@@ -1449,10 +1458,6 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // positions before sL build L, the rest H.  When u = (f + g) / 2
                     // the H list holds halves (exact doubles), so that
                     // u = L[i] * H[j] + L[i'] * H[j'] in both cases.
-                    if (carried) {   // (a 4-set set up in this round has its words from the set-up)
-                        u32 nd2;
-                        ex_stages(g1, g2, o1n & 0x0FFFFFFFu, e2p, st, &nd2);
-                    }
                     const u32 *sw = st;
                     const double h0 = half ? 0.5 : 1.0;
                     double *lst = s_ar + loff;
