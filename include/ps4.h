@@ -147,7 +147,7 @@ int ps4_submit_range(ps4_ctx *ctx, uint64_t tag, uint64_t S_lo, uint64_t S_hi);
 /* Queue a triple request (checked as by ps4_extend_triple). */
 int ps4_submit_triple(ps4_ctx *ctx, uint64_t tag, const uint64_t abc[3], uint64_t d_max);
 /* Run one step; requests it finishes go to done(user, ...).  max_quads: the
- * step's budget of expected 4-sets (0: 2^23, about a millisecond of work);
+ * step's budget of expected 4-sets (0: 2^26, a whole 10^8-wide request at 10^12);
  * *state (may be NULL) gets what remains queued and the step's capacities. */
 int ps4_step(ps4_ctx *ctx, uint64_t max_quads, ps4_done_fn done, void *user, ps4_stream_state *state);
 

@@ -45,8 +45,8 @@ class Request:
 class Server:
     """Continuous-batching loop over one ps4.Engine (the engine must not be
     used by anyone else while the server runs).  step_quads: the budget of
-    expected 4-sets per step (0: the library's default, about a millisecond of
-    device work)."""
+    expected 4-sets per step (0: the library's default, 2^26: a whole 10^8-wide
+    request at 10^12)."""
 
     def __init__(self, engine, step_quads=0, clock=time.perf_counter):
         self.eng = engine
