@@ -10,11 +10,13 @@
 //        every S with >= 3 representations, S != 2 mod 4, into 4-sets (pass 4,
 //        :311-341).  Triples are spread evenly over the threads; each wave
 //        appends its 4-sets to its own 64-entry tiles of a global queue.
-// k_ext  one single-wave workgroup per tile of 64 queued 4-sets (persistent,
-//        work queue): per lane the pair choice (:163-171), the factorisation of
-//        D = (ra-rb)(ra+rb) by a resident smallest-prime table, the closed-form
-//        candidate count, and two half divisor lists in an LDS arena; then the
-//        divisor pairs of all 64 4-sets are flattened over the lanes.  Each
+// k_ext  single-wave workgroups over the queued 4-sets (persistent, claims of
+//        4 tiles of 64): each lane holds one 4-set at a time and sets it up --
+//        the pair choice (:163-171), the factorisation of D = (ra-rb)(ra+rb) by
+//        one gather of each odd part's entry in a resident odd-factor table, the
+//        closed-form candidate count, and the split of its divisors into two
+//        lists; a pass then writes the lists of as many held 4-sets as fit an
+//        LDS arena and flattens all their divisor pairs over the lanes.  Each
 //        candidate is filtered by per-4-set quadratic-residue masks (mod 64, 63,
 //        65: the ext_sqrt rule of :192-198), by the exact e>0 / e not in the set
 //        conditions, then by residue masks mod 11, 13, 17, 19 and a size bound;

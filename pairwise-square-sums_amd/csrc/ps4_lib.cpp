@@ -971,8 +971,8 @@ static int open_impl(const ps4_config *cfg, ps4_ctx *ctx)
     if (ctx->cap_divs > 16384) { ctx->last_error = "cap_divs above 16384 exceeds the divisor-list arena"; return PS4_E_ARG; }
     if (cfg->cap_ext) ctx->cap_ext = cfg->cap_ext;
     ctx->S_hi_max = cfg->S_hi_max;
-    // 2^50: keeps every exponent of an odd prime in D below 32 (5-bit table field)
-    // and every divisor product an exact double
+    // 2^50: keeps every root below 2^26 (the odd-factor table's entries and the
+    // kernels' 32-bit root arithmetic) and every divisor product an exact double
     if (ctx->S_hi_max < 2 || ctx->S_hi_max > (1ull << 50)) { ctx->last_error = "S_hi_max out of range"; return PS4_E_ARG; }
     HIP_OK(hipSetDevice(ctx->device));
     hipDeviceProp_t prop;
