@@ -252,6 +252,8 @@ __device__ __forceinline__ u32 tri3(u32 k) { return __umulhi((k * (k - 1) >> 1) 
 #define RQ_QS_CAP 512       // counted S of a window (about 140, at most about 160 in samples)
 #define RQ_NB_MAX 1024
 #define RQ_K_MAX 2000u      // largest bucket whose triples a window indexes (C(2000,3) < 2^31)
+#define RQ_RUNS 2           // column runs a thread walks in lockstep (4 measured the same, and
+                            // spills one register at the 96-register bound)
 
 struct RqArgs {
     const ps4_item *items;
@@ -508,46 +510,102 @@ __global__ __launch_bounds__(NT, NT == 256 ? 5 : 1) void k_rq(RqArgs a)
         __syncthreads();
 
         // ---- every lattice point 1 <= p <= q, X0 <= p^2+q^2 < X1, binned by window.
-        // Each thread owns a run of consecutive columns p and follows the inner
-        // circle incrementally: off = p^2 + q^2 - X0 with q the least admissible root.
+        // The columns p = 1..pmax are cut into NT * RQ_RUNS runs of equal length;
+        // a thread follows RQ_RUNS runs in lockstep (runs tid, tid + NT, ...), each
+        // along the inner circle: q the least admissible root of its column and
+        // off = p^2 + q^2 - X0.  All of it fits in 32 bits: q > p gives
+        // off < 2q - 1 < 2^27, and q = p (the diagonal) gives off = 2p^2 - X0 <
+        // len.  Moving to the next column takes q down by at most one: if q >= p + 2,
+        // p^2 + (q-1)^2 < X0 gives (p+1)^2 + (q-2)^2 = p^2 + (q-1)^2 + 2(p - q + 2) < X0;
+        // if q = p + 1 the new column's diagonal bounds q; if q = p, q moves onto the
+        // new diagonal.  So a step is straight-line selects; the points of a step's
+        // columns (rare at high S, where a column meets a narrow window in a point
+        // with probability about len / 2q) are emitted only when some lane has one,
+        // their bin slots claimed together.  About 20 VALU operations per column:
+        // when a short job at high S leaves one wave per SIMD, the walk is bound
+        // by that wave's issue rate (a 4096-wide superwindow at 9e11 walks
+        // 6.7e5 columns in about 0.3 ms; the 64-bit loop with branches it
+        // replaces took 0.8 ms).
         {
-            const u64 pmax = d_isqrt64((X1 - 1) >> 1);
-            const u64 per = (pmax + NT - 1) / NT;
-            u64 p = 1 + (u64)tid * per;
-            if (per > 0 && p <= pmax) {
-                const u64 pend = (p + per - 1 < pmax) ? p + per - 1 : pmax;
+            const u32 pmax = (u32)d_isqrt64((X1 - 1) >> 1);
+            const u32 runs = (u32)NT * RQ_RUNS;
+            const u32 per = (pmax + runs - 1) / runs;   // columns per run (uniform)
+            const u32 ulen = (u32)len;
+            // a run's columns are p0 .. p0 + per - 1, cut at pmax: as the walk takes
+            // per steps, a run is at a column of its own exactly while p <= pmax
+            u32 cp[RQ_RUNS], cq[RQ_RUNS], co[RQ_RUNS];
+#pragma unroll
+            for (int r = 0; r < RQ_RUNS; ++r) {
+                const u64 p0 = 1 + (u64)(tid + (u32)r * NT) * per;
+                cp[r] = (u32)min(p0, (u64)pmax + 1u);
+                const u64 pp = (u64)cp[r] * cp[r];
                 u64 q;
-                const u64 pp = p * p;
                 if (X0 <= 2 * pp) {
-                    q = p;
+                    q = cp[r];
                 } else {
-                    const u64 t = X0 - pp;
-                    q = d_isqrt64(t);
+                    const u64 t = X0 - pp;   // in [1, 2^50)
+                    q = d_isqrt52r(t);
                     if (q * q < t) ++q;
                 }
-                u64 off = pp + q * q - X0;
-                for (;;) {
-                    u64 o = off, qq = q;
-                    while (o < len) {
-                        const u32 bin = (u32)(o >> RQ_LOGW);
-                        if (bin >= nb) break;     // o < len implies bin < nb
-                        const u32 slot = atomicAdd(&s_bin[bin], 1u);
-                        // a point: offset in its window | p << RQ_LOGW | q << 38 (p < 2^25
-                        // and q < 2^26 for S < 2^50)
-                        if (slot < a.bin_cap)
-                            scr[(u64)bin * a.bin_cap + slot] = (qq << 38) | (p << RQ_LOGW) | (o & (RQ_W - 1));
-                        o += 2 * qq + 1;
-                        ++qq;
+                cq[r] = (u32)q;
+                co[r] = (u32)(pp + q * q - X0);
+            }
+            // a point: offset in its window | p << RQ_LOGW | q << 38 (p < 2^25 and
+            // q < 2^26 for S < 2^50), in the bin of its window
+            auto point = [](u32 pc, u32 qc, u32 o) {
+                return ((u64)qc << 38) | ((u64)pc << RQ_LOGW) | (o & (RQ_W - 1));
+            };
+            for (u32 st = 0; st < per; ++st) {
+                u32 hm = 0;   // bit r: run r's column has a point at its least root
+#pragma unroll
+                for (int r = 0; r < RQ_RUNS; ++r)   // p <= pmax and off < len, as sign bits (|differences| < 2^31)
+                    hm |= (((co[r] - ulen) & ~(pmax - cp[r])) >> 31) << r;
+                if (__ballot(hm != 0u) != 0ull) {
+                    // the slots of the first points, claimed before any is used (a lane
+                    // without a point adds 0 to one of 32 counters, one per LDS bank)
+                    u32 sl[RQ_RUNS];
+#pragma unroll
+                    for (int r = 0; r < RQ_RUNS; ++r) {
+                        const bool h = (hm >> r) & 1u;   // o < len: the bin is below nb
+                        sl[r] = atomicAdd(&s_bin[h ? co[r] >> RQ_LOGW : lane_id() & 31u], h ? 1u : 0u);
                     }
-                    if (p == pend) break;
-                    off += 2 * p + 1;
-                    ++p;
-                    if (q < p) {
-                        off += 2 * p - 1;   // q was p-1: move onto the diagonal q = p
-                        q = p;
-                    } else {
-                        while (q > p && off >= 2 * q - 1) { off -= 2 * q - 1; --q; }
+#pragma unroll
+                    for (int r = 0; r < RQ_RUNS; ++r)   // (element indices below nb_alloc * bin_cap < 2^23)
+                        if (((hm >> r) & 1u) && sl[r] < a.bin_cap)
+                            scr[(co[r] >> RQ_LOGW) * a.bin_cap + sl[r]] = point(cp[r], cq[r], co[r]);
+                    // the further points of these columns (several per column at low S)
+#pragma unroll
+                    for (int r = 0; r < RQ_RUNS; ++r) {
+                        u32 o = co[r] + 2u * cq[r] + 1u, qq = cq[r] + 1u;
+                        bool more = ((hm >> r) & 1u) && o < ulen;
+                        while (__ballot(more) != 0ull) {
+                            if (more) {
+                                const u32 bin = o >> RQ_LOGW;
+                                const u32 slot = atomicAdd(&s_bin[bin], 1u);
+                                if (slot < a.bin_cap) scr[bin * a.bin_cap + slot] = point(cp[r], qq, o);
+                            }
+                            o += 2u * qq + 1u;
+                            ++qq;
+                            more = more && o < ulen;
+                        }
                     }
+                }
+                // next column: p^2 grows by 2p + 1; then one step of q down, or onto
+                // the diagonal (q was p - 1); runs past their last column go on
+                // harmlessly (their columns never count)
+#pragma unroll
+                for (int r = 0; r < RQ_RUNS; ++r) {
+                    // (the tests as sign bits of differences below 2^31 in size, and
+                    // selects: VALU work only, nothing through the scalar unit)
+                    const u32 pc = cp[r] + 1u, qc = cq[r];
+                    const u32 t = 2u * qc - 1u;
+                    u32 o = co[r] + 2u * cp[r] + 1u;
+                    const bool dec = (((pc - qc) & ~(o - t)) >> 31) != 0u;   // q > p and o >= 2q - 1
+                    const bool diag = ((qc - pc) >> 31) != 0u;               // q < p
+                    o = o - (dec ? t : 0u) + (diag ? 2u * pc - 1u : 0u);
+                    cq[r] = qc - (dec ? 1u : 0u) + (diag ? 1u : 0u);
+                    cp[r] = pc;
+                    co[r] = o;
                 }
             }
         }

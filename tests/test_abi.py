@@ -111,8 +111,11 @@ def test_kernel_residues_mod_63_65_are_exact(name, m):
 def test_codegen_has_no_signed_shift_of_unsigned_products(tmp_path):
     """Regression guard for a bring-up fault: the gfx950 code of the kernels must
     not shift a multiply result arithmetically (with __umul24 the compiler emitted
-    v_mul_u32_u24 + v_ashrrev_i32, turning residues >= 2^31/magic into indices
-    ~90000 and reading far past the residue tables)."""
+    v_mul_u32_u24 + v_ashrrev_i32 by 19 or 20, turning residues >= 2^31/magic into
+    indices ~90000 and reading far past the residue tables).  An arithmetic shift
+    by 31 is a sign mask (0 or all ones, whatever its operand: the compiler makes
+    it from selects on the sign of a difference) and is allowed; any other
+    amount is not."""
     import shutil
     import subprocess
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -125,7 +128,9 @@ def test_codegen_has_no_signed_shift_of_unsigned_products(tmp_path):
                     "--cuda-device-only", "-S", src, "-o", str(out)], check=True, stdout=subprocess.PIPE,
                    stderr=subprocess.PIPE)
     asm = out.read_text()
-    assert "v_ashrrev_i32" not in asm
+    shifts = re.findall(r"v_ashrrev_i32(?:_e32|_e64)?\s+v\d+,\s*([^,\s]+),", asm)
+    assert len(shifts) == asm.count("v_ashrrev_i32")   # every one parsed
+    assert all(a == "31" for a in shifts), sorted(set(shifts))
     assert "__umul24(" not in open(src).read()
 
 
