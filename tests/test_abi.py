@@ -1,6 +1,8 @@
 """C-ABI surface and host-side constants -- CPU only, no device calls."""
 import ctypes
 import os
+import math
+import random
 import re
 
 import pytest
@@ -199,3 +201,74 @@ def test_triple_count_of_a_bucket_is_exact_in_32_bits():
         x = (k * (k - 1) >> 1) * (k - 2)
         assert x < 1 << 32
         assert ((x * 0xAAAAAAAB) >> 32) >> 1 == math.comb(k, 3), k
+
+
+def _walk_run(X0, X1, p_first, p_last):
+    """The representation kernel's column walk over columns p_first..p_last of
+    the superwindow [X0, X1), step for step in 32-bit arithmetic: the start from
+    an integer square root, then per column the points at q, q+1, ... while
+    p^2 + q^2 < X1, and the move to the next column with at most one step of q
+    down (or onto the diagonal).  Returns the points (p, q)."""
+    M = 0xFFFFFFFF
+    ulen = X1 - X0
+    pp = p_first * p_first
+    if X0 <= 2 * pp:
+        q = p_first
+    else:
+        t = X0 - pp
+        q = math.isqrt(t)
+        if q * q < t:
+            q += 1
+    o = pp + q * q - X0
+    assert 0 <= o < 1 << 32
+    pc, pts = p_first, []
+    while pc <= p_last:
+        oo, qq = o, q
+        while oo < ulen:
+            pts.append((pc, qq))
+            oo = (oo + 2 * qq + 1) & M
+            qq += 1
+        pn = pc + 1
+        t = (2 * q - 1) & M
+        o = (o + 2 * pc + 1) & M
+        dec = (((pn - q) & M) & (~((o - t) & M) & M)) >> 31
+        diag = ((q - pn) & M) >> 31
+        o = (o - (t if dec else 0) + ((2 * pn - 1) if diag else 0)) & M
+        q = q - dec + diag
+        pc = pn
+    return pts
+
+
+def test_lattice_walk_matches_brute_force():
+    """The column walk of the representation kernel finds exactly the lattice
+    points 1 <= p <= q, X0 <= p^2 + q^2 < X1: whole superwindows at small S
+    (every column, including the diagonal and X0 < 64), and runs of columns at
+    high S (up to 1e13) against the direct least root of every column.  Pins
+    the claim that one column step moves q down at most once."""
+    import math
+    rng = random.Random(7)
+    cases = [(1, 64), (2, 3), (5, 4101), (4096, 8192), (60, 64)]
+    cases += [(x, x + rng.choice([64, 4096, 1 << 14])) for x in (rng.randrange(1, 10**7) for _ in range(30))]
+    for X0, X1 in cases:
+        pmax = math.isqrt((X1 - 1) // 2)
+        got = _walk_run(X0, X1, 1, pmax) if pmax >= 1 else []
+        want = [(p, q) for p in range(1, pmax + 1)
+                for q in range(max(p, math.isqrt(max(X0 - p * p, 0))), math.isqrt(X1 - 1 - p * p) + 1)
+                if X0 <= p * p + q * q < X1]
+        assert got == want, (X0, X1)
+    for _ in range(40):
+        X0 = rng.randrange(10**9, 10**13)
+        X1 = X0 + rng.choice([4096, 1 << 17, 1 << 21])
+        pmax = math.isqrt((X1 - 1) // 2)
+        a = rng.randrange(1, pmax)
+        b = min(pmax, a + 400)
+        got = _walk_run(X0, X1, a, b)
+        want = []
+        for p in range(a, b + 1):
+            q0 = max(p, math.isqrt(X0 - p * p - 1) + 1 if X0 > p * p else p)
+            q = q0
+            while p * p + q * q < X1:
+                if p * p + q * q >= X0:
+                    want.append((p, q))
+                q += 1
+        assert got == want, (X0, X1, a)
