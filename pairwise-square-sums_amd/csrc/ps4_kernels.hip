@@ -1010,31 +1010,42 @@ __device__ __forceinline__ void ex_stages(const ExOdd &f1, const ExOdd &f2, u32 
 {
     const u32 w1[EX_OSLOT] = {f1.a.x, f1.a.y, f1.a.z, f1.a.w, f1.b.x, f1.b.y, f1.b.z};
     const u32 w2[EX_OSLOT] = {f2.a.x, f2.a.y, f2.a.z, f2.a.w, f2.b.x, f2.b.y, f2.b.z};
-    st[0] = e2p ? (2u << 6) | e2p : 0u;
-#pragma unroll
-    for (int i = 0; i < EX_OSLOT; ++i) st[1 + i] = w1[i];
     u32 d = (f1.b.w >> 4) * (f2.b.w >> 4);
     const double o = (double)o1;
     const u32 n2 = f2.b.w & 15u;   // slots of o2 in use (the others hold 0)
+    // The merges are recorded in three values only (the exponents added to o1's
+    // slots, six bits each; the merged o2 slots; d), so that the branches around
+    // them, taken by some lane in most waves, do not carry the stage words; the
+    // stage words are written once afterwards.
+    u64 add1 = 0;
+    u32 gone2 = 0;
 #pragma unroll
     for (int j = 0; j < EX_OSLOT; ++j) {
-        u32 w = w2[j];
-        if ((u32)j < n2 && ex_divides(w >> 6, o)) {   // a prime of both parts (rare)
+        const u32 w = w2[j];
+        if ((u32)j < n2 && ex_divides(w >> 6, o)) {   // a prime of both parts
             const u32 p = w >> 6, eb = w & 63u;
 #pragma unroll
             for (int i = 0; i < EX_OSLOT; ++i) {
-                const u32 v = st[1 + i];
-                if ((v >> 6) == p) {
-                    // d: the factors (ea + 1)(eb + 1) of the two parts become ea + eb + 1
-                    const u32 ea = v & 63u;
-                    d = d / ((ea + 1u) * (eb + 1u)) * (ea + eb + 1u);
-                    st[1 + i] = v + eb;
+                if ((w1[i] >> 6) == p) {
+                    // d: the factors (ea + 1)(eb + 1) of the two parts become ea + eb + 1.
+                    // The quotient is exact in single precision: den divides d, and
+                    // d <= d(o1) d(o2) <= 2^16 (an odd n < 2^26 + 64 has at most 256
+                    // divisors), so the hardware reciprocal (1 ulp) leaves an error
+                    // below 2^-6 before rounding
+                    const u32 ea = w1[i] & 63u;
+                    const float den = (float)((ea + 1u) * (eb + 1u));
+                    d = (u32)__builtin_rintf((float)d * __builtin_amdgcn_rcpf(den)) * (ea + eb + 1u);
+                    add1 += (u64)eb << (6 * i);
                 }
             }
-            w = 0u;
+            gone2 |= 1u << j;
         }
-        st[1 + EX_OSLOT + j] = w;
     }
+    st[0] = e2p ? (2u << 6) | e2p : 0u;
+#pragma unroll
+    for (int i = 0; i < EX_OSLOT; ++i) st[1 + i] = w1[i] + (u32)((add1 >> (6 * i)) & 63u);
+#pragma unroll
+    for (int j = 0; j < EX_OSLOT; ++j) st[1 + EX_OSLOT + j] = ((gone2 >> j) & 1u) ? 0u : w2[j];
     *nd = d;
 }
 

@@ -272,3 +272,37 @@ def test_lattice_walk_matches_brute_force():
                     want.append((p, q))
                 q += 1
         assert got == want, (X0, X1, a)
+
+
+def test_divisor_count_correction_is_exact_in_single_precision():
+    """When ra - rb and ra + rb share a prime, the extension kernel corrects
+    d = d(o1) d(o2) by d / ((ea+1)(eb+1)) * (ea+eb+1) with a single-precision
+    quotient: rint(d * rcp(den)).  Exact when den | d, d <= 2^16 (no odd
+    n < 2^26 + 64 has more than 256 divisors) and the reciprocal and the
+    product are within 2^-22 relative of the true values (1 ulp each)."""
+    import numpy as np
+    lim = (1 << 26) + 64
+    primes = [3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47]
+    best = [1]
+
+    def dfs(i, n, d, emax):
+        best[0] = max(best[0], d)
+        if i == len(primes):
+            return
+        m, e = n, 0
+        while e < emax and m * primes[i] < lim:
+            m *= primes[i]
+            e += 1
+            dfs(i + 1, m, d * (e + 1), e)
+
+    dfs(0, 1, 1, 64)
+    assert best[0] == 256
+    emax = 16   # 3^17 > 2^26 + 64
+    for ea in range(emax + 1):
+        for eb in range(emax + 1):
+            den = (ea + 1) * (eb + 1)
+            q = np.arange(1, (1 << 16) // den + 1, dtype=np.float64)
+            d = q * den
+            for rel in (-2.0**-22, 2.0**-22):
+                approx = np.rint(d * (1.0 / den) * (1.0 + rel))
+                assert np.array_equal(approx, q), (ea, eb)
