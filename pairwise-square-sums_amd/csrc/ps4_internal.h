@@ -81,7 +81,7 @@ typedef struct {
     uint64_t t64[64];
     uint64_t t63[63];
     uint64_t t65[65][2];
-    uint32_t t11[11], t13[13], t17[17], t19[19];
+    uint32_t t11[11], t17[17], t19[19];
 } ps4_qrtab;
 
 #endif

@@ -19,7 +19,7 @@
 //        LDS arena and flattens all their divisor pairs over the lanes.  Each
 //        candidate is filtered by per-4-set quadratic-residue masks (mod 64, 63,
 //        65: the ext_sqrt rule of :192-198), by the exact e>0 / e not in the set
-//        conditions, then by residue masks mod 11, 13, 17, 19 and a size bound;
+//        conditions, then by residue masks mod 11, 17, 19 and a size bound;
 //        the few survivors get the exact 128-bit square tests (:199-200).
 //
 // All integer work is exact: uint64, doubles only where every value is an
@@ -833,7 +833,7 @@ struct ExArgs {
 //         exactly for t < 8192 and nL < 1024 (checked for every pair on the host)
 //   q1 = {mask mod 64 (lo, hi),  mask mod 63 (lo, hi)}
 //   q2 = {mask mod 65 bits 0..63 (lo, hi),  msm (lo, hi)}: msm bit 0 is bit 64 of
-//         the mod-65 mask, then the masks mod 11 (bit 1), 13 (12), 17 (25), 19 (42);
+//         the mod-65 mask, then the masks mod 11 (bit 1), 17 (25), 19 (42);
 //         bits 61..63 hold the pair choice
 // Read only for queued candidates: s_mem[k] = {isqrt(A), F1, F2, F3} (the roots
 // of the pair sums containing A), s_aux[k] = isqrt(2A) when 2A is a square,
@@ -889,7 +889,6 @@ __device__ __forceinline__ u32 mulhi_u24(u32 a, u32 b)
 __device__ __forceinline__ u32 mod63(u32 s) { return PS4_MAD_I24(mulhi_u24(s << 3, 8521761u), -63, s); }
 __device__ __forceinline__ u32 mod65(u32 s) { return PS4_MAD_I24(mulhi_u24(s << 3, 8259553u), -65, s); }
 __device__ __forceinline__ u32 mod11(u32 s) { return s - 11u * ((s * 47663u) >> 19); }
-__device__ __forceinline__ u32 mod13(u32 s) { return s - 13u * ((s * 40330u) >> 19); }
 __device__ __forceinline__ u32 mod17(u32 s) { return s - 17u * ((s * 30841u) >> 19); }
 __device__ __forceinline__ u32 mod19(u32 s) { return s - 19u * ((s * 55189u) >> 20); }
 
@@ -1137,13 +1136,14 @@ __device__ __forceinline__ bool ex_counted(double u, const uint4 &m, u32 aux)
 }
 
 // The later filters of a counted candidate u of record word q2: the residue
-// masks mod 11, 13, 17, 19 (e + C1 and e + C2 both squares mod each).  True:
-// it needs the exact test.
+// masks mod 11, 17, 19 (e + C1 and e + C2 both squares mod each).  True: it
+// needs the exact test.  (Mod 13 would reject nothing: a square mod 65 is a
+// square mod 13, and every counted candidate passed the mod-65 mask.)
 __device__ __forceinline__ bool ex_late(double u, const uint4 &q2)
 {
     const u32 r2 = mod_m2(u);
-    const u32 m2b = bit64(q2.z, q2.w, 1u + mod11(r2)) & bit64(q2.z, q2.w, 12u + mod13(r2)) &
-                    bit64(q2.z, q2.w, 25u + mod17(r2)) & bit64(q2.z, q2.w, 42u + mod19(r2));
+    const u32 m2b = bit64(q2.z, q2.w, 1u + mod11(r2)) & bit64(q2.z, q2.w, 25u + mod17(r2)) &
+                    bit64(q2.z, q2.w, 42u + mod19(r2));
     return m2b != 0u;
 }
 
@@ -1388,15 +1388,14 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // the clamps change nothing.
                     const u32 r1a = rep_m1(d1) & 0xFFFFFu, r1b = rep_m1(d2) & 0xFFFFFu;
                     const u32 r2a = min(mod_m2(d1), 46188u), r2b = min(mod_m2(d2), 46188u);
-                    const u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(r1a), mod63(r1b), mod65(r1a), mod65(r1b), mod11(r2a),
-                                        mod11(r2b), mod13(r2a), mod13(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
+                    const u32 ix[12] = {r1a & 63u, r1b & 63u, mod63(r1a), mod63(r1b), mod65(r1a), mod65(r1b), mod11(r2a),
+                                        mod11(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
                     // (read here, combined after the walk)
                     const u64 t64a = qr->t64[ix[0]], t64b = qr->t64[ix[1]], t63a = qr->t63[ix[2]],
                               t63b = qr->t63[ix[3]], t65a = qr->t65[ix[4]][0], t65b = qr->t65[ix[5]][0],
                               t65c = qr->t65[ix[4]][1], t65d = qr->t65[ix[5]][1];
-                    const u32 t11a = qr->t11[ix[6]], t11b = qr->t11[ix[7]], t13a = qr->t13[ix[8]], t13b = qr->t13[ix[9]],
-                              t17a = qr->t17[ix[10]], t17b = qr->t17[ix[11]], t19a = qr->t19[ix[12]],
-                              t19b = qr->t19[ix[13]];
+                    const u32 t11a = qr->t11[ix[6]], t11b = qr->t11[ix[7]], t17a = qr->t17[ix[8]], t17b = qr->t17[ix[9]],
+                              t19a = qr->t19[ix[10]], t19b = qr->t19[ix[11]];
                     // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m,
                     // i.e. u is a root of a pair sum containing A, or u^2 = 2A
                     // (every root is below 2^26 for S < 2^50)
@@ -1409,8 +1408,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     pm64 = t64a & t64b;
                     pm63 = t63a & t63b;
                     pm65 = t65a & t65b;
-                    pmsm = (t65c & t65d & 1ull) | ((u64)(t11a & t11b) << 1) | ((u64)(t13a & t13b) << 12) |
-                           ((u64)(t17a & t17b) << 25) | ((u64)(t19a & t19b) << 42);
+                    pmsm = (t65c & t65d & 1ull) | ((u64)(t11a & t11b) << 1) | ((u64)(t17a & t17b) << 25) |
+                           ((u64)(t19a & t19b) << 42);
                     // Divisor pairs f < g of D' = D (D odd, u = (f+g)/2) or D/4
                     // (4 | D, u = f+g); none when D = 2 mod 4.  Their number is the
                     // reference's ext_cand.

@@ -320,7 +320,6 @@ int build_tables(ps4_ctx *ctx)
     for (u32 a = 0; a < 63; ++a) for (u32 r = 0; r < 63; ++r) if (isq((r * r + a) % 63, 63)) qr.t63[a] |= 1ull << r;
     for (u32 a = 0; a < 65; ++a) for (u32 r = 0; r < 65; ++r) if (isq((r * r + a) % 65, 65)) qr.t65[a][r >> 6] |= 1ull << (r & 63);
     for (u32 a = 0; a < 11; ++a) for (u32 r = 0; r < 11; ++r) if (isq((r * r + a) % 11, 11)) qr.t11[a] |= 1u << r;
-    for (u32 a = 0; a < 13; ++a) for (u32 r = 0; r < 13; ++r) if (isq((r * r + a) % 13, 13)) qr.t13[a] |= 1u << r;
     for (u32 a = 0; a < 17; ++a) for (u32 r = 0; r < 17; ++r) if (isq((r * r + a) % 17, 17)) qr.t17[a] |= 1u << r;
     for (u32 a = 0; a < 19; ++a) for (u32 r = 0; r < 19; ++r) if (isq((r * r + a) % 19, 19)) qr.t19[a] |= 1u << r;
     // the extension kernel's odd-factor table: for odd i < n, entry i / 2 holds
