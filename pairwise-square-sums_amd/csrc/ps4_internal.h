@@ -81,7 +81,7 @@ typedef struct {
     uint64_t t64[64];
     uint64_t t63[63];
     uint64_t t65[65][2];
-    uint32_t t11[11], t17[17], t19[19];
+    uint32_t t11[11], t17[17], t19[19], t23[23];   // t23: folded, bit min(r, 23 - r)
 } ps4_qrtab;
 
 #endif

@@ -19,7 +19,7 @@
 //        LDS arena and flattens all their divisor pairs over the lanes.  Each
 //        candidate is filtered by per-4-set quadratic-residue masks (mod 64, 63,
 //        65: the ext_sqrt rule of :192-198), by the exact e>0 / e not in the set
-//        conditions, then by residue masks mod 11, 17, 19 and a size bound;
+//        conditions, then by residue masks mod 11, 17, 19, 23 and a size bound;
 //        the few survivors get the exact 128-bit square tests (:199-200).
 //
 // All integer work is exact: uint64, doubles only where every value is an
@@ -833,7 +833,7 @@ struct ExArgs {
 //         exactly for t < 8192 and nL < 1024 (checked for every pair on the host)
 //   q1 = {mask mod 64 (lo, hi),  mask mod 63 (lo, hi)}
 //   q2 = {mask mod 65 bits 0..63 (lo, hi),  msm (lo, hi)}: msm bit 0 is bit 64 of
-//         the mod-65 mask, then the masks mod 11 (bit 1), 17 (25), 19 (42);
+//         the mod-65 mask, then the masks mod 11 (bit 1), 23 folded (12), 17 (25), 19 (42);
 //         bits 61..63 hold the pair choice
 // Read only for queued candidates: s_mem[k] = {isqrt(A), F1, F2, F3} (the roots
 // of the pair sums containing A), s_aux[k] = isqrt(2A) when 2A is a square,
@@ -881,19 +881,22 @@ __device__ __forceinline__ u32 mulhi_u24(u32 a, u32 b)
 // s mod 63 and s mod 65 for s < 2^20 (a representative of a class mod
 // 262080 = 64*63*65): floor(s / m) = (8 s * ceil(2^29 / m)) >> 32 by the 24-bit
 // multiply-high, exact because the rounding error s * 2^-29 stays below 1/m.
-// s mod 11..19 for s < 46189 by a 32-bit multiply and a logical shift (the
-// products stay below 2^32).  All constants are checked exhaustively on the
-// host.  Plain multiplies, not __umul24: with __umul24 the compiler emitted an
-// arithmetic shift of the product, which is wrong once the product reaches 2^31
-// (the residue then came out near 90000 and indexed far past the residue tables).
+// s mod 11, 17, 19 and 23 for s < 2^17 (a representative of a class mod
+// 81719 = 11*17*19*23) likewise: floor(s / m) = (128 s * ceil(2^25 / m)) >> 32.
+// All constants are checked exhaustively on the host.  The multiply-high is a
+// plain 64-bit product of the masked operands, not __umul24: with __umul24 the
+// compiler emitted an arithmetic shift of the product, which is wrong once the
+// product reaches 2^31 (a residue then came out near 90000 and indexed far past
+// the residue tables).
 __device__ __forceinline__ u32 mod63(u32 s) { return PS4_MAD_I24(mulhi_u24(s << 3, 8521761u), -63, s); }
 __device__ __forceinline__ u32 mod65(u32 s) { return PS4_MAD_I24(mulhi_u24(s << 3, 8259553u), -65, s); }
-__device__ __forceinline__ u32 mod11(u32 s) { return s - 11u * ((s * 47663u) >> 19); }
-__device__ __forceinline__ u32 mod17(u32 s) { return s - 17u * ((s * 30841u) >> 19); }
-__device__ __forceinline__ u32 mod19(u32 s) { return s - 19u * ((s * 55189u) >> 20); }
+__device__ __forceinline__ u32 mod11(u32 s) { return PS4_MAD_I24(mulhi_u24(s << 7, 3050403u), -11, s); }
+__device__ __forceinline__ u32 mod17(u32 s) { return PS4_MAD_I24(mulhi_u24(s << 7, 1973791u), -17, s); }
+__device__ __forceinline__ u32 mod19(u32 s) { return PS4_MAD_I24(mulhi_u24(s << 7, 1766023u), -19, s); }
+__device__ __forceinline__ u32 mod23(u32 s) { return PS4_MAD_I24(mulhi_u24(s << 7, 1458889u), -23, s); }
 
 #define M1 262080.0
-#define M2 46189.0
+#define M2 81719.0   // 11 * 17 * 19 * 23
 
 // x mod M2 in [0, M2) for an integer-valued double |x| < 2^52 (the
 // representative of rep_m1's kind, then two conditional subtractions as
@@ -902,9 +905,9 @@ __device__ __forceinline__ u32 mod_m2(double x)
 {
     const double q = __builtin_floor(x * (1.0 / M2));
     u32 r = (u32)(__builtin_fma(-q, M2, x) + M2);   // [0, 3*M2)
-    r = r < r - 46189u ? r : r - 46189u;
-    r = r < r - 46189u ? r : r - 46189u;
-    return r & 0xFFFFu;   // (no-op: tells the compiler r < 2^16, so 24-bit multiplies suffice)
+    r = r < r - 81719u ? r : r - 81719u;
+    r = r < r - 81719u ? r : r - 81719u;
+    return r & 0x1FFFFu;   // (no-op: tells the compiler r < 2^17, so 24-bit multiplies suffice)
 }
 
 // A representative in [0, 3*M1) of x mod M1 for an integer-valued double
@@ -1136,14 +1139,16 @@ __device__ __forceinline__ bool ex_counted(double u, const uint4 &m, u32 aux)
 }
 
 // The later filters of a counted candidate u of record word q2: the residue
-// masks mod 11, 17, 19 (e + C1 and e + C2 both squares mod each).  True: it
-// needs the exact test.  (Mod 13 would reject nothing: a square mod 65 is a
-// square mod 13, and every counted candidate passed the mod-65 mask.)
+// masks mod 11, 23, 17, 19 (e + C1 and e + C2 both squares mod each; the
+// mask mod 23 folded, as u and -u pass alike: bit min(r, 23 - r)).  True: it
+// needs the exact test.  (No mask mod 13: it would reject nothing, a square mod
+// 65 being a square mod 13, and every counted candidate passed the mod-65 mask.)
 __device__ __forceinline__ bool ex_late(double u, const uint4 &q2)
 {
     const u32 r2 = mod_m2(u);
-    const u32 m2b = bit64(q2.z, q2.w, 1u + mod11(r2)) & bit64(q2.z, q2.w, 25u + mod17(r2)) &
-                    bit64(q2.z, q2.w, 42u + mod19(r2));
+    const u32 r23 = mod23(r2);
+    const u32 m2b = bit64(q2.z, q2.w, 1u + mod11(r2)) & bit64(q2.z, q2.w, 12u + min(r23, 23u - r23)) &
+                    bit64(q2.z, q2.w, 25u + mod17(r2)) & bit64(q2.z, q2.w, 42u + mod19(r2));
     return m2b != 0u;
 }
 
@@ -1379,23 +1384,24 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     // filters of exhaustive/s4.c:192-194)
                     const double d1 = (double)(i64)(P.C1 - P.A), d2 = (double)(i64)(P.C2 - P.A);
                     // representatives mod 262080 (residues mod 64, 63, 65 follow) and
-                    // exact residues mod 46189; |C - A| < 2^51
+                    // exact residues mod 81719; |C - A| < 2^51
                     // The residues index small tables.  The representatives are
                     // clamped to the domains on which every residue below is exact
-                    // and in range (all s < 2^20 for mod 63 and 65, s < 46189 for
-                    // mod 11..19; checked exhaustively on the host), so no index
+                    // and in range (all s < 2^20 for mod 63 and 65, s < 81719 for
+                    // mod 11..23; checked exhaustively on the host), so no index
                     // can leave its table, whatever the input; for |C - A| < 2^51
                     // the clamps change nothing.
                     const u32 r1a = rep_m1(d1) & 0xFFFFFu, r1b = rep_m1(d2) & 0xFFFFFu;
-                    const u32 r2a = min(mod_m2(d1), 46188u), r2b = min(mod_m2(d2), 46188u);
-                    const u32 ix[12] = {r1a & 63u, r1b & 63u, mod63(r1a), mod63(r1b), mod65(r1a), mod65(r1b), mod11(r2a),
-                                        mod11(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b)};
+                    const u32 r2a = min(mod_m2(d1), 81718u), r2b = min(mod_m2(d2), 81718u);
+                    const u32 ix[14] = {r1a & 63u, r1b & 63u, mod63(r1a), mod63(r1b), mod65(r1a), mod65(r1b), mod11(r2a),
+                                        mod11(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b), mod23(r2a), mod23(r2b)};
                     // (read here, combined after the walk)
                     const u64 t64a = qr->t64[ix[0]], t64b = qr->t64[ix[1]], t63a = qr->t63[ix[2]],
                               t63b = qr->t63[ix[3]], t65a = qr->t65[ix[4]][0], t65b = qr->t65[ix[5]][0],
                               t65c = qr->t65[ix[4]][1], t65d = qr->t65[ix[5]][1];
                     const u32 t11a = qr->t11[ix[6]], t11b = qr->t11[ix[7]], t17a = qr->t17[ix[8]], t17b = qr->t17[ix[9]],
-                              t19a = qr->t19[ix[10]], t19b = qr->t19[ix[11]];
+                              t19a = qr->t19[ix[10]], t19b = qr->t19[ix[11]], t23a = qr->t23[ix[12]],
+                              t23b = qr->t23[ix[13]];
                     // e = u^2 - A > 0 <=> u > isqrt(A);  e = v_m <=> u^2 = A + v_m,
                     // i.e. u is a root of a pair sum containing A, or u^2 = 2A
                     // (every root is below 2^26 for S < 2^50)
@@ -1408,8 +1414,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     pm64 = t64a & t64b;
                     pm63 = t63a & t63b;
                     pm65 = t65a & t65b;
-                    pmsm = (t65c & t65d & 1ull) | ((u64)(t11a & t11b) << 1) | ((u64)(t17a & t17b) << 25) |
-                           ((u64)(t19a & t19b) << 42);
+                    pmsm = (t65c & t65d & 1ull) | ((u64)(t11a & t11b) << 1) | ((u64)(t23a & t23b) << 12) |
+                           ((u64)(t17a & t17b) << 25) | ((u64)(t19a & t19b) << 42);
                     // Divisor pairs f < g of D' = D (D odd, u = (f+g)/2) or D/4
                     // (4 | D, u = f+g); none when D = 2 mod 4.  Their number is the
                     // reference's ext_cand.

@@ -322,6 +322,7 @@ int build_tables(ps4_ctx *ctx)
     for (u32 a = 0; a < 11; ++a) for (u32 r = 0; r < 11; ++r) if (isq((r * r + a) % 11, 11)) qr.t11[a] |= 1u << r;
     for (u32 a = 0; a < 17; ++a) for (u32 r = 0; r < 17; ++r) if (isq((r * r + a) % 17, 17)) qr.t17[a] |= 1u << r;
     for (u32 a = 0; a < 19; ++a) for (u32 r = 0; r < 19; ++r) if (isq((r * r + a) % 19, 19)) qr.t19[a] |= 1u << r;
+    for (u32 a = 0; a < 23; ++a) for (u32 r = 0; r < 12; ++r) if (isq((r * r + a) % 23, 23)) qr.t23[a] |= 1u << r;
     // the extension kernel's odd-factor table: for odd i < n, entry i / 2 holds
     // the distinct prime powers of i as p << 6 | e, ascending (at most 7 for
     // i < 2^26), then k | d(i) << 4 with k their number (one gather factors an

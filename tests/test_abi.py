@@ -73,23 +73,36 @@ def test_tuning_file_round_trip_and_errors():
     assert ps4.parse_tuning("ext_chunk = 1")["ext_chunk"] == 1
 
 
-@pytest.mark.parametrize("name,m,shift,lim", [("mod11", 11, 19, 46189), ("mod17", 17, 19, 46189),
-                                              ("mod19", 19, 20, 46189)])
-def test_kernel_residue_constants_are_exact(name, m, shift, lim):
-    """The multiply-shift divisions in the extension kernel equal true division
-    over their whole input range (inputs are residues below lim)."""
+@pytest.mark.parametrize("name,m", [("mod11", 11), ("mod17", 17), ("mod19", 19), ("mod23", 23)])
+def test_kernel_residue_constants_are_exact(name, m):
+    """s mod 11, 17, 19, 23 by the 24-bit multiply-high in the extension
+    kernel, for every s < 2^17 (its representatives of a class mod
+    81719 = 11*17*19*23 are below 81719)."""
+    import numpy as np
     src = open(KERNELS).read()
-    mt = re.search(name + r"\(u32 s\) \{ return s - (\d+)u \* \(\(s \* (\d+)u\) >> (\d+)\); \}", src)
+    mt = re.search(name + r"\(u32 s\) \{ return PS4_MAD_I24\(mulhi_u24\(s << 7, (\d+)u\), -(\d+), s\); \}", src)
     assert mt, name
-    mm, magic, sh = int(mt.group(1)), int(mt.group(2)), int(mt.group(3))
-    assert mm == m and sh == shift and magic < 2**24
-    for s in range(lim):
-        assert s * magic < 2**32
-        assert (s * magic) >> sh == s // m
+    magic, mm = int(mt.group(1)), int(mt.group(2))
+    assert mm == m and magic < 2**24
+    s = np.arange(1 << 17, dtype=np.uint64)
+    assert int(s[-1] << np.uint64(7)) < 2**24                         # the first factor fits 24 bits
+    q = ((s << np.uint64(7)) * np.uint64(magic)) >> np.uint64(32)
+    assert np.array_equal(s - np.uint64(m) * q, s % np.uint64(m))
+    assert 11 * 17 * 19 * 23 == 81719 and "#define M2 81719.0" in src
     # the kernel clamps its representatives to the domains checked here, so no
     # residue can index past its table whatever the input
-    assert "min(mod_m2(d1), 46188u)" in src and "min(mod_m2(d2), 46188u)" in src
+    assert "min(mod_m2(d1), 81718u)" in src and "min(mod_m2(d2), 81718u)" in src
     assert "rep_m1(d1) & 0xFFFFFu" in src and "rep_m1(d2) & 0xFFFFFu" in src
+
+
+def test_folded_mask_mod_23_is_exact():
+    """The late mask mod 23 keeps 12 bits, bit min(r, 23 - r) for u = r mod 23:
+    u^2 + c is a square mod 23 exactly when (23 - u)^2 + c is."""
+    sq = {r * r % 23 for r in range(23)}
+    for c in range(23):
+        full = [(r * r + c) % 23 in sq for r in range(23)]
+        assert all(full[r] == full[min(r, 23 - r)] for r in range(23))
+    assert 12 + 12 <= 25   # bits 12..23 of the late word, below the mask mod 17 at bit 25
 
 
 @pytest.mark.parametrize("name,m", [("mod63", 63), ("mod65", 65)])
