@@ -66,7 +66,7 @@ PEAK_LANE_OPS = 256 * 64 * 2.4e9      # 256 CUs x 64 lanes x 2.4 GHz, 32-bit int
 # HBM bytes per k_ext launch, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the
 # [1,1e10) census (3 launches: their average), each counter scaled by what it reads for a 2 GiB device copy
 # (profiles/census_1e10_counters.md); reported for that workload only
-K_EXT_TRAFFIC = {(1, 10**10): 32.83e9}
+K_EXT_TRAFFIC = {(1, 10**10): 27.59e9}
 GOLDEN = os.path.join(REPO, "tests", "golden", "golden.json")
 TUNE = None   # tuning file (--tune): launch shape and batch plan of every engine opened here
 GRAPH = False  # --graph: every batch launched as one captured hipGraph
