@@ -783,7 +783,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 5 : 1) void k_rq(RqArgs a)
 
 #define EX_NT 64          // one wave per workgroup: every barrier below is a wave's own
 #define EX_ARENA 861      // doubles of divisor lists per pass, + 3 spare (the workgroup's LDS
-                          // stays at 13568 bytes, so 12 workgroups share a CU's 160 KB); one
+                          // stays at 13616 bytes, so 12 workgroups share a CU's 160 KB); one
                           // 4-set with d(D) <= 16384 needs at most 748 (nL + nH over the best
                           // prefix split, reached by the radices 2, 11, 33, 2, 11)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
@@ -1204,7 +1204,8 @@ template <int EX_CHUNK>
 __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIMD: <= 168 VGPRs
 {
     __shared__ double s_ar[EX_ARENA + 3];      // L and H lists, exact doubles (+ 3 spare: see the list building)
-    __shared__ uint4 s_par[EX_NT * EX_PAR_WORDS];   // parameters of the 4-sets of the current pass
+    __shared__ uint4 s_par[(EX_NT + 1) * EX_PAR_WORDS];   // parameters of the 4-sets of the current pass,
+                                                          // and the guard record after them
     __shared__ uint4 s_mem[EX_NT];             // their e > 0 and member bounds
     __shared__ u32 s_aux[EX_NT];               // isqrt(2A) or 0
     __shared__ u32 s_qix[EX_NT];               // their queue indices
@@ -1517,6 +1518,18 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
             // first candidate in the pass
             for (u32 w = tid; w < (ctot + 127) / 128 * 4; w += EX_NT) s_bits[w] = 0u;   // whole 128-bit words
             __syncthreads();
+            // The guard record after the pass's records: the candidates of the last
+            // iteration past ctot (fewer than 128) fall to it, so the loop needs no
+            // range tests.  Its masks are 0 (no candidate passes) and its lists are
+            // the first 256 entries of the arena read as L = {x}, H of 256 entries
+            // (nL = 1: j = t, i = 0), so every read stays inside the arena.
+            if (tid == 0 && ctot > 0u) {
+                uint4 *pw = s_par + (u32)__popcll(fmask) * EX_PAR_WORDS;
+                pw[0] = make_uint4(2040u << 16, 2040u << 16, 0x800000u, ((0u - 8u) & 0xFFFFu) | (ctot << 16));
+                pw[1] = make_uint4(0u, 0u, 0u, 0u);
+                pw[2] = make_uint4(0u, 0u, 0u, 0u);
+                atomicOr(&s_bits[(ctot - 1u) >> 5], 1u << ((ctot - 1u) & 31u));
+            }
                 if (fits) {
                     uint4 *pw = s_par + slot * EX_PAR_WORDS;
                     const u32 aL = 8u * loff, aH = 8u * (loff + nL);
@@ -1601,10 +1614,11 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 u32 qn1 = 0;    // entries in the first queue
                 // the records of candidates base + tid and base + 64 + tid from the
                 // bitmap words of base: every 4-set but the first marks the candidate
-                // before its first, so the record of candidate g is the number of
-                // marks below g (a ballot-style prefix count, v_mbcnt); past the end
-                // of the pass a lane evaluates record 0 at its first candidate (in
-                // range) and discards it
+                // before its first, and so does the guard record, so the record of
+                // candidate g is the number of marks below g (a ballot-style prefix
+                // count, v_mbcnt), the guard for g >= ctot; the index is clamped to
+                // the guard's slot for the fetch past the last iteration (bitmap
+                // words there are stale, and what it reads is not used)
                 u32 ka, kb, ia, ib;       // ia, ib: byte offsets of the records read (of ka, kb or 0)
                 uint4 A0, A1, A2, B0, B1, B2;   // their words
                 auto fetch = [&](u32 base, const uint4 &wv) {
@@ -1614,8 +1628,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     kb = kp + lanes_below(wb);
                     kp += (u32)__popcll(wb);
                     // byte offsets of the records (record indices are below 64)
-                    ia = PS4_MAD_U24(base + tid < ctot ? ka : 0u, EX_PAR_WORDS * 16u, 0u);
-                    ib = PS4_MAD_U24(base + 64u + tid < ctot ? kb : 0u, EX_PAR_WORDS * 16u, 0u);
+                    ia = PS4_MAD_U24(min(ka, (u32)EX_NT), EX_PAR_WORDS * 16u, 0u);
+                    ib = PS4_MAD_U24(min(kb, (u32)EX_NT), EX_PAR_WORDS * 16u, 0u);
                     A0 = *(const uint4 *)((const char *)s_par + ia);
                     B0 = *(const uint4 *)((const char *)s_par + ib);
                 };
@@ -1632,9 +1646,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 uint4 wnext = s_bits4[1];
                 for (u32 base = 0; base < ctot; base += 128) {
                     const u32 ga = base + tid, gb = ga + 64u;
-                    const bool va = ga < ctot, vb = gb < ctot;
                     const u32 ka0 = ka, kb0 = kb;
-                    const ExLists LA = ex_lists(A0, s_ar, va ? ga : 0u), LB = ex_lists(B0, s_ar, vb ? gb : 0u);
+                    const ExLists LA = ex_lists(A0, s_ar, ga), LB = ex_lists(B0, s_ar, gb);
                     __builtin_amdgcn_sched_barrier(0);   // both candidates' list reads issued before any use
                     fetch(base + 128u, wnext);
                     wnext = s_bits4[min((base + 256u) >> 7, EX_CMAX / 128u - 1u)];
@@ -1643,7 +1656,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     const ExCand A = ex_cand(A1, A2, LA, a.neg_m1);
                     const ExCand B = ex_cand(B1, B2, LB, a.neg_m1);
                     fetch_masks();
-                    const bool oka = va & A.ok, okb = vb & B.ok;
+                    const bool oka = A.ok, okb = B.ok;   // (the guard record's candidates never pass)
                     my_esq += (oka ? 1u : 0u) + (okb ? 1u : 0u);
                     // candidates past the residue filters wait on an LDS stack (their
                     // order does not matter) for the rest of the ext_sqrt tests and the
