@@ -75,12 +75,14 @@ typedef struct {
     uint64_t pad;
 } ps4_ctl;
 
-/* QR shift tables, one 64-bit word per offset a (65 needs a second word):
- * bit r of T_m[a] is set iff (r*r + a) mod m is a square mod m. */
+/* QR shift tables, one 64-bit word per offset a: bit r of T_m[a] is set iff
+ * (r*r + a) mod m is a square mod m.  T_65 is indexed by b = (u + 32) mod 65
+ * taken mod 64: bit b is the test of u = b - 32, and b = 0 also stands for
+ * u = 32, whose square is that of u = -32. */
 typedef struct {
     uint64_t t64[64];
     uint64_t t63[63];
-    uint64_t t65[65][2];
+    uint64_t t65[65];
     uint32_t t11[11], t17[17], t19[19], t23[23];   // t23: folded, bit min(r, 23 - r)
 } ps4_qrtab;
 

@@ -99,6 +99,20 @@ __device__ __forceinline__ bool d_is_square128(u128 n)
 #ifndef PS4_KEEP
 #define PS4_KEEP(...) asm volatile("" ::__VA_ARGS__)
 #endif
+// Makes a register value opaque (an empty asm statement that may have changed
+// it), so that every use starts from the same compare.
+#ifndef PS4_OPAQUE
+#define PS4_OPAQUE(x) asm volatile("" : "+v"(x))
+#endif
+// popcount(x) + a in one instruction (the compiler adds separately)
+#ifndef PS4_BCNT_ADD
+#define PS4_BCNT_ADD(x, a)                                                                  \
+    ({                                                                                    \
+        u32 r_;                                                                           \
+        asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r_) : "v"((u32)(x)), "v"((u32)(a)));       \
+        r_;                                                                               \
+    })
+#endif
 
 // Two 16-bit halves of a word at once: the halfwise maximum of a and b, and
 // |a.lo - b.lo| + |a.hi - b.hi| + c.
@@ -832,8 +846,8 @@ struct ExArgs {
 //         pass; magic = ceil(2^23 / nL) gives t / nL = (t << 9) * magic >> 32
 //         exactly for t < 8192 and nL < 1024 (checked for every pair on the host)
 //   q1 = {mask mod 64 (lo, hi),  mask mod 63 (lo, hi)}
-//   q2 = {mask mod 65 bits 0..63 (lo, hi),  msm (lo, hi)}: msm bit 0 is bit 64 of
-//         the mod-65 mask, then the masks mod 11 (bit 1), 23 folded (12), 17 (25), 19 (42);
+//   q2 = {mask mod 65, bit (u + 32) mod 65 mod 64 (lo, hi),  msm (lo, hi)}: msm
+//         holds the masks mod 11 (bit 1), 23 folded (12), 17 (25), 19 (42);
 //         bits 61..63 hold the pair choice
 // Read only for queued candidates: s_mem[k] = {isqrt(A), F1, F2, F3} (the roots
 // of the pair sums containing A), s_aux[k] = isqrt(2A) when 2A is a square,
@@ -1075,9 +1089,21 @@ __device__ __forceinline__ bool ex_pos_live(int s, u32 n1, u32 n2)
 // get the remaining tests of the ext_sqrt count (e > 0, e not a member) and
 // the later filters (ex_drain1).
 struct ExCand {
-    double u;
+    double us;   // u + EX_UOFF
     bool ok;
 };
+
+// The candidate loop carries u + EX_UOFF: 64512 = 16 * 64 * 63 is 0 mod 64 and
+// mod 63 and 32 mod 65, so its residue r65 mod 65 is (u + 32) mod 65, whose
+// values 64 and 0 (u = 32 and u = -32) have the same square: the mod-65 test
+// is one 64-bit shift by r65 mod 64.  The offset rides in the first product's
+// fused add, and the stack entries hold u + EX_UOFF.
+#define EX_UOFF 64512.0
+// 1/262080 lowered by 2^-45 relative (with the two roundings of this
+// expression, by 2^-45 +- 2^-52): for an integer 0 <= x < 2^53 the rounded
+// product x * EX_INV_M1 lies below x / 262080 and within 2^-9 of it, so its
+// floor q is floor(x / 262080) or one less, and x - 262080 q is in [0, 2 * 262080).
+#define EX_INV_M1 ((1.0 / 262080.0) * (1.0 - 0x1p-45))
 
 // The four list entries of candidate t: L[i], H[j], L[nL-1-i], H[nH-1-j].
 struct ExLists {
@@ -1107,26 +1133,30 @@ __device__ __forceinline__ ExLists ex_lists(const uint4 &q0, const double *s_ar,
 __device__ __forceinline__ ExCand ex_cand(const uint4 &q1, const uint4 &q2, const ExLists &Ls, double neg_m1)
 {
     ExCand c;
-    c.u = __builtin_fma(Ls.l1, Ls.h1, Ls.l0 * Ls.h0);   // H holds halves when u = (f+g)/2
-    const double q = __builtin_floor(c.u * (1.0 / M1));
-    const u32 r1 = (u32)(__builtin_fma(q, neg_m1, c.u) + M1);   // rep_m1(c.u)
+    c.us = __builtin_fma(Ls.l1, Ls.h1, __builtin_fma(Ls.l0, Ls.h0, EX_UOFF));   // H holds halves when u = (f+g)/2
+    const double q = __builtin_floor(c.us * EX_INV_M1);
+    const u32 r1 = (u32)__builtin_fma(q, neg_m1, c.us);   // (u + EX_UOFF) mod 262080, + 0 or 262080: below 2^20
     const u32 r63 = mod63(r1), r65 = mod65(r1);
-    const u32 qrb = bit64(q1.x, q1.y, r1 & 63u) & bit64(q1.z, q1.w, r63) &
-                    (r65 < 64u ? bit64(q2.x, q2.y, r65) : (q2.z & 1u));
+    u32 qrb = bit64(q1.x, q1.y, r1 & 63u) & bit64(q1.z, q1.w, r63) & bit64(q2.x, q2.y, r65 & 63u);
+    PS4_OPAQUE(qrb);   // (else the ballot and the branch on c.ok get a test each)
     c.ok = qrb != 0u;
     return c;
 }
 
-// Queue entries: an integer-valued u in [0, 2^52) and the record index k in
-// the top bits.  u + 2^52 has u as its mantissa (exact below 2^53).
-__device__ __forceinline__ u64 ex_qent(double u, u32 k)
+// Stack entries: an integer-valued us = u + EX_UOFF in [0, 2^52) as the
+// mantissa of us + 2^52 (exact below 2^53), whose exponent field 0x433 gets
+// the record index k < 64 added (one add to the high word).
+__device__ __forceinline__ u64 ex_qent(double us, u32 k)
 {
-    const u64 b = __builtin_bit_cast(u64, u + 4503599627370496.0);
-    return (b & ((1ull << 52) - 1ull)) | ((u64)k << 52);
+    const u64 b = __builtin_bit_cast(u64, us + 4503599627370496.0);
+    return ((u64)((u32)(b >> 32) + (k << 20)) << 32) | (u32)b;
 }
+__device__ __forceinline__ u32 ex_qk(u64 e) { return (u32)(e >> 52) - 0x433u; }
+// u from the low 52 bits of an entry (us = u + EX_UOFF)
 __device__ __forceinline__ double ex_qu(u64 e)
 {
-    return __builtin_bit_cast(double, (e & ((1ull << 52) - 1ull)) | 0x4330000000000000ull) - 4503599627370496.0;
+    return __builtin_bit_cast(double, (e & ((1ull << 52) - 1ull)) | 0x4330000000000000ull) -
+           (4503599627370496.0 + EX_UOFF);
 }
 
 // The rest of the ext_sqrt tests for a queued candidate u of 4-set record k:
@@ -1398,8 +1428,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                                         mod11(r2b), mod17(r2a), mod17(r2b), mod19(r2a), mod19(r2b), mod23(r2a), mod23(r2b)};
                     // (read here, combined after the walk)
                     const u64 t64a = qr->t64[ix[0]], t64b = qr->t64[ix[1]], t63a = qr->t63[ix[2]],
-                              t63b = qr->t63[ix[3]], t65a = qr->t65[ix[4]][0], t65b = qr->t65[ix[5]][0],
-                              t65c = qr->t65[ix[4]][1], t65d = qr->t65[ix[5]][1];
+                              t63b = qr->t63[ix[3]], t65a = qr->t65[ix[4]], t65b = qr->t65[ix[5]];
                     const u32 t11a = qr->t11[ix[6]], t11b = qr->t11[ix[7]], t17a = qr->t17[ix[8]], t17b = qr->t17[ix[9]],
                               t19a = qr->t19[ix[10]], t19b = qr->t19[ix[11]], t23a = qr->t23[ix[12]],
                               t23b = qr->t23[ix[13]];
@@ -1415,7 +1444,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     pm64 = t64a & t64b;
                     pm63 = t63a & t63b;
                     pm65 = t65a & t65b;
-                    pmsm = (t65c & t65d & 1ull) | ((u64)(t11a & t11b) << 1) | ((u64)(t23a & t23b) << 12) |
+                    pmsm = ((u64)(t11a & t11b) << 1) | ((u64)(t23a & t23b) << 12) |
                            ((u64)(t17a & t17b) << 25) | ((u64)(t19a & t19b) << 42);
                     // Divisor pairs f < g of D' = D (D odd, u = (f+g)/2) or D/4
                     // (4 | D, u = f+g); none when D = 2 mod 4.  Their number is the
@@ -1513,7 +1542,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
             (void)wave_excl_scan(fits ? ncand : 0u, &ctot);
             // the stage positions the fitting 4-sets use (uniform)
             const u32 p1max = wave_max7(fits ? (o1n >> 28) : 0u), p2max = wave_max7(fits ? (o2n >> 28) : 0u);
-            u32 my_esq = 0, my_unc = 0;   // ext_sqrt: residue-filter passes, minus the queued ones that fail the rest
+            u32 w_esq = 0, my_unc = 0;   // ext_sqrt: residue-filter passes (the wave's, from its ballots), minus
+                                         // the queued ones that fail the rest (per lane)
             // fitting 4-sets are a prefix of the pending ones: coff is each one's
             // first candidate in the pass
             for (u32 w = tid; w < (ctot + 127) / 128 * 4; w += EX_NT) s_bits[w] = 0u;   // whole 128-bit words
@@ -1622,11 +1652,11 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 u32 ka, kb, ia, ib;       // ia, ib: byte offsets of the records read (of ka, kb or 0)
                 uint4 A0, A1, A2, B0, B1, B2;   // their words
                 auto fetch = [&](u32 base, const uint4 &wv) {
-                    const u64 wa = (u64)wv.x | ((u64)wv.y << 32), wb = (u64)wv.z | ((u64)wv.w << 32);
-                    ka = kp + lanes_below(wa);
-                    kp += (u32)__popcll(wa);
-                    kb = kp + lanes_below(wb);
-                    kp += (u32)__popcll(wb);
+                    // (prefix and population counts with kp as their addend)
+                    ka = __builtin_amdgcn_mbcnt_hi(wv.y, __builtin_amdgcn_mbcnt_lo(wv.x, kp));
+                    kp = PS4_BCNT_ADD(wv.y, PS4_BCNT_ADD(wv.x, kp));
+                    kb = __builtin_amdgcn_mbcnt_hi(wv.w, __builtin_amdgcn_mbcnt_lo(wv.z, kp));
+                    kp = PS4_BCNT_ADD(wv.w, PS4_BCNT_ADD(wv.z, kp));
                     // byte offsets of the records (record indices are below 64)
                     ia = PS4_MAD_U24(min(ka, (u32)EX_NT), EX_PAR_WORDS * 16u, 0u);
                     ib = PS4_MAD_U24(min(kb, (u32)EX_NT), EX_PAR_WORDS * 16u, 0u);
@@ -1657,7 +1687,6 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     const ExCand B = ex_cand(B1, B2, LB, a.neg_m1);
                     fetch_masks();
                     const bool oka = A.ok, okb = B.ok;   // (the guard record's candidates never pass)
-                    my_esq += (oka ? 1u : 0u) + (okb ? 1u : 0u);
                     // candidates past the residue filters wait on an LDS stack (their
                     // order does not matter) for the rest of the ext_sqrt tests and the
                     // later filters, which run 64 at a time with every lane busy; the
@@ -1673,7 +1702,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                         u32 aux = 0;
                         if (tid < m) {
                             ent = s_q1[b1 + tid];
-                            const u32 k = (u32)(ent >> 52);
+                            const u32 k = ex_qk(ent);
                             const double u = ex_qu(ent);
                             aux = s_aux[k];
                             if (ex_counted(u, s_mem[k], aux)) {
@@ -1697,18 +1726,20 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     };
                     // (the stack holds < 64 entries before each push of at most 64)
                     const u64 boa = __ballot(oka);
-                    if (oka) s_q1[qn1 + lanes_below(boa)] = ex_qent(A.u, ka0);
+                    if (oka) s_q1[qn1 + lanes_below(boa)] = ex_qent(A.us, ka0);
                     qn1 += (u32)__popcll(boa);
+                    w_esq += (u32)__popcll(boa);
                     if (qn1 >= 64u) round1();
                     const u64 bob = __ballot(okb);
-                    if (okb) s_q1[qn1 + lanes_below(bob)] = ex_qent(B.u, kb0);
+                    if (okb) s_q1[qn1 + lanes_below(bob)] = ex_qent(B.us, kb0);
                     qn1 += (u32)__popcll(bob);
+                    w_esq += (u32)__popcll(bob);
                     while (qn1 >= 64u || (base + 128u >= ctot && qn1 > 0u))   // (the pass's records
                         round1();                                              //  are about to be replaced)
                 }
             __syncthreads();
             if (fits || drop) has = false;
-            acc_esq += my_esq;
+            acc_esq += tid == 0 ? w_esq : 0u;   // (w_esq < 2^14: at most the pass's 8192 candidates)
             acc_esq -= my_unc;   // lane sums may wrap; the wave's sum is exact
         }
     }

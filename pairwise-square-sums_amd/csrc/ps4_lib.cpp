@@ -318,7 +318,8 @@ int build_tables(ps4_ctx *ctx)
     auto isq = [](u32 x, u32 m) { for (u32 r = 0; r < m; ++r) if ((r * r) % m == x) return true; return false; };
     for (u32 a = 0; a < 64; ++a) for (u32 r = 0; r < 64; ++r) if (isq((r * r + a) % 64, 64)) qr.t64[a] |= 1ull << r;
     for (u32 a = 0; a < 63; ++a) for (u32 r = 0; r < 63; ++r) if (isq((r * r + a) % 63, 63)) qr.t63[a] |= 1ull << r;
-    for (u32 a = 0; a < 65; ++a) for (u32 r = 0; r < 65; ++r) if (isq((r * r + a) % 65, 65)) qr.t65[a][r >> 6] |= 1ull << (r & 63);
+    for (u32 a = 0; a < 65; ++a)
+        for (u32 b = 0; b < 64; ++b) { const u32 r = (b + 33) % 65; if (isq((r * r + a) % 65, 65)) qr.t65[a] |= 1ull << b; }
     for (u32 a = 0; a < 11; ++a) for (u32 r = 0; r < 11; ++r) if (isq((r * r + a) % 11, 11)) qr.t11[a] |= 1u << r;
     for (u32 a = 0; a < 17; ++a) for (u32 r = 0; r < 17; ++r) if (isq((r * r + a) % 17, 17)) qr.t17[a] |= 1u << r;
     for (u32 a = 0; a < 19; ++a) for (u32 r = 0; r < 19; ++r) if (isq((r * r + a) % 19, 19)) qr.t19[a] |= 1u << r;

@@ -105,11 +105,52 @@ def test_folded_mask_mod_23_is_exact():
     assert 12 + 12 <= 25   # bits 12..23 of the late word, below the mask mod 17 at bit 25
 
 
+def test_shifted_mask_mod_65_is_exact():
+    """The candidate loop tests u + 64512, whose residue b = (u + 64512) mod 65
+    is taken mod 64 as the mask's bit, and the host's table gives bit b the
+    test of r = (b + 33) mod 65.  That is exact when r^2 = u^2 mod 65 for every
+    u, and the offset leaves the residues mod 64 and 63 alone."""
+    src = open(KERNELS).read()
+    lib = open(os.path.join(ps4.PKG_DIR, "csrc", "ps4_lib.cpp")).read()
+    off = int(float(re.search(r"#define EX_UOFF ([\d.]+)", src).group(1)))
+    rot = int(re.search(r"const u32 r = \(b \+ (\d+)\) % 65; if \(isq\(\(r \* r \+ a\) % 65, 65\)\) qr\.t65\[a\]", lib).group(1))
+    assert off % 64 == 0 and off % 63 == 0
+    for u in range(65):
+        b = (u + off) % 65 % 64
+        r = (b + rot) % 65
+        assert (r * r - u * u) % 65 == 0, u
+    assert "bit64(q2.x, q2.y, r65 & 63u)" in src
+
+
+def test_candidate_representative_mod_262080_is_in_range():
+    """x - 262080 * floor(x * EX_INV_M1) lies in [0, 2 * 262080) for integers
+    0 <= x < 2^53, with EX_INV_M1 = (1/262080) (1 - 2^-45) in doubles: it is
+    below 1/262080 by more than the product's rounding, and by little enough
+    that the floor is at most one below.  Checked at the bound's extremes:
+    multiples of 262080 and their neighbours, up to 2^53."""
+    from fractions import Fraction
+    import numpy as np
+    src = open(KERNELS).read()
+    assert "#define EX_INV_M1 ((1.0 / 262080.0) * (1.0 - 0x1p-45))" in src
+    m = 262080
+    inv = (1.0 / 262080.0) * (1.0 - 2.0**-45)
+    dev = 1 - Fraction(inv) * m                      # relative deficit of inv
+    assert Fraction(2**-45) - Fraction(2**-52) <= dev <= Fraction(2**-45) + Fraction(2**-52)
+    rng = random.Random(5)
+    ks = [0, 1, 2, 3] + [rng.randrange(1, 2**53 // m) for _ in range(20000)] + [2**53 // m - j for j in range(1, 4)]
+    xs = np.array(sorted({k * m + d for k in ks for d in (-2, -1, 0, 1, 2) if 0 <= k * m + d < 2**53}), dtype=np.int64)
+    q = np.floor(xs.astype(np.float64) * inv).astype(np.int64)
+    r = xs - q * m
+    assert r.min() >= 0 and r.max() < 2 * m
+    assert 2 * m < (1 << 20)   # the domain of the mod 63 and mod 65 multiply-highs
+
+
 @pytest.mark.parametrize("name,m", [("mod63", 63), ("mod65", 65)])
 def test_kernel_residues_mod_63_65_are_exact(name, m):
     """s mod 63 and s mod 65 by the 24-bit multiply-high, for every s < 2^20 (the
     representatives of a class mod 262080 that the extension kernel forms are
-    below 3 * 262080 < 2^20)."""
+    below 3 * 262080 < 2^20 in the set-up and below 2 * 262080 in the
+    candidate loop)."""
     import numpy as np
     src = open(KERNELS).read()
     mt = re.search(name + r"\(u32 s\) \{ return PS4_MAD_I24\(mulhi_u24\(s << 3, (\d+)u\), -(\d+), s\); \}", src)
