@@ -104,6 +104,15 @@ __device__ __forceinline__ bool d_is_square128(u128 n)
 #ifndef PS4_OPAQUE
 #define PS4_OPAQUE(x) asm volatile("" : "+v"(x))
 #endif
+// (a << 3) + b for a uniform b, in one instruction (the compiler splits it)
+#ifndef PS4_LSHL3_ADD
+#define PS4_LSHL3_ADD(a, b)                                                               \
+    ({                                                                                    \
+        u32 r_;                                                                           \
+        asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(r_) : "v"((u32)(a)), "s"((u32)(b)));    \
+        r_;                                                                               \
+    })
+#endif
 // popcount(x) + a in one instruction (the compiler adds separately)
 #ifndef PS4_BCNT_ADD
 #define PS4_BCNT_ADD(x, a)                                                                  \
@@ -797,7 +806,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 5 : 1) void k_rq(RqArgs a)
 
 #define EX_NT 64          // one wave per workgroup: every barrier below is a wave's own
 #define EX_ARENA 861      // doubles of divisor lists per pass, + 3 spare (the workgroup's LDS
-                          // stays at 13616 bytes, so 12 workgroups share a CU's 160 KB); one
+                          // stays at 13632 bytes, so 12 workgroups share a CU's 160 KB); one
                           // 4-set with d(D) <= 16384 needs at most 748 (nL + nH over the best
                           // prefix split, reached by the radices 2, 11, 33, 2, 11)
 #define EX_CMAX 8192      // candidates per pass; one 4-set with d(D) <= 16384 has <= 8192
@@ -840,7 +849,7 @@ struct ExArgs {
 
 // Per-4-set parameters of the candidate loop: one record of three 16-byte
 // words per 4-set of a pass (read together, before any branch):
-//   q0 = {aL | aL2 << 16,  aH | aH2 << 16,  magic,  (-8 nL mod 2^16) | coff << 16}
+//   q0 = {aL | aL2 << 16,  aH | aH2 << 16,  magic,  (-8 nL mod 2^16) | 8 coff << 16}
 //         aL, aL2: byte offsets in the arena of L[0] and L[nL-1]; aH, aH2: of
 //         H[0] and H[nH-1]; coff < 8192: the 4-set's first candidate in the
 //         pass; magic = ceil(2^23 / nL) gives t / nL = (t << 9) * magic >> 32
@@ -1116,13 +1125,14 @@ __device__ __forceinline__ double ar_at(const double *s_ar, u32 off)
     return *(const double *)((const char *)s_ar + off);
 }
 
-// g: the candidate's index in the pass; t = g - coff is its index in its 4-set,
-// i = t mod nL, j = t / nL; q0: the first word of the 4-set's record.
-__device__ __forceinline__ ExLists ex_lists(const uint4 &q0, const double *s_ar, u32 g)
+// g8 = 8 g, g the candidate's index in the pass; t = g - coff is its index in
+// its 4-set, i = t mod nL, j = t / nL; q0: the first word of the 4-set's record
+// (with 8 coff in its top half).
+__device__ __forceinline__ ExLists ex_lists(const uint4 &q0, const double *s_ar, u32 g8)
 {
-    const u32 t = g - (q0.w >> 16);
-    const u32 j = mulhi_u24(t << 9, q0.z);
-    const u32 i8 = PS4_MAD_U24(j, q0.w, t << 3) & 0xFFFFu;   // 8 (t - j nL) < 2^16 from the low halves
+    const u32 t8 = g8 - (q0.w >> 16);
+    const u32 j = mulhi_u24(t8 << 6, q0.z);
+    const u32 i8 = PS4_MAD_U24(j, q0.w, t8) & 0xFFFFu;   // 8 (t - j nL) < 2^16 from the low halves
     const u32 j8 = j << 3;
     return ExLists{ar_at(s_ar, (q0.x & 0xFFFFu) + i8), ar_at(s_ar, (q0.y & 0xFFFFu) + j8),
                    ar_at(s_ar, (q0.x >> 16) - i8), ar_at(s_ar, (q0.y >> 16) - j8)};
@@ -1239,7 +1249,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
     __shared__ uint4 s_mem[EX_NT];             // their e > 0 and member bounds
     __shared__ u32 s_aux[EX_NT];               // isqrt(2A) or 0
     __shared__ u32 s_qix[EX_NT];               // their queue indices
-    __shared__ uint4 s_bits4[EX_CMAX / 128];   // bit g: a 4-set's candidates start at g + 1
+    __shared__ uint4 s_bits4[EX_CMAX / 128 + 1];   // bit g: a 4-set's candidates start at g + 1
     __shared__ u64 s_q1[EX_Q1];                // candidates past the residue filters: u | record << 52
     u32 *s_bits = (u32 *)s_bits4;
 
@@ -1546,16 +1556,20 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                                          // the queued ones that fail the rest (per lane)
             // fitting 4-sets are a prefix of the pending ones: coff is each one's
             // first candidate in the pass
-            for (u32 w = tid; w < (ctot + 127) / 128 * 4; w += EX_NT) s_bits[w] = 0u;   // whole 128-bit words
+            // whole 128-bit words, and one more: the fetch past the pass's last
+            // iteration reads it
+            for (u32 w = tid; w < ((ctot + 127) / 128 + 1) * 4; w += EX_NT) s_bits[w] = 0u;
             __syncthreads();
             // The guard record after the pass's records: the candidates of the last
             // iteration past ctot (fewer than 128) fall to it, so the loop needs no
             // range tests.  Its masks are 0 (no candidate passes) and its lists are
             // the first 256 entries of the arena read as L = {x}, H of 256 entries
-            // (nL = 1: j = t, i = 0), so every read stays inside the arena.
+            // (nL = 1: j = t, i = 0); its first candidate is 128 before ctot (or
+            // 0), so t < 256 and every read stays inside the arena.
             if (tid == 0 && ctot > 0u) {
                 uint4 *pw = s_par + (u32)__popcll(fmask) * EX_PAR_WORDS;
-                pw[0] = make_uint4(2040u << 16, 2040u << 16, 0x800000u, ((0u - 8u) & 0xFFFFu) | (ctot << 16));
+                const u32 gc = ctot >= 128u ? ctot - 128u : 0u;
+                pw[0] = make_uint4(2040u << 16, 2040u << 16, 0x800000u, ((0u - 8u) & 0xFFFFu) | ((8u * gc) << 16));
                 pw[1] = make_uint4(0u, 0u, 0u, 0u);
                 pw[2] = make_uint4(0u, 0u, 0u, 0u);
                 atomicOr(&s_bits[(ctot - 1u) >> 5], 1u << ((ctot - 1u) & 31u));
@@ -1564,7 +1578,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     uint4 *pw = s_par + slot * EX_PAR_WORDS;
                     const u32 aL = 8u * loff, aH = 8u * (loff + nL);
                     pw[0] = make_uint4(aL | ((aL + 8u * (nL - 1u)) << 16), aH | ((aH + 8u * (lsize - nL - 1u)) << 16),
-                                       (0x800000u + nL - 1u) / nL, ((0u - 8u * nL) & 0xFFFFu) | (coff << 16));
+                                       (0x800000u + nL - 1u) / nL, ((0u - 8u * nL) & 0xFFFFu) | ((8u * coff) << 16));
                     pw[1] = make_uint4((u32)pm64, (u32)(pm64 >> 32), (u32)pm63, (u32)(pm63 >> 32));
                     pw[2] = make_uint4((u32)pm65, (u32)(pm65 >> 32), (u32)pmsm, (u32)(pmsm >> 32) | (best << 29));
                     s_mem[slot] = make_uint4(psqA, pf1, pf2, pf3);
@@ -1646,9 +1660,9 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 // bitmap words of base: every 4-set but the first marks the candidate
                 // before its first, and so does the guard record, so the record of
                 // candidate g is the number of marks below g (a ballot-style prefix
-                // count, v_mbcnt), the guard for g >= ctot; the index is clamped to
-                // the guard's slot for the fetch past the last iteration (bitmap
-                // words there are stale, and what it reads is not used)
+                // count, v_mbcnt), the guard for g >= ctot; the fetch past the last
+                // iteration reads a cleared word, so every record index is at most
+                // the guard's (the number of fitting 4-sets <= EX_NT)
                 u32 ka, kb, ia, ib;       // ia, ib: byte offsets of the records read (of ka, kb or 0)
                 uint4 A0, A1, A2, B0, B1, B2;   // their words
                 auto fetch = [&](u32 base, const uint4 &wv) {
@@ -1658,8 +1672,8 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     kb = __builtin_amdgcn_mbcnt_hi(wv.w, __builtin_amdgcn_mbcnt_lo(wv.z, kp));
                     kp = PS4_BCNT_ADD(wv.w, PS4_BCNT_ADD(wv.z, kp));
                     // byte offsets of the records (record indices are below 64)
-                    ia = PS4_MAD_U24(min(ka, (u32)EX_NT), EX_PAR_WORDS * 16u, 0u);
-                    ib = PS4_MAD_U24(min(kb, (u32)EX_NT), EX_PAR_WORDS * 16u, 0u);
+                    ia = PS4_MAD_U24(ka, EX_PAR_WORDS * 16u, 0u);
+                    ib = PS4_MAD_U24(kb, EX_PAR_WORDS * 16u, 0u);
                     A0 = *(const uint4 *)((const char *)s_par + ia);
                     B0 = *(const uint4 *)((const char *)s_par + ib);
                 };
@@ -1675,12 +1689,12 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                 fetch_masks();
                 uint4 wnext = s_bits4[1];
                 for (u32 base = 0; base < ctot; base += 128) {
-                    const u32 ga = base + tid, gb = ga + 64u;
+                    const u32 ga8 = 8u * (base + tid), gb8 = ga8 + 512u;
                     const u32 ka0 = ka, kb0 = kb;
-                    const ExLists LA = ex_lists(A0, s_ar, ga), LB = ex_lists(B0, s_ar, gb);
+                    const ExLists LA = ex_lists(A0, s_ar, ga8), LB = ex_lists(B0, s_ar, gb8);
                     __builtin_amdgcn_sched_barrier(0);   // both candidates' list reads issued before any use
                     fetch(base + 128u, wnext);
-                    wnext = s_bits4[min((base + 256u) >> 7, EX_CMAX / 128u - 1u)];
+                    wnext = s_bits4[min((base + 256u) >> 7, EX_CMAX / 128u)];
                     PS4_KEEP("v"(LA.l0), "v"(LA.h0), "v"(LA.l1), "v"(LA.h1), "v"(LB.l0), "v"(LB.h0), "v"(LB.l1),
                              "v"(LB.h1));   // both candidates' list entries (the next reads may still be in flight)
                     const ExCand A = ex_cand(A1, A2, LA, a.neg_m1);
@@ -1726,12 +1740,12 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     };
                     // (the stack holds < 64 entries before each push of at most 64)
                     const u64 boa = __ballot(oka);
-                    if (oka) s_q1[qn1 + lanes_below(boa)] = ex_qent(A.us, ka0);
+                    if (oka) *(u64 *)((char *)s_q1 + PS4_LSHL3_ADD(lanes_below(boa), 8u * qn1)) = ex_qent(A.us, ka0);
                     qn1 += (u32)__popcll(boa);
                     w_esq += (u32)__popcll(boa);
                     if (qn1 >= 64u) round1();
                     const u64 bob = __ballot(okb);
-                    if (okb) s_q1[qn1 + lanes_below(bob)] = ex_qent(B.us, kb0);
+                    if (okb) *(u64 *)((char *)s_q1 + PS4_LSHL3_ADD(lanes_below(bob), 8u * qn1)) = ex_qent(B.us, kb0);
                     qn1 += (u32)__popcll(bob);
                     w_esq += (u32)__popcll(bob);
                     while (qn1 >= 64u || (base + 128u >= ctot && qn1 > 0u))   // (the pass's records

@@ -31,6 +31,7 @@
 #define __align__(n) __attribute__((aligned(n)))
 #define PS4_KEEP(...) ((void)0)
 #define PS4_OPAQUE(x) ((void)0)
+#define PS4_LSHL3_ADD(a, b) (((uint32_t)(a) << 3) + (uint32_t)(b))
 #define PS4_BCNT_ADD(x, a) ((uint32_t)__builtin_popcount((uint32_t)(x)) + (uint32_t)(a))
 #define PS4_MAX_U16X2(a, b)                                                                       \
     ((std::max((uint32_t)(a) & 0xFFFFu, (uint32_t)(b) & 0xFFFFu)) | (std::max((uint32_t)(a) >> 16, (uint32_t)(b) >> 16) << 16))
