@@ -1711,27 +1711,25 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     auto round1 = [&]() {   // the top min(64, qn1) entries
                         __builtin_amdgcn_wave_barrier();
                         const u32 m = qn1 < 64u ? qn1 : 64u, b1 = qn1 - m;
-                        bool go = false;
-                        u64 ent = 0;
-                        u32 aux = 0;
-                        if (tid < m) {
-                            ent = s_q1[b1 + tid];
-                            const u32 k = ex_qk(ent);
-                            const double u = ex_qu(ent);
-                            aux = s_aux[k];
-                            if (ex_counted(u, s_mem[k], aux)) {
-                                const uint4 q2 = s_par[k * EX_PAR_WORDS + 2];
-                                go = ex_late(u, q2);
-                                aux = k | ((q2.w >> 29) << 6);   // record | pair choice << 6
-                            } else {
-                                ++my_unc;
-                            }
-                        }
+                        const bool on = tid < m;
+                        // the entry, then every read of its record at once (one wait
+                        // for all of them: the round is bound by these latencies);
+                        // lanes without an entry read record 0 and count nothing
+                        const u64 ent = s_q1[on ? b1 + tid : 0u];
+                        const u32 k = on ? ex_qk(ent) : 0u;
+                        const double u = ex_qu(ent);
+                        const u32 aux = s_aux[k];
+                        const uint4 mem = s_mem[k];
+                        const u64 q2h = *(const u64 *)((const char *)s_par + k * (EX_PAR_WORDS * 16u) + 40u);   // msm
+                        const u64 qi = s_qix[k];
+                        const bool counted = on && ex_counted(u, mem, aux);
+                        my_unc += (on && !counted) ? 1u : 0u;
+                        const bool go = counted && ex_late(u, make_uint4(0u, 0u, (u32)q2h, (u32)(q2h >> 32)));
                         const u64 bg = __ballot(go);
                         if (go) {
-                            const u64 qi = s_qix[aux & 63u];
+                            // record | pair choice << 6
                             exq2[qn2 + lanes_below(bg)] =
-                                make_uint4((u32)ent, ((u32)(ent >> 32) & 0xFFFFFu) | ((aux >> 6) << 20), (u32)qi,
+                                make_uint4((u32)ent, ((u32)(ent >> 32) & 0xFFFFFu) | ((u32)(q2h >> 61) << 20), (u32)qi,
                                            (u32)(qi >> 32));
                         }
                         qn2 += (u32)__popcll(bg);
