@@ -1671,7 +1671,7 @@ __global__ __launch_bounds__(EX_NT, 3) void k_ext(ExArgs a)   // 3 waves per SIM
                     kp = PS4_BCNT_ADD(wv.y, PS4_BCNT_ADD(wv.x, kp));
                     kb = __builtin_amdgcn_mbcnt_hi(wv.w, __builtin_amdgcn_mbcnt_lo(wv.z, kp));
                     kp = PS4_BCNT_ADD(wv.w, PS4_BCNT_ADD(wv.z, kp));
-                    // byte offsets of the records (record indices are below 64)
+                    // byte offsets of the records (record indices are at most 64, the guard's)
                     ia = PS4_MAD_U24(ka, EX_PAR_WORDS * 16u, 0u);
                     ib = PS4_MAD_U24(kb, EX_PAR_WORDS * 16u, 0u);
                     A0 = *(const uint4 *)((const char *)s_par + ia);
